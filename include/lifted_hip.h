@@ -1,0 +1,118 @@
+/* C ABI of lifted-hip: the lifting-scheme DWT of jcapriot/lifted on gfx950.
+ *
+ * The reference's boundary is four C++ overloads of lifted::lwt
+ * (src/lifted.hpp:13-39; defined in src/lifted-{float,double,sep-float,
+ * sep-double}.cpp:20-31) taking std::vector shapes/strides. A C ABI cannot pass
+ * std::vector, so each overload becomes one function taking (pointer, count)
+ * pairs with the same meaning; include/lifted.hpp rebuilds the exact C++
+ * overloads on top of these for C++ callers.
+ *
+ * Enum values are the reference's ABI (src/lifted-common.hpp:33-98):
+ *   wavelets 0..100907, BoundaryCondition 0..4, Transform 0..3.
+ * Strides are in elements and may be negative. in == out is allowed when both
+ * use the same strides. All functions return 0 on success and a negative
+ * LFX_E* code on error; lfx_last_error() then holds a message (per thread).
+ */
+#ifndef LIFTED_HIP_H_
+#define LIFTED_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LFX_OK 0
+#define LFX_EINVAL (-1)       /* invalid argument (shape, axes, enum, level) */
+#define LFX_EUNSUPPORTED (-2) /* valid but not supported by this build */
+#define LFX_EDEVICE (-3)      /* HIP runtime error */
+#define LFX_ENOMEM (-4)       /* device allocation failed */
+
+/* storage types of lfx_desc.storage; bf16/fp16 are computed in fp32 */
+#define LFX_F32 0
+#define LFX_F64 1
+#define LFX_BF16 2
+#define LFX_F16 3
+
+/* lfx_desc.flags */
+#define LFX_FLAG_PLAIN 1u /* one launch per level (no multi-level fusion) */
+
+typedef struct {
+  uint32_t wavelet;         /* lifted::Wavelet value, or an id from lfx_program_create */
+  uint32_t transform;       /* lifted::Transform value */
+  uint32_t bc;              /* lifted::BoundaryCondition value */
+  int32_t ndim;             /* 1..8 */
+  const int64_t* shape;     /* ndim sizes */
+  const int64_t* stride_in; /* ndim element strides of the input */
+  const int64_t* stride_out;/* ndim element strides of the output */
+  int32_t naxes;            /* number of transformed axes (1..ndim) */
+  const int64_t* axes;      /* axes in application order; negative counts from the end */
+  int32_t separable;        /* 1: levels[naxes] (separable), 0: levels[0] (level by level) */
+  const int64_t* levels;    /* 0 = max_level */
+  int32_t storage;          /* LFX_F32 / LFX_F64 / LFX_BF16 / LFX_F16 */
+  uint32_t flags;           /* LFX_FLAG_* */
+} lfx_desc;
+
+/* Transform device-resident arrays, stream-ordered and asynchronous.
+ * stream is a hipStream_t (NULL = default stream). Replaces the work done by
+ * lifting_transform / seperable_lifting_transform (src/lifted-drivers-inl.hpp:630-656). */
+int lfx_lwt(const lfx_desc* desc, const void* in_dev, void* out_dev, void* stream);
+
+/* Same on host arrays: copies to the current device, transforms, copies back,
+ * blocking -- the calling convention of lifted::lwt. */
+int lfx_lwt_host(const lfx_desc* desc, const void* in_host, void* out_host);
+
+/* The four lifted::lwt overloads (src/lifted.hpp:13-39), host pointers.
+ * level: scalar level (level-by-level path, src/lifted.hpp:13-25);
+ * levels[naxes]: separable path (src/lifted.hpp:27-39). n_threads is accepted
+ * for signature parity and ignored (the work runs on the GPU). */
+int lfx_lwt_f32(uint32_t wavelet, uint32_t transform, uint32_t bc, size_t ndim, const size_t* shape,
+                const ptrdiff_t* stride_in, const ptrdiff_t* stride_out, size_t naxes, const size_t* axes,
+                size_t level, const float* data_in, float* data_out, size_t n_threads);
+int lfx_lwt_f64(uint32_t wavelet, uint32_t transform, uint32_t bc, size_t ndim, const size_t* shape,
+                const ptrdiff_t* stride_in, const ptrdiff_t* stride_out, size_t naxes, const size_t* axes,
+                size_t level, const double* data_in, double* data_out, size_t n_threads);
+int lfx_lwt_sep_f32(uint32_t wavelet, uint32_t transform, uint32_t bc, size_t ndim, const size_t* shape,
+                    const ptrdiff_t* stride_in, const ptrdiff_t* stride_out, size_t naxes, const size_t* axes,
+                    const size_t* levels, const float* data_in, float* data_out, size_t n_threads);
+int lfx_lwt_sep_f64(uint32_t wavelet, uint32_t transform, uint32_t bc, size_t ndim, const size_t* shape,
+                    const ptrdiff_t* stride_in, const ptrdiff_t* stride_out, size_t naxes, const size_t* axes,
+                    const size_t* levels, const double* data_in, double* data_out, size_t n_threads);
+
+/* max_level(width(wavelet), n) of src/lifted-common.hpp:1099-1108.
+ * Returns a negative code for an unknown wavelet or a width-1 wavelet (the
+ * reference's loop never terminates there). */
+int64_t lfx_max_level(uint32_t wavelet, int64_t n);
+
+/* Filter width of a wavelet (the `width` member, src/lifted-wavelets.hpp). */
+int64_t lfx_wavelet_width(uint32_t wavelet);
+
+/* Caller-defined lifting scheme (the reference only has compile-time ones,
+ * src/lifted-common.hpp:966-1032). Coefficients are doubles, cast to the
+ * transform type like the reference casts its literals. */
+typedef struct {
+  int32_t kind;    /* 0 generic (update_s/update_d), 1 unit, 2 repeat, 3 scale */
+  int32_t target;  /* 0 update evens (s), 1 update odds (d) */
+  int32_t offset;
+  int32_t ntaps;   /* generic: taps; repeat: summed taps; unit: 1 */
+  int32_t sign;    /* unit: +1 add / -1 subtract */
+  int32_t pad;
+  double taps[16]; /* generic: taps; repeat: taps[0]; scale: taps[0]=s_mul, taps[1]=d_div */
+} lfx_step;
+
+/* Registers a step program; *wavelet_id receives an id usable as lfx_desc.wavelet. */
+int lfx_program_create(const lfx_step* steps, int32_t nsteps, uint32_t width, uint32_t* wavelet_id);
+
+/* Coefficients the GPU uses for (wavelet, transform, storage), in execution
+ * order, as doubles: per step [kind, y_is_d, adjoint, sub, off, N, nf, nb,
+ * fold_off, c0..c15] (25 values). Returns the step count. */
+int lfx_program_dump(uint32_t wavelet, uint32_t transform, int32_t storage, double* out, int32_t cap_steps);
+
+/* Message of the last error on this thread. */
+const char* lfx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LIFTED_HIP_H_ */

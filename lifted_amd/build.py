@@ -1,0 +1,60 @@
+"""Build lifted_amd/liblifted_hip.so for gfx950 with hipcc (in-tree).
+
+    python -m lifted_amd.build        (or lifted_amd.build.build())
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "liblifted_hip.so")
+OBJDIR = os.path.join(HERE, "_build")
+
+ARCH = os.environ.get("LFX_OFFLOAD_ARCH", "gfx950")
+COMMON = [
+    "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-parameter",
+]
+# device code: explicit fma builtins only (contraction off), no packed fp32 ops
+DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+SOURCES = [
+    ("lfx_kernels.hip", True),
+    ("lfx_api.cpp", False),
+]
+
+
+def _hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def build(verbose=False):
+    hipcc = _hipcc()
+    os.makedirs(OBJDIR, exist_ok=True)
+    objs = []
+    for src, device in SOURCES:
+        obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
+        cmd = [hipcc, "-c"] + COMMON
+        cmd += DEVICE if device else ["-x", "c++", "-D__HIP_PLATFORM_AMD__"]
+        cmd += [os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd))
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
+    tmp = OUT + ".tmp"
+    cmd = [hipcc, "-shared", "-o", tmp] + objs + [f"--offload-arch={ARCH}"]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print(OUT)

@@ -1,0 +1,296 @@
+// Device-side lifting steps for gfx950, applied to bands resident in LDS.
+//
+// Semantics follow src/lifted-steps-inl.hpp of the reference, restated in a
+// per-element *gather* form so every output element is produced by exactly one
+// lane with no atomics:
+//
+//  normal (Forward / Inverse), element i of band y:
+//     y[i] += sum_k c[k] * x[BC(i + o + k, nx)]        (terms with Zero-BC skipped)
+//   The reference splits i into a front loop [0, n1), an interior [n1, n2) and a
+//   back loop [n2, ny) (n1 = min(n_front, ny), n2 = nd - n_back; steps:409-410).
+//   When n1 > n2 (bands shorter than the stencil) elements in [n2, n1) are
+//   updated by both the front and the back loop; the gather form reproduces that.
+//
+//  adjoint (ForwardAdjoint / InverseAdjoint), element j of band y (the forward
+//  step's x band):
+//     y[j] += front folds, then the zero-BC transposed stencil
+//             sum_k c_r[k] * x[j + o_r + k], then back folds,
+//   where a fold is the transposed contribution of a virtual index m outside the
+//   band that the BC maps onto j (steps:416-432, 466-484). Folds are applied in
+//   increasing m, exactly in the reference's order.
+//
+// Accumulation: y = fma(c[k], x[k], y) for k = 0..N-1 (steps:97-102); Repeat
+// sums its taps first and applies one fma (steps:116-122); Unit is y +/- x.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "lfx_program.hpp"
+
+namespace lfx {
+namespace dev {
+
+template <class C>
+__device__ __forceinline__ C fmac(C a, C b, C c);
+template <>
+__device__ __forceinline__ float fmac<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <>
+__device__ __forceinline__ double fmac<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// Boundary map of the reference (src/lifted-common.hpp:719-779) on one band of
+// length n: returns the mapped index, or -1 when the term is skipped (Zero BC).
+// Periodic keeps the reference's unsigned modulo for negative indices.
+__device__ __forceinline__ long long bc_map(int bc, long long m, long long n) {
+  if (m >= 0 && m < n) return m;
+  switch (bc) {
+    case kZero:
+      return -1;
+    case kConstant:
+      return m < 0 ? 0 : n - 1;
+    case kPeriodic:
+      return (long long)((unsigned long long)m % (unsigned long long)n);
+    case kSymmetric: {
+      long long io = m;
+      while (io >= n || io < 0) io = io < 0 ? -(io + 1) : 2 * (n - 1) - (io - 1);
+      return io;
+    }
+    default: {
+      long long io = m;
+      while (io >= n || io < 0) io = io < 0 ? -io : 2 * (n - 1) - io;
+      return io;
+    }
+  }
+}
+
+// A band held in LDS: local index i <-> global band index g0 + i.
+template <class C>
+struct Band {
+  C* p;
+  long long g0;
+  int len;
+};
+
+// Sum of the taps a Generic step reads at local x index xi (all in the window).
+template <class C, int NT>
+__device__ __forceinline__ C gen_interior(const C* __restrict__ x, int xi, const C* __restrict__ c, C acc) {
+#pragma unroll
+  for (int k = 0; k < NT; ++k) acc = fmac<C>(c[k], x[xi + k], acc);
+  return acc;
+}
+template <class C, int NT>
+__device__ __forceinline__ C rep_interior(const C* __restrict__ x, int xi, C v, C acc) {
+  C t = x[xi];
+#pragma unroll
+  for (int k = 1; k < NT; ++k) t += x[xi + k];
+  return fmac<C>(v, t, acc);
+}
+
+template <class C>
+__device__ __forceinline__ C interior(const DStep& st, const C* __restrict__ cf, const C* __restrict__ x, int xi,
+                                      C acc) {
+  if (st.kind == kUnit) return st.sub ? acc - x[xi] : acc + x[xi];
+  if (st.kind == kRepeat) {
+    const C v = cf[0];
+    switch (st.N) {
+      case 1: return rep_interior<C, 1>(x, xi, v, acc);
+      case 2: return rep_interior<C, 2>(x, xi, v, acc);
+      case 3: return rep_interior<C, 3>(x, xi, v, acc);
+      case 4: return rep_interior<C, 4>(x, xi, v, acc);
+      default: {
+        C t = x[xi];
+        for (int k = 1; k < st.N; ++k) t += x[xi + k];
+        return fmac<C>(v, t, acc);
+      }
+    }
+  }
+  switch (st.N) {
+    case 1: return gen_interior<C, 1>(x, xi, cf, acc);
+    case 2: return gen_interior<C, 2>(x, xi, cf, acc);
+    case 3: return gen_interior<C, 3>(x, xi, cf, acc);
+    case 4: return gen_interior<C, 4>(x, xi, cf, acc);
+    case 5: return gen_interior<C, 5>(x, xi, cf, acc);
+    case 6: return gen_interior<C, 6>(x, xi, cf, acc);
+    case 7: return gen_interior<C, 7>(x, xi, cf, acc);
+    case 8: return gen_interior<C, 8>(x, xi, cf, acc);
+    case 9: return gen_interior<C, 9>(x, xi, cf, acc);
+    default:
+      for (int k = 0; k < st.N; ++k) acc = fmac<C>(cf[k], x[xi + k], acc);
+      return acc;
+  }
+}
+
+// Stencil at global x index base+k with the boundary map `bc` (normal steps) or
+// only the in-range terms (bc = kZero: adjoint main loop and folds).
+template <class C>
+__device__ __forceinline__ C bc_stencil(const DStep& st, const C* __restrict__ cf, int bc, const Band<C>& x,
+                                        long long nx, long long base, C acc) {
+  if (st.kind == kUnit) {
+    const long long mi = bc_map(bc, base, nx);
+    if (mi >= 0) {
+      const C v = x.p[mi - x.g0];
+      acc = st.sub ? acc - v : acc + v;
+    }
+    return acc;
+  }
+  if (st.kind == kRepeat) {
+    long long mi = bc_map(bc, base, nx);
+    C t = mi >= 0 ? x.p[mi - x.g0] : C(0);
+    for (int k = 1; k < st.N; ++k) {
+      mi = bc_map(bc, base + k, nx);
+      if (mi >= 0) t += x.p[mi - x.g0];
+    }
+    return fmac<C>(cf[0], t, acc);
+  }
+  for (int k = 0; k < st.N; ++k) {
+    const long long mi = bc_map(bc, base + k, nx);
+    if (mi >= 0) acc = fmac<C>(cf[k], x.p[mi - x.g0], acc);
+  }
+  return acc;
+}
+
+// Uniform scan of the fold sources of an adjoint step: virtual indices
+// m in [m0, m1) whose BC image lies inside the band and differs from m.
+// Returns the [lo, hi] range of their images (lo > hi when there are none).
+__device__ __forceinline__ void fold_range(int bc, long long m0, long long m1, long long ny, long long& lo,
+                                           long long& hi) {
+  lo = 1;
+  hi = 0;
+  if (bc == kZero || ny <= 0) return;
+  for (long long m = m0; m < m1; ++m) {
+    const long long io = bc_map(bc, m, ny);
+    if (io >= 0 && io != m) {
+      if (hi < lo) { lo = io; hi = io; }
+      else { lo = io < lo ? io : lo; hi = io > hi ? io : hi; }
+    }
+  }
+}
+
+template <class C>
+__device__ __forceinline__ C apply_folds(const DStep& st, const C* __restrict__ cf, int bc, const Band<C>& x,
+                                         long long nx, long long ny, long long m0, long long m1, long long j,
+                                         C acc) {
+  for (long long m = m0; m < m1; ++m) {
+    const long long io = bc_map(bc, m, ny);
+    if (io == j && io != m) acc = bc_stencil<C>(st, cf, kZero, x, nx, m + st.off, acc);
+  }
+  return acc;
+}
+
+// Apply one lifting step (not Scale) to the s/d bands held in LDS. ns/nd are
+// the global band lengths of this level. Only elements whose global index lies
+// in [ylo, yhi) are updated (the whole window when called with the full band).
+// Caller synchronizes afterwards.
+template <class C>
+__device__ void lift_step(const DStep& st, const C* __restrict__ cf, int bc, const Band<C>& s, const Band<C>& d,
+                          long long ns, long long nd, long long ylo, long long yhi, int tid, int nthr) {
+  const Band<C>& y = st.y_is_d ? d : s;
+  const Band<C>& x = st.y_is_d ? s : d;
+  const long long ny = st.y_is_d ? nd : ns;
+  const long long nx = st.y_is_d ? ns : nd;
+  const long long n1 = (long long)st.nf < ny ? (long long)st.nf : ny;
+  const long long n2 = nd > (long long)st.nb ? nd - (long long)st.nb : 0;
+  const long long xshift = y.g0 - x.g0 + st.off;  // local x index of tap 0 = i + xshift
+  const long long i0 = ylo - y.g0, i1 = yhi - y.g0;
+
+  if (!st.adjoint) {
+    for (long long i = i0 + tid; i < i1; i += nthr) {
+      const long long g = y.g0 + i;
+      C acc = y.p[i];
+      if (g >= n1 && g < n2) {
+        acc = interior<C>(st, cf, x.p, (int)(i + xshift), acc);
+      } else {
+        if (g < n1) acc = bc_stencil<C>(st, cf, bc, x, nx, g + st.off, acc);
+        if (g >= n2) acc = bc_stencil<C>(st, cf, bc, x, nx, g + st.off, acc);
+      }
+      y.p[i] = acc;
+    }
+    return;
+  }
+  // adjoint: fold sources. Front: m in [o, 0) (unit steps also stop at m = o + nx,
+  // steps:697); back: m in [nd, nx - o_r).
+  const long long f0 = st.fold_off;
+  long long f1 = 0;
+  if (st.kind == kUnit && f0 + nx < f1) f1 = f0 + nx;
+  const long long b0 = nd, b1 = nx - st.off;
+  long long flo, fhi, blo, bhi;
+  fold_range(bc, f0, f1, ny, flo, fhi);
+  fold_range(bc, b0, b1, ny, blo, bhi);
+  for (long long i = i0 + tid; i < i1; i += nthr) {
+    const long long g = y.g0 + i;
+    C acc = y.p[i];
+    if (g >= flo && g <= fhi) acc = apply_folds<C>(st, cf, bc, x, nx, ny, f0, f1, g, acc);
+    if (g >= n1 && g < n2) {
+      acc = interior<C>(st, cf, x.p, (int)(i + xshift), acc);
+    } else {
+      if (g < n1) acc = bc_stencil<C>(st, cf, kZero, x, nx, g + st.off, acc);
+      if (g >= n2) acc = bc_stencil<C>(st, cf, kZero, x, nx, g + st.off, acc);
+    }
+    if (g >= blo && g <= bhi) acc = apply_folds<C>(st, cf, bc, x, nx, ny, b0, b1, g, acc);
+    y.p[i] = acc;
+  }
+}
+
+template <class C>
+__device__ __forceinline__ void scale_step(const C* __restrict__ cf, const Band<C>& s, const Band<C>& d, int tid,
+                                           int nthr) {
+  const C sm = cf[0], dm = cf[1];
+  for (int i = tid; i < s.len; i += nthr) s.p[i] = s.p[i] * sm;
+  for (int i = tid; i < d.len; i += nthr) d.p[i] = d.p[i] * dm;
+}
+
+// All steps of one level on full bands held in LDS.
+template <class C>
+__device__ void level_steps(const DProgram<C>* __restrict__ prog, int bc, const Band<C>& s, const Band<C>& d,
+                            long long ns, long long nd, int tid, int nthr) {
+  const int nsteps = prog->nsteps;
+  for (int k = 0; k < nsteps; ++k) {
+    const DStep st = prog->steps[k];
+    const C* cf = prog->coef + st.coef;
+    if (st.kind == kScale) {
+      scale_step<C>(cf, s, d, tid, nthr);
+    } else {
+      const long long ylo = st.y_is_d ? d.g0 : s.g0;
+      const long long yhi = ylo + (st.y_is_d ? d.len : s.len);
+      lift_step<C>(st, cf, bc, s, d, ns, nd, ylo, yhi, tid, nthr);
+    }
+    __syncthreads();
+  }
+}
+
+// ---- storage conversions (bf16 / fp16 storage, fp32 arithmetic) ----
+struct bf16_t { uint16_t v; };
+struct f16_t { uint16_t v; };
+
+template <class S, class C>
+__device__ __forceinline__ C load_as(const S* p, long long i) { return (C)p[i]; }
+template <>
+__device__ __forceinline__ float load_as<bf16_t, float>(const bf16_t* p, long long i) {
+  return __uint_as_float(((uint32_t)p[i].v) << 16);
+}
+template <>
+__device__ __forceinline__ float load_as<f16_t, float>(const f16_t* p, long long i) {
+  _Float16 h;
+  __builtin_memcpy(&h, &p[i].v, 2);
+  return (float)h;
+}
+
+template <class S, class C>
+__device__ __forceinline__ void store_as(S* p, long long i, C v) { p[i] = (S)v; }
+template <>
+__device__ __forceinline__ void store_as<bf16_t, float>(bf16_t* p, long long i, float v) {
+  uint32_t u = __float_as_uint(v);
+  if ((u & 0x7fffffffu) > 0x7f800000u) {
+    p[i].v = (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
+  } else {
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    p[i].v = (uint16_t)(u >> 16);
+  }
+}
+template <>
+__device__ __forceinline__ void store_as<f16_t, float>(f16_t* p, long long i, float v) {
+  _Float16 h = (_Float16)v;
+  __builtin_memcpy(&p[i].v, &h, 2);
+}
+
+}  // namespace dev
+}  // namespace lfx

@@ -1,0 +1,695 @@
+// CPU restatement of the reference lifting-scheme DWT -- TEST INFRASTRUCTURE ONLY.
+//
+// This file is the correctness oracle and the CPU baseline of the benchmark.
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it;
+// the product (lifted_amd/) never links, imports or calls it.
+//
+// It restates, loop for loop, the reference's scalar ("Along") code path:
+//   * boundary maps            src/lifted-common.hpp:719-779   (Periodic keeps the
+//                              unsigned-modulo behaviour for negative indices)
+//   * step descriptors          src/lifted-common.hpp:798-964  (offset, n_front,
+//                              n_back, reversed offset/taps, scale reciprocals in
+//                              long double)
+//   * Generic/Unit/Repeat/Scale src/lifted-steps-inl.hpp:376-1204, including the
+//                              adjoint "fold" loops (:416-432, :466-484) and the
+//                              front/interior/back split with n1/n2 (:409-410)
+//   * step order                src/lifted-drivers-inl.hpp:37-64
+//   * multi-level line drivers  src/lifted-drivers-inl.hpp:66-312
+//   * ND separable / level-by-level drivers src/lifted-drivers-inl.hpp:417-610
+//   * max_level                 src/lifted-common.hpp:1099-1115
+// Accumulation order follows the vector path: y = fma(c[k], x[k], y) for
+// k = 0..N-1 (steps-inl.hpp:97-102), Repeat sums the taps first and then applies
+// one fma (:116-122); scalar boundary terms use the same fma form.
+//
+// Deliberate differences from the reference (documented, never parity cases):
+//   * unknown wavelet / transform / BC ids are errors (reference: silent Lazy / UB);
+//   * level 0 on a width-1 wavelet is an error (reference never terminates);
+//   * bands of length 0 (or 1 under Reflect) with a non-Zero BC are errors
+//     (reference divides by zero or loops forever);
+//   * the level-0 "max level" over several axes takes the minimum over the named
+//     axes (reference indexes axes[] with the dimension counter).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace ora {
+
+enum Bc { ZERO = 0, CONSTANT = 1, PERIODIC = 2, SYMMETRIC = 3, REFLECT = 4 };
+enum Tf { FORWARD = 0, INVERSE = 1, FORWARD_ADJOINT = 2, INVERSE_ADJOINT = 3 };
+enum Kind { GENERIC = 0, UNIT = 1, REPEAT = 2, SCALE = 3 };
+
+static thread_local std::string g_err;
+
+// ---- boundary maps (lifted-common.hpp:719-779) ----
+static inline size_t bc_index(int bc, ptrdiff_t i, size_t n) {
+  switch (bc) {
+    case ZERO:
+      return (i < 0 || i >= ptrdiff_t(n)) ? n : size_t(i);
+    case CONSTANT:
+      if (i < 0) return 0;
+      if (i >= ptrdiff_t(n)) return n - 1;
+      return size_t(i);
+    case PERIODIC:
+      return size_t(i) % n;  // unsigned modulo, as in the reference
+    case SYMMETRIC: {
+      ptrdiff_t io = i;
+      while (io >= ptrdiff_t(n) || io < 0) io = (io < 0) ? -(io + 1) : 2 * (ptrdiff_t(n) - 1) - (io - 1);
+      return size_t(io);
+    }
+    case REFLECT: {
+      ptrdiff_t io = i;
+      while (io >= ptrdiff_t(n) || io < 0) io = (io < 0) ? -io : 2 * (ptrdiff_t(n) - 1) - io;
+      return size_t(io);
+    }
+  }
+  throw std::invalid_argument("Unknown Boundary Condition");
+}
+
+// ---- step definitions in long double, as written in the wavelet table ----
+struct StepL {
+  int kind, target /*0: update evens (s), 1: update odds (d)*/, offset, ntaps, sign;
+  long double taps[16];
+};
+struct WaveletL {
+  uint32_t id = 0;
+  uint32_t width = 0;
+  std::vector<StepL> steps;
+};
+
+static std::vector<WaveletL> build_table() {
+  std::vector<WaveletL> v;
+  WaveletL* cur = nullptr;
+  auto add = [&](int kind, int target, int off, int n, int sign, const long double* t, int nt) {
+    StepL s{};
+    s.kind = kind; s.target = target; s.offset = off; s.ntaps = n; s.sign = sign;
+    for (int k = 0; k < nt; ++k) s.taps[k] = t[k];
+    cur->steps.push_back(s);
+  };
+  const int S = 0, D = 1;
+#define LFX_WAVELET(nm, idv, w) v.emplace_back(); cur = &v.back(); cur->id = idv; cur->width = w;
+#define LFX_END(nm)
+#define LFX_GENERIC(b, off, ...) { const long double t_[] = {__VA_ARGS__}; \
+    add(GENERIC, b, off, int(sizeof(t_) / sizeof(t_[0])), 1, t_, int(sizeof(t_) / sizeof(t_[0]))); }
+#define LFX_UNIT(b, off, sg) add(UNIT, b, off, 1, sg, nullptr, 0);
+#define LFX_REPEAT(b, off, n, val) { const long double t_[] = {val}; add(REPEAT, b, off, n, 1, t_, 1); }
+#define LFX_SCALE(a, b) { const long double t_[] = {a, b}; add(SCALE, 0, 0, 0, 1, t_, 2); }
+#include "../lifted_amd/csrc/wavelet_table.inc"
+#undef LFX_WAVELET
+#undef LFX_END
+#undef LFX_GENERIC
+#undef LFX_UNIT
+#undef LFX_REPEAT
+#undef LFX_SCALE
+  return v;
+}
+
+static const std::vector<WaveletL>& table() {
+  static const std::vector<WaveletL> t = build_table();
+  return t;
+}
+
+// ---- per-type step, mirroring OffsetData / *UpdateStep / ScaleStep ----
+template <class T>
+struct Step {
+  int kind = 0;
+  bool upd_odds = false;
+  size_t N = 1;
+  int sign = 1;  // unit step: declared +/-
+  ptrdiff_t offset = 0, offset_r = 0;
+  size_t n_front = 0, n_back = 0, n_front_r = 0, n_back_r = 0;
+  T vals[16] = {}, vals_r[16] = {};
+  T val = 0;
+  T s_mul = 1, s_div = 1, d_div = 1, d_mul = 1;
+};
+
+template <class T>
+static Step<T> make_step(const StepL& d) {
+  Step<T> s;
+  s.kind = d.kind;
+  if (d.kind == SCALE) {
+    s.s_mul = static_cast<T>(d.taps[0]);
+    s.s_div = static_cast<T>(1.0L / d.taps[0]);
+    s.d_div = static_cast<T>(d.taps[1]);
+    s.d_mul = static_cast<T>(1.0L / d.taps[1]);
+    return s;
+  }
+  s.upd_odds = d.target == 1;
+  s.N = size_t(d.ntaps);
+  s.sign = d.sign;
+  s.offset = d.offset;
+  const ptrdiff_t maxo = ptrdiff_t(s.N) - 1 + s.offset;
+  s.n_front = s.offset < 0 ? size_t(-s.offset) : 0;
+  s.n_back = maxo < 0 ? 0 : size_t(maxo);
+  s.offset_r = -maxo;
+  const ptrdiff_t maxo_r = ptrdiff_t(s.N) - 1 + s.offset_r;
+  s.n_front_r = s.offset_r < 0 ? size_t(-s.offset_r) : 0;
+  s.n_back_r = maxo_r < 0 ? 0 : size_t(maxo_r);
+  if (d.kind == GENERIC) {
+    for (size_t k = 0; k < s.N; ++k) s.vals[k] = static_cast<T>(d.taps[k]);
+    for (size_t k = 0; k < s.N; ++k) s.vals_r[k] = s.vals[s.N - 1 - k];
+  } else if (d.kind == REPEAT) {
+    s.val = static_cast<T>(d.taps[0]);
+  }
+  return s;
+}
+
+static inline bool is_normal(int tf) { return tf == FORWARD || tf == INVERSE; }
+static inline bool is_forward_tf(int tf) { return tf == FORWARD || tf == FORWARD_ADJOINT; }
+static inline bool forward_stepping(int tf) { return tf == FORWARD || tf == INVERSE_ADJOINT; }
+
+// y op= c*x with op = + or -, fused like the reference's MulAdd/NegMulAdd.
+template <class T>
+static inline void fop(bool sub, T& y, T c, T x) { y = std::fma(sub ? -c : c, x, y); }
+
+// inner_product with a boundary map (steps-inl.hpp:205-248)
+template <class T>
+static inline void ip_bc(const Step<T>& st, bool sub, int bc, T& y, const T* vs, ptrdiff_t off,
+                         const T* x, ptrdiff_t i, size_t n) {
+  const ptrdiff_t io = i + off;
+  if (st.kind == GENERIC) {
+    for (size_t k = 0; k < st.N; ++k) {
+      size_t ib = bc_index(bc, io + ptrdiff_t(k), n);
+      if (ib < n) fop(sub, y, vs[k], x[ib]);
+    }
+  } else {  // REPEAT
+    size_t ib = bc_index(bc, io, n);
+    T x0 = (ib < n) ? x[ib] : T(0);
+    for (size_t k = 1; k < st.N; ++k) {
+      ib = bc_index(bc, io + ptrdiff_t(k), n);
+      if (ib < n) x0 += x[ib];
+    }
+    fop(sub, y, st.val, x0);
+  }
+}
+
+// inner_product_safe (steps-inl.hpp:310-339): only in-range terms
+template <class T>
+static inline void ip_safe(const Step<T>& st, bool sub, T& y, const T* vs, const T* x, ptrdiff_t i,
+                           size_t n) {
+  if (st.kind == GENERIC) {
+    for (size_t k = 0; k < st.N; ++k) {
+      ptrdiff_t ii = i + ptrdiff_t(k);
+      if (ii >= 0 && ii < ptrdiff_t(n)) fop(sub, y, vs[k], x[ii]);
+    }
+  } else {
+    T x0 = (i >= 0 && i < ptrdiff_t(n)) ? x[i] : T(0);
+    for (size_t k = 1; k < st.N; ++k) {
+      ptrdiff_t ii = i + ptrdiff_t(k);
+      if (ii >= 0 && ii < ptrdiff_t(n)) x0 += x[ii];
+    }
+    fop(sub, y, st.val, x0);
+  }
+}
+
+// Interior element, all taps in range (steps-inl.hpp:85-122)
+template <class T>
+static inline void ip_interior(const Step<T>& st, bool sub, T& y, const T* vs, ptrdiff_t off,
+                               const T* x, ptrdiff_t i) {
+  const ptrdiff_t io = i + off;
+  if (st.kind == GENERIC) {
+    for (size_t k = 0; k < st.N; ++k) fop(sub, y, vs[k], x[io + ptrdiff_t(k)]);
+  } else {
+    T x0 = x[io];
+    for (size_t k = 1; k < st.N; ++k) x0 += x[io + ptrdiff_t(k)];
+    fop(sub, y, st.val, x0);
+  }
+}
+
+// Step<Generic/Repeat>::operator()(TF, Along, ...) -- steps-inl.hpp:384-497, 897-1010
+template <class T>
+static void apply_update(const Step<T>& st, int tf, int bc, T* s, T* d, size_t ns, size_t nd) {
+  const bool normal = is_normal(tf);
+  const bool sub = !is_forward_tf(tf);
+  const bool tgt_odds = normal;  // target_update_type
+  const size_t nf = normal ? st.n_front : st.n_front_r;
+  const size_t nb = normal ? st.n_back : st.n_back_r;
+  const ptrdiff_t off = normal ? st.offset : st.offset_r;
+  const T* vs = normal ? st.vals : st.vals_r;
+  const bool xs = (st.upd_odds == tgt_odds);
+  const size_t nx = xs ? ns : nd;
+  const T* x = xs ? s : d;
+  const size_t ny = xs ? nd : ns;
+  T* y = xs ? d : s;
+  const size_t n1 = std::min(nf, ny);
+  const size_t n2 = (nd > nb) ? nd - nb : 0;
+  const ptrdiff_t N = ptrdiff_t(st.N);
+
+  if (!normal) {
+    for (ptrdiff_t i = st.offset, j = 1 - N; i < 0; ++i, ++j) {
+      size_t io = bc_index(bc, i, ny);
+      if (io != ny && ptrdiff_t(io) != i) ip_safe(st, sub, y[io], vs, x, j, nx);
+    }
+    for (ptrdiff_t i = 0; i < ptrdiff_t(n1); ++i) ip_bc(st, sub, ZERO, y[i], vs, off, x, i, nx);
+  } else {
+    for (ptrdiff_t i = 0; i < ptrdiff_t(n1); ++i) ip_bc(st, sub, bc, y[i], vs, off, x, i, nx);
+  }
+  for (ptrdiff_t i = ptrdiff_t(n1); i < ptrdiff_t(n2); ++i) ip_interior(st, sub, y[i], vs, off, x, i);
+  if (!normal) {
+    for (ptrdiff_t i = ptrdiff_t(n2); i < ptrdiff_t(ny); ++i) ip_bc(st, sub, ZERO, y[i], vs, off, x, i, nx);
+    for (ptrdiff_t i = ptrdiff_t(nd), j = ptrdiff_t(nd) + off; i < ptrdiff_t(nx) - off; ++i, ++j) {
+      size_t io = bc_index(bc, i, ny);
+      if (io != ny && ptrdiff_t(io) != i) ip_safe(st, sub, y[io], vs, x, j, nx);
+    }
+  } else {
+    for (ptrdiff_t i = ptrdiff_t(n2); i < ptrdiff_t(ny); ++i) ip_bc(st, sub, bc, y[i], vs, off, x, i, nx);
+  }
+}
+
+// Step<UnitUpdateStep>::operator()(TF, Along, ...) -- steps-inl.hpp:629-750
+template <class T>
+static void apply_unit(const Step<T>& st, int tf, int bc, T* s, T* d, size_t ns, size_t nd) {
+  const bool normal = is_normal(tf);
+  // unit_op = declared sign for Forward/ForwardAdjoint, reversed for Inverse/InverseAdjoint
+  const bool add = (st.sign > 0) == is_forward_tf(tf);
+  const bool tgt_odds = normal;
+  const size_t nf = normal ? st.n_front : st.n_front_r;
+  const size_t nb = normal ? st.n_back : st.n_back_r;
+  const ptrdiff_t off = normal ? st.offset : st.offset_r;
+  const bool xs = (st.upd_odds == tgt_odds);
+  const size_t nx = xs ? ns : nd;
+  const T* x = xs ? s : d;
+  const size_t ny = xs ? nd : ns;
+  T* y = xs ? d : s;
+  const size_t n1 = std::min(nf, ny);
+  const size_t n2 = (nd > nb) ? nd - nb : 0;
+  auto iop = [&](T& a, T b) { a = add ? a + b : a - b; };
+
+  if (!normal) {
+    for (ptrdiff_t i = st.offset, j = 0; i < 0 && j < ptrdiff_t(nx); ++i, ++j) {
+      size_t io = bc_index(bc, i, ny);
+      if (io != ny && ptrdiff_t(io) != i) iop(y[io], x[j]);
+    }
+    for (ptrdiff_t i = 0; i < ptrdiff_t(n1); ++i) {
+      size_t ib = bc_index(ZERO, i + off, nx);
+      if (ib < nx) iop(y[i], x[ib]);
+    }
+  } else {
+    for (ptrdiff_t i = 0; i < ptrdiff_t(n1); ++i) {
+      size_t ib = bc_index(bc, i + off, nx);
+      if (ib < nx) iop(y[i], x[ib]);
+    }
+  }
+  for (ptrdiff_t i = ptrdiff_t(n1); i < ptrdiff_t(n2); ++i) iop(y[i], x[i + off]);
+  if (!normal) {
+    for (ptrdiff_t i = ptrdiff_t(n2); i < ptrdiff_t(ny); ++i) {
+      size_t ib = bc_index(ZERO, i + off, nx);
+      if (ib < nx) iop(y[i], x[ib]);
+    }
+    for (ptrdiff_t i = ptrdiff_t(nd), j = ptrdiff_t(nd) + off; i < ptrdiff_t(nx) - off; ++i, ++j) {
+      size_t io = bc_index(bc, i, ny);
+      // the reference reads x[j] unchecked; j < 0 only happens for bands
+      // shorter than the step offset, which this restatement skips
+      if (io != ny && ptrdiff_t(io) != i && j >= 0) iop(y[io], x[j]);
+    }
+  } else {
+    for (ptrdiff_t i = ptrdiff_t(n2); i < ptrdiff_t(ny); ++i) {
+      size_t ib = bc_index(bc, i + off, nx);
+      if (ib < nx) iop(y[i], x[ib]);
+    }
+  }
+}
+
+template <class T>
+static void apply_scale(const Step<T>& st, int tf, T* s, T* d, size_t ns, size_t nd) {
+  const bool fwd = is_forward_tf(tf);
+  const T dm = fwd ? st.d_mul : st.d_div;
+  const T sm = fwd ? st.s_mul : st.s_div;
+  for (size_t i = 0; i < nd; ++i) d[i] = d[i] * dm;
+  for (size_t i = 0; i < ns; ++i) s[i] = s[i] * sm;
+}
+
+// FixedTransform::apply (drivers-inl.hpp:51-63)
+template <class T>
+static void fixed_transform(const std::vector<Step<T>>& steps, int tf, int bc, T* s, T* d, size_t ns,
+                            size_t nd) {
+  const size_t n = steps.size();
+  for (size_t k = 0; k < n; ++k) {
+    const Step<T>& st = steps[forward_stepping(tf) ? k : n - 1 - k];
+    if (st.kind == SCALE) apply_scale(st, tf, s, d, ns, nd);
+    else if (st.kind == UNIT) apply_unit(st, tf, bc, s, d, ns, nd);
+    else apply_update(st, tf, bc, s, d, ns, nd);
+  }
+}
+
+// ---- line drivers (drivers-inl.hpp:75-311), operating on a contiguous copy ----
+template <class T>
+struct LineWork {
+  std::vector<T> a, b, c;
+};
+
+// Forward-stepping multi-level: in (length n) -> out [s_L | d_L | ... | d_1]
+template <class T>
+static void line_fwd_multi(const std::vector<Step<T>>& st, int tf, int bc, const T* in, T* out,
+                           size_t n, size_t L, LineWork<T>& w) {
+  if (L == 0) return;
+  w.a.assign(in, in + n);
+  size_t m = n;
+  for (size_t l = 0; l < L; ++l) {
+    const size_t nd = m / 2, ns = m - nd;
+    w.b.resize(m);
+    for (size_t i = 0; i < nd; ++i) { w.b[i] = w.a[2 * i]; w.b[ns + i] = w.a[2 * i + 1]; }
+    if (ns > nd) w.b[ns - 1] = w.a[m - 1];
+    fixed_transform(st, tf, bc, w.b.data(), w.b.data() + ns, ns, nd);
+    for (size_t i = 0; i < nd; ++i) out[ns + i] = w.b[ns + i];
+    w.a.assign(w.b.begin(), w.b.begin() + ns);
+    m = ns;
+  }
+  for (size_t i = 0; i < m; ++i) out[i] = w.a[i];
+}
+
+// Reverse-stepping multi-level: in [s_L | d_L | ... | d_1] -> out (length n)
+template <class T>
+static void line_rev_multi(const std::vector<Step<T>>& st, int tf, int bc, const T* in, T* out,
+                           size_t n, size_t L, LineWork<T>& w) {
+  if (L == 0) return;
+  std::vector<size_t> nss(L), nds(L);
+  size_t m = n;
+  for (size_t l = 0; l < L; ++l) { nds[l] = m / 2; nss[l] = m - nds[l]; m = nss[l]; }
+  w.a.assign(in, in + nss[L - 1]);
+  for (size_t l = L; l-- > 0;) {
+    const size_t ns = nss[l], nd = nds[l];
+    w.b.resize(ns + nd);
+    std::copy(w.a.begin(), w.a.begin() + ns, w.b.begin());
+    for (size_t i = 0; i < nd; ++i) w.b[ns + i] = in[ns + i];
+    fixed_transform(st, tf, bc, w.b.data(), w.b.data() + ns, ns, nd);
+    w.a.resize(ns + nd);
+    for (size_t i = 0; i < nd; ++i) { w.a[2 * i] = w.b[i]; w.a[2 * i + 1] = w.b[ns + i]; }
+    if (ns > nd) w.a[ns + nd - 1] = w.b[ns - 1];
+  }
+  for (size_t i = 0; i < n; ++i) out[i] = w.a[i];
+}
+
+// single level (drivers-inl.hpp:144-179, 276-311)
+template <class T>
+static void line_single(const std::vector<Step<T>>& st, int tf, int bc, const T* in, T* out, size_t n,
+                        LineWork<T>& w) {
+  const size_t nd = n / 2, ns = n - nd;
+  w.b.resize(n);
+  if (forward_stepping(tf)) {
+    for (size_t i = 0; i < nd; ++i) { w.b[i] = in[2 * i]; w.b[ns + i] = in[2 * i + 1]; }
+    if (ns > nd) w.b[ns - 1] = in[n - 1];
+    fixed_transform(st, tf, bc, w.b.data(), w.b.data() + ns, ns, nd);
+    for (size_t i = 0; i < n; ++i) out[i] = w.b[i];
+  } else {
+    for (size_t i = 0; i < n; ++i) w.b[i] = in[i];
+    fixed_transform(st, tf, bc, w.b.data(), w.b.data() + ns, ns, nd);
+    for (size_t i = 0; i < nd; ++i) { out[2 * i] = w.b[i]; out[2 * i + 1] = w.b[ns + i]; }
+    if (ns > nd) out[n - 1] = w.b[ns - 1];
+  }
+}
+
+// max_level (lifted-common.hpp:1099-1108); width 1 never terminates there
+static long max_level(size_t width, size_t n) {
+  if (width == 0) return 0;
+  if (width == 1) return -1;
+  if (n < width - 1) return 0;
+  long lvl = 0;
+  while (n >= 2 * (width - 1)) { ++lvl; n = (n + 1) / 2; }
+  return lvl;
+}
+
+// Enumerate the 1-D lines along `ax` of a box `shape` (other dims iterated).
+static void line_offsets(const std::vector<size_t>& shape, const std::vector<ptrdiff_t>& sin,
+                         const std::vector<ptrdiff_t>& sout, size_t ax, std::vector<ptrdiff_t>& oi,
+                         std::vector<ptrdiff_t>& oo) {
+  const size_t nd = shape.size();
+  size_t nlines = 1;
+  for (size_t k = 0; k < nd; ++k)
+    if (k != ax) nlines *= shape[k];
+  oi.assign(nlines, 0);
+  oo.assign(nlines, 0);
+  if (nlines == 0) return;
+  std::vector<size_t> pos(nd, 0);
+  ptrdiff_t pi = 0, po = 0;
+  for (size_t l = 0; l < nlines; ++l) {
+    oi[l] = pi;
+    oo[l] = po;
+    for (size_t k = nd; k-- > 0;) {  // last dim fastest
+      if (k == ax) continue;
+      pi += sin[k];
+      po += sout[k];
+      if (++pos[k] < shape[k]) break;
+      pos[k] = 0;
+      pi -= ptrdiff_t(shape[k]) * sin[k];
+      po -= ptrdiff_t(shape[k]) * sout[k];
+    }
+  }
+}
+
+template <class F>
+static void parallel_for(size_t n, int n_threads, F&& f) {
+  int nt = n_threads <= 0 ? int(std::max(1u, std::thread::hardware_concurrency())) : n_threads;
+  if (size_t(nt) > n) nt = int(std::max<size_t>(n, 1));
+  if (nt <= 1) { f(size_t(0), n, 0); return; }
+  std::vector<std::thread> th;
+  const size_t base = n / nt, extra = n % nt;
+  size_t lo = 0;
+  for (int t = 0; t < nt; ++t) {
+    size_t cnt = base + (size_t(t) < extra ? 1 : 0);
+    th.emplace_back([&f, lo, cnt, t] { f(lo, lo + cnt, t); });
+    lo += cnt;
+  }
+  for (auto& x : th) x.join();
+}
+
+struct Desc {
+  int tf, bc;
+  std::vector<size_t> shape;
+  std::vector<ptrdiff_t> sin, sout;
+  std::vector<size_t> axes;
+  bool separable;
+  std::vector<long> levels;
+  int n_threads;
+};
+
+static void check_bands(int bc, bool has_steps, size_t n, size_t L) {
+  if (!has_steps || bc == ZERO) return;
+  size_t m = n;
+  for (size_t l = 0; l < L; ++l) {
+    const size_t nd = m / 2, ns = m - nd;
+    if (nd == 0 || (bc == REFLECT && (nd < 2 || ns < 2)))
+      throw std::invalid_argument("band too short for this boundary condition");
+    m = ns;
+  }
+}
+
+template <class T>
+static void run(const WaveletL& w, const Desc& dsc, const T* in, T* out) {
+  std::vector<Step<T>> steps;
+  bool has_steps = false;
+  for (auto& s : w.steps) {
+    steps.push_back(make_step<T>(s));
+    if (s.kind != SCALE) has_steps = true;
+  }
+  const size_t ndim = dsc.shape.size();
+  const int tf = dsc.tf, bc = dsc.bc;
+  if (dsc.separable) {
+    const T* src = in;
+    std::vector<ptrdiff_t> ssrc = dsc.sin;
+    for (size_t ia = 0; ia < dsc.axes.size(); ++ia) {
+      const size_t ax = dsc.axes[ia];
+      const size_t len = dsc.shape[ax];
+      long L = dsc.levels[ia];
+      if (L == 0) L = max_level(w.width, len);
+      if (L < 0) throw std::invalid_argument("level 0 (max level) is undefined for a width-1 wavelet");
+      check_bands(bc, has_steps, len, size_t(L));
+      std::vector<ptrdiff_t> oi, oo;
+      line_offsets(dsc.shape, ssrc, dsc.sout, ax, oi, oo);
+      const ptrdiff_t si = ssrc[ax], so = dsc.sout[ax];
+      parallel_for(oi.size(), dsc.n_threads, [&](size_t lo, size_t hi, int) {
+        LineWork<T> wk;
+        std::vector<T> li(len), lo_(len);
+        for (size_t l = lo; l < hi; ++l) {
+          for (size_t i = 0; i < len; ++i) li[i] = src[oi[l] + ptrdiff_t(i) * si];
+          if (L == 0) continue;
+          if (forward_stepping(tf)) line_fwd_multi(steps, tf, bc, li.data(), lo_.data(), len, size_t(L), wk);
+          else line_rev_multi(steps, tf, bc, li.data(), lo_.data(), len, size_t(L), wk);
+          for (size_t i = 0; i < len; ++i) out[oo[l] + ptrdiff_t(i) * so] = lo_[i];
+        }
+      });
+      src = out;
+      ssrc = dsc.sout;
+    }
+    return;
+  }
+  // non-separable (drivers-inl.hpp:488-610)
+  long L = dsc.levels[0];
+  if (L == 0) {
+    size_t mn = dsc.shape[dsc.axes[0]];
+    for (auto a : dsc.axes) mn = std::min(mn, dsc.shape[a]);
+    L = max_level(w.width, mn);
+    if (L < 0) throw std::invalid_argument("level 0 (max level) is undefined for a width-1 wavelet");
+  }
+  for (auto a : dsc.axes) check_bands(bc, has_steps, dsc.shape[a], size_t(L));
+  std::vector<std::vector<size_t>> shapes(size_t(std::max<long>(L, 0)));
+  if (L > 0) shapes[0] = dsc.shape;
+  for (long l = 1; l < L; ++l) {
+    shapes[l] = shapes[l - 1];
+    for (auto a : dsc.axes) shapes[l][a] = shapes[l - 1][a] - shapes[l - 1][a] / 2;
+  }
+  const T* src = in;
+  std::vector<ptrdiff_t> ssrc = dsc.sin;
+  if (!forward_stepping(tf) && (const void*)in != (void*)out) {
+    // copy in -> out over the full array
+    std::vector<ptrdiff_t> oi, oo;
+    const size_t ax = ndim - 1;
+    line_offsets(dsc.shape, dsc.sin, dsc.sout, ax, oi, oo);
+    for (size_t l = 0; l < oi.size(); ++l)
+      for (size_t i = 0; i < dsc.shape[ax]; ++i)
+        out[oo[l] + ptrdiff_t(i) * dsc.sout[ax]] = in[oi[l] + ptrdiff_t(i) * dsc.sin[ax]];
+    src = out;
+    ssrc = dsc.sout;
+  }
+  for (long il = 0; il < L; ++il) {
+    const auto& shp = forward_stepping(tf) ? shapes[il] : shapes[L - 1 - il];
+    for (size_t ia = 0; ia < dsc.axes.size(); ++ia) {
+      const size_t ax = dsc.axes[ia];
+      const size_t len = shp[ax];
+      std::vector<ptrdiff_t> oi, oo;
+      line_offsets(shp, ssrc, dsc.sout, ax, oi, oo);
+      const ptrdiff_t si = ssrc[ax], so = dsc.sout[ax];
+      parallel_for(oi.size(), dsc.n_threads, [&](size_t lo, size_t hi, int) {
+        LineWork<T> wk;
+        std::vector<T> li(len), lo_(len);
+        for (size_t l = lo; l < hi; ++l) {
+          for (size_t i = 0; i < len; ++i) li[i] = src[oi[l] + ptrdiff_t(i) * si];
+          line_single(steps, tf, bc, li.data(), lo_.data(), len, wk);
+          for (size_t i = 0; i < len; ++i) out[oo[l] + ptrdiff_t(i) * so] = lo_[i];
+        }
+      });
+      src = out;
+      ssrc = dsc.sout;
+    }
+  }
+}
+
+static const WaveletL* find(uint32_t id) {
+  for (auto& w : table())
+    if (w.id == id) return &w;
+  return nullptr;
+}
+
+}  // namespace ora
+
+extern "C" {
+
+typedef struct {
+  int32_t kind;    // 0 generic, 1 unit, 2 repeat, 3 scale
+  int32_t target;  // 0 update evens (s), 1 update odds (d)
+  int32_t offset;
+  int32_t ntaps;   // generic: taps; repeat: N; unit: 1
+  int32_t sign;    // unit: +1 / -1
+  int32_t pad;
+  double taps[16]; // generic: taps; repeat: taps[0]; scale: taps[0]=s_mul, taps[1]=d_div
+} oracle_step;
+
+const char* oracle_last_error(void) { return ora::g_err.c_str(); }
+
+long oracle_max_level(uint32_t wavelet, int64_t n) {
+  const ora::WaveletL* w = ora::find(wavelet);
+  if (!w) return -2;
+  return ora::max_level(w->width, size_t(n));
+}
+
+// Resolved coefficients of a table wavelet for dtype 0 (float) / 1 (double), as
+// doubles: per step [kind, target, offset, ntaps, sign, taps..16] -> 21 values.
+int oracle_wavelet_dump(uint32_t wavelet, int32_t dtype, double* out, int32_t cap_steps) {
+  const ora::WaveletL* w = ora::find(wavelet);
+  if (!w) return -1;
+  int n = 0;
+  for (auto& s : w->steps) {
+    if (n >= cap_steps) break;
+    double* o = out + 21 * n;
+    o[0] = s.kind; o[1] = s.target; o[2] = s.offset; o[3] = s.ntaps; o[4] = s.sign;
+    for (int k = 0; k < 16; ++k) o[5 + k] = 0;
+    if (s.kind == ora::SCALE) {
+      if (dtype == 0) {
+        o[5] = float(s.taps[0]); o[6] = float(1.0L / s.taps[0]); o[7] = float(s.taps[1]); o[8] = float(1.0L / s.taps[1]);
+      } else {
+        o[5] = double(s.taps[0]); o[6] = double(1.0L / s.taps[0]); o[7] = double(s.taps[1]); o[8] = double(1.0L / s.taps[1]);
+      }
+    } else {
+      int nt = s.kind == ora::GENERIC ? s.ntaps : (s.kind == ora::REPEAT ? 1 : 0);
+      for (int k = 0; k < nt; ++k) o[5 + k] = dtype == 0 ? double(float(s.taps[k])) : double(s.taps[k]);
+    }
+    ++n;
+  }
+  return n;
+}
+
+static int oracle_run(const ora::WaveletL& w, uint32_t transform, uint32_t bc, int32_t ndim,
+                      const int64_t* shape, const int64_t* stride_in, const int64_t* stride_out,
+                      int32_t naxes, const int64_t* axes, int32_t separable, const int64_t* levels,
+                      int32_t dtype, const void* in, void* out, int32_t n_threads) {
+  try {
+    if (transform > 3) throw std::invalid_argument("unknown transform");
+    if (bc > 4) throw std::invalid_argument("Unknown Boundary Condition");
+    if (ndim <= 0 || naxes <= 0) throw std::invalid_argument("empty shape or axes");
+    ora::Desc d;
+    d.tf = int(transform);
+    d.bc = int(bc);
+    d.shape.assign(shape, shape + ndim);
+    d.sin.assign(stride_in, stride_in + ndim);
+    d.sout.assign(stride_out, stride_out + ndim);
+    for (int i = 0; i < naxes; ++i) {
+      int64_t a = axes[i];
+      if (a < 0) a += ndim;
+      if (a < 0 || a >= ndim) throw std::invalid_argument("axis out of range");
+      d.axes.push_back(size_t(a));
+    }
+    d.separable = separable != 0;
+    const int nl = d.separable ? naxes : 1;
+    for (int i = 0; i < nl; ++i) {
+      if (levels[i] < 0) throw std::invalid_argument("negative level");
+      d.levels.push_back(long(levels[i]));
+    }
+    d.n_threads = n_threads;
+    if (dtype == 0) ora::run<float>(w, d, (const float*)in, (float*)out);
+    else if (dtype == 1) ora::run<double>(w, d, (const double*)in, (double*)out);
+    else throw std::invalid_argument("dtype must be 0 (float32) or 1 (float64)");
+    return 0;
+  } catch (const std::exception& e) {
+    ora::g_err = e.what();
+    return -1;
+  }
+}
+
+int oracle_lwt(uint32_t wavelet, uint32_t transform, uint32_t bc, int32_t ndim, const int64_t* shape,
+               const int64_t* stride_in, const int64_t* stride_out, int32_t naxes, const int64_t* axes,
+               int32_t separable, const int64_t* levels, int32_t dtype, const void* in, void* out,
+               int32_t n_threads) {
+  const ora::WaveletL* w = ora::find(wavelet);
+  if (!w) { ora::g_err = "unknown wavelet"; return -1; }
+  return oracle_run(*w, transform, bc, ndim, shape, stride_in, stride_out, naxes, axes, separable,
+                    levels, dtype, in, out, n_threads);
+}
+
+// A caller-defined step program (coefficients given as doubles, as the
+// reference's test wavelets write them; scale reciprocals still in long double).
+int oracle_lwt_custom(const oracle_step* steps, int32_t nsteps, uint32_t width, uint32_t transform,
+                      uint32_t bc, int32_t ndim, const int64_t* shape, const int64_t* stride_in,
+                      const int64_t* stride_out, int32_t naxes, const int64_t* axes, int32_t separable,
+                      const int64_t* levels, int32_t dtype, const void* in, void* out, int32_t n_threads) {
+  ora::WaveletL w;
+  w.id = 0xffffffffu;
+  w.width = width;
+  for (int i = 0; i < nsteps; ++i) {
+    ora::StepL s{};
+    s.kind = steps[i].kind; s.target = steps[i].target; s.offset = steps[i].offset;
+    s.ntaps = steps[i].ntaps; s.sign = steps[i].sign;
+    if (s.ntaps > 16 || (s.kind != ora::SCALE && s.ntaps < 1)) { ora::g_err = "bad step"; return -1; }
+    for (int k = 0; k < 16; ++k) s.taps[k] = (long double)steps[i].taps[k];
+    w.steps.push_back(s);
+  }
+  return oracle_run(w, transform, bc, ndim, shape, stride_in, stride_out, naxes, axes, separable, levels,
+                    dtype, in, out, n_threads);
+}
+
+}  // extern "C"

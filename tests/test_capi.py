@@ -1,0 +1,140 @@
+"""C ABI checks that need no GPU: the library loads, exports every symbol the
+headers declare, resolves coefficients exactly like the oracle, and rejects
+invalid descriptors before touching the device."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import WAVELETS, WIDTHS
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    syms = set()
+    for fn in os.listdir(os.path.join(REPO, "include")):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(REPO, "include", fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(lfx_\w+)\s*\(", text):
+            syms.add(m.group(1))
+    return syms
+
+
+def test_library_exports_every_declared_symbol():
+    from lifted_amd import _lib
+    L = _lib.lib()
+    syms = _declared_symbols()
+    assert len(syms) >= 10
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = sorted(s for s in syms if s not in exported)
+    assert not missing, missing
+    for s in syms:
+        getattr(L, s)
+
+
+def test_max_level_matches_oracle(oracle):
+    import lifted_amd as la
+    for w in WAVELETS:
+        for n in (1, 2, 3, 7, 16, 31, 64, 100, 511, 512, 4096, 65536, 1 << 20):
+            if WIDTHS[w] == 1:
+                with pytest.raises(la.LfxError):
+                    la.max_level(w, n)
+                continue
+            assert la.max_level(w, n) == oracle.max_level(w, n), (w, n)
+    assert la.max_level(2, 65536) == 14
+    assert la.max_level(100907, 4096) == 9 and la.max_level(100907, 512) == 6 and la.max_level(100907, 64) == 3
+
+
+@pytest.mark.parametrize("tf", [0, 1, 2, 3])
+def test_resolved_programs_match_oracle_coefficients(oracle, tf):
+    """The GPU's per-transform program (order, band, offsets, signed taps)
+    agrees with the oracle's step data for every wavelet, in both precisions."""
+    import lifted_amd as la
+    fwd_stepping = tf in (0, 3)
+    normal = tf in (0, 1)
+    fwd_tf = tf in (0, 2)
+    for w in WAVELETS:
+        for dt, st in ((np.float32, 0), (np.float64, 1)):
+            rows = oracle.wavelet_dump(w, dt)
+            prog = la.program_dump(w, tf, st)
+            assert len(prog) == len(rows)
+            order = list(range(len(rows))) if fwd_stepping else list(range(len(rows)))[::-1]
+            for p, k in zip(prog, order):
+                r = rows[k]
+                kind = int(r[0])
+                assert int(p[0]) == kind
+                if kind == 3:
+                    sm = r[5] if fwd_tf else r[6]
+                    dm = r[8] if fwd_tf else r[7]
+                    assert p[9] == sm and p[10] == dm
+                    continue
+                N, off = int(r[3]), int(r[2])
+                assert int(p[1]) == (1 if (int(r[1]) == 1) == normal else 0)
+                assert int(p[4]) == (off if normal else -(N - 1 + off))
+                assert int(p[8]) == off
+                if kind == 0:
+                    taps = r[5:5 + N] if normal else r[5:5 + N][::-1]
+                    sign = 1.0 if fwd_tf else -1.0
+                    assert list(p[9:9 + N]) == list(sign * taps)
+                elif kind == 2:
+                    assert p[9] == (r[5] if fwd_tf else -r[5])
+                else:
+                    assert int(p[3]) == (0 if (r[4] > 0) == fwd_tf else 1)
+
+
+def test_invalid_descriptors_are_rejected_without_device():
+    import lifted_amd as la
+    from lifted_amd import _lib
+    L = _lib.lib()
+    base = dict(wavelet=2, transform=0, bc=0, shape=[8], stride_in=[1], stride_out=[1], axes=[0], levels=[1],
+                separable=True, storage=0)
+
+    def call(**kw):
+        a = dict(base, **kw)
+        d = _lib.make_desc(a["wavelet"], a["transform"], a["bc"], a["shape"], a["stride_in"], a["stride_out"],
+                           a["axes"], a["levels"], a["separable"], a["storage"])
+        return L.lfx_lwt(d, None, None, None)
+
+    assert call(transform=4) == -1
+    assert call(bc=5) == -1
+    assert call(storage=7) == -1
+    assert call(axes=[1]) == -1
+    assert call(levels=[-1]) == -1
+    assert call(wavelet=12345) == -1
+    assert b"unknown wavelet" in L.lfx_last_error()
+
+
+def test_python_api_argument_errors():
+    import lifted_amd as la
+    x = np.zeros((4, 4), np.float32)
+    with pytest.raises(KeyError):
+        la.lwt_forward(x, "db99")
+    with pytest.raises(ValueError):
+        la.lwt_forward(x, "db2", axes=2)
+    with pytest.raises(TypeError):
+        la.lwt_forward(x, "db2", axes=[0, 1], level=[1])
+    with pytest.raises(TypeError):
+        la.lwt_forward(x.astype(np.int32), "db2")
+    with pytest.raises(ValueError):
+        la.lwt_forward(x, "db2", out=np.zeros((4, 4), np.float64))
+    with pytest.raises(ValueError):
+        la.lwt_forward(x, "db2", out=np.zeros((4, 5), np.float32))
+    with pytest.raises(KeyError):
+        la.lwt_forward(x, "db2", mode="Bogus")
+
+
+def test_custom_program_validation():
+    import lifted_amd as la
+    with pytest.raises(la.LfxError):
+        la.CustomWavelet([{"kind": "generic", "target": "d", "offset": 0, "taps": [0.0] * 17}])
+    w = la.CustomWavelet([{"kind": "generic", "target": "d", "offset": -1, "taps": [0.5, 0.5]}], width=3)
+    p = la.program_dump(w, 1)
+    assert p.shape[0] == 1 and p[0, 9] == -0.5 and p[0, 10] == -0.5

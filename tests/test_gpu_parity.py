@@ -1,0 +1,167 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle on the same
+seeded inputs.
+
+Bar: float64 within 1e-12 relative to max|ref|, float32 within 4 ulp of
+max|ref|. Both paths perform the same operations in the same order with the
+same fused multiply-adds, so the expected difference is zero; the measured
+maximum is asserted against those tolerances and printed.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from helpers import (BCS, MOCK_WAVELETS, WAVELETS, WIDTHS, fill_sin, max_ulp_diff, oracle_steps, rng)
+
+pytestmark = pytest.mark.gpu
+F, I, FA, IA = 0, 1, 2, 3
+
+
+def _check(got, ref, dtype, what):
+    if dtype == np.float64:
+        scale = max(float(np.abs(ref).max()), 1e-300)
+        err = float(np.abs(got - ref).max()) / scale
+        assert err <= 1e-12, f"{what}: rel err {err}"
+    else:
+        u = max_ulp_diff(got, ref)
+        assert u <= 4.0, f"{what}: {u} ulp"
+
+
+def _gpu(x_np, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x_np)).to(dev)
+
+
+def _ok_bands(bc, n, L, steps=True):
+    if bc == 0 or not steps:
+        return True
+    m = n
+    for _ in range(L):
+        nd = m // 2
+        ns = m - nd
+        if nd == 0 or (bc == 4 and (nd < 2 or ns < 2)):
+            return False
+        m = ns
+    return True
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("w", WAVELETS)
+def test_1d_all_wavelets_bcs_transforms(gpu, oracle, w, dtype):
+    import lifted_amd as la
+    g = rng(w)
+    for n, L, bc, tf in itertools.product((37, 64), (1, 3), BCS, (F, I, FA, IA)):
+        if not _ok_bands(bc, n, L, w != 0):
+            continue
+        x = g.standard_normal(n).astype(dtype)
+        ref = oracle.lwt(x, w, tf, bc, level=[L])
+        got = la.lwt(_gpu(x, gpu), w, tf, mode=bc, level=[L]).cpu().numpy()
+        _check(got, ref, dtype, f"w={w} n={n} L={L} bc={bc} tf={tf}")
+
+
+@pytest.mark.parametrize("name", list(MOCK_WAVELETS))
+def test_mock_wavelets(gpu, oracle, name):
+    """The reference's mock step programs (tests/test_helpers.h:102-360), all BCs,
+    lengths 153 / 510 (tests/test_single_levels.cpp:395-405)."""
+    import lifted_amd as la
+    cw = la.CustomWavelet(MOCK_WAVELETS[name])
+    st = oracle_steps(MOCK_WAVELETS[name])
+    for n, bc, tf, dtype in itertools.product((153, 510), BCS, (F, I, FA, IA), (np.float64, np.float32)):
+        x = fill_sin(n, dtype=dtype)
+        ref = oracle.lwt(x, 0, tf, bc, level=1, steps=st)
+        got = la.lwt(_gpu(x, gpu), cw, tf, mode=bc, level=1).cpu().numpy()
+        _check(got, ref, dtype, f"{name} n={n} bc={bc} tf={tf}")
+
+
+REF_SHAPES = [((31,), (0,)), ((32,), (0,)), ((32, 1), (0,)), ((1, 32), (1,)), ((34, 31), (1,)),
+              ((23, 41), (0,)), ((34, 31), (1, 0)), ((23, 41), (0, 1)), ((25, 24, 31), (2,)),
+              ((24, 23, 41), (1,)), ((24, 23, 41), (0,)), ((25, 24, 31), (2, 1)), ((24, 23, 41), (1, 0)),
+              ((24, 23, 41), (0, 2)), ((23, 24, 31), (2, 1, 0))]
+
+
+@pytest.mark.parametrize("shape,axes", REF_SHAPES)
+def test_nd_shapes_of_reference_tests(gpu, oracle, shape, axes):
+    """tests/test_dwt.cpp shapes/axes, separable and level-by-level, levels 1 and 3."""
+    import lifted_amd as la
+    n = int(np.prod(shape))
+    for (w, bc), level, tf, sep in itertools.product(((2, 0), (100907, 3), (3, 2), (10204, 1)), (1, 3),
+                                                      (F, I, FA, IA), (True, False)):
+        if any(not _ok_bands(bc, shape[a], level) for a in axes):
+            continue
+        x = fill_sin(n, -1000.0, 1001.0).reshape(shape)
+        lvl = [level] * len(axes) if sep else level
+        ref = oracle.lwt(x, w, tf, bc, axes=list(axes), level=lvl)
+        got = la.lwt(_gpu(x, gpu), w, tf, mode=bc, axes=list(axes), level=lvl).cpu().numpy()
+        _check(got, ref, np.float64, f"shape={shape} axes={axes} w={w} L={level} tf={tf} sep={sep}")
+
+
+def test_layouts_fortran_strided_inplace(gpu, oracle):
+    import torch
+    import lifted_amd as la
+    x = rng(7).standard_normal((40, 52))
+    for tf in (F, I, FA, IA):
+        ref = oracle.lwt(x, 100907, tf, 3, axes=[0, 1], level=[2, 3])
+        xt = torch.from_numpy(x).to(gpu)
+        # Fortran-ordered input and output
+        xf = xt.t().contiguous().t()
+        got = la.lwt(xf, 100907, tf, mode=3, axes=[0, 1], level=[2, 3])
+        assert got.stride() == xf.stride()
+        _check(got.cpu().numpy(), ref, np.float64, f"F-order tf={tf}")
+        # strided view in, contiguous out
+        big = torch.zeros((80, 104), dtype=torch.float64, device=gpu)
+        big[::2, ::2] = xt
+        got = la.lwt(big[::2, ::2], 100907, tf, mode=3, axes=[0, 1], level=[2, 3],
+                     out=torch.empty((40, 52), dtype=torch.float64, device=gpu))
+        _check(got.cpu().numpy(), ref, np.float64, f"strided tf={tf}")
+        # in place
+        xi = xt.clone()
+        la.lwt(xi, 100907, tf, mode=3, axes=[0, 1], level=[2, 3], out=xi)
+        _check(xi.cpu().numpy(), ref, np.float64, f"in-place tf={tf}")
+        # level by level in place
+        ref2 = oracle.lwt(x, 100907, tf, 3, axes=[1, 0], level=2)
+        xi = xt.clone()
+        la.lwt(xi, 100907, tf, mode=3, axes=[1, 0], level=2, out=xi)
+        _check(xi.cpu().numpy(), ref2, np.float64, f"in-place non-sep tf={tf}")
+
+
+def test_numpy_host_path(gpu, oracle):
+    """numpy in/out goes through lfx_lwt_host (the reference's calling convention)."""
+    import lifted_amd as la
+    x = rng(3).standard_normal((33, 20)).astype(np.float32)
+    for tf in (F, I, FA, IA):
+        ref = oracle.lwt(x, 3, tf, 4, level=1)
+        got = la.lwt(x, 3, tf, mode=4, level=1)
+        assert isinstance(got, np.ndarray)
+        _check(got, ref, np.float32, f"host tf={tf}")
+    xf = np.asfortranarray(x)
+    got = la.lwt_forward(xf, "db3", mode="Reflect", level=1)
+    assert got.flags.f_contiguous
+    _check(got, oracle.lwt(x, 3, F, 4, level=1), np.float32, "host F-order")
+
+
+def test_kat_mixed_bc_2d_dot_on_gpu(gpu):
+    """<fwd_sym(y2), inv_adj_zero(y2)> of scratch/test_python_package.ipynb (1974.8783750525286)."""
+    import torch
+    import lifted_amd as la
+    x = np.linspace(-10, 10, 1024)
+    y2 = 1 / ((x - 2)[:, None] ** 2 + (x + 3) ** 2 + 4)
+    t = torch.from_numpy(y2).to(gpu)
+    a = la.lwt_forward(t, "db3", mode=la.BoundaryCondition.Symmetric, level=1)
+    b = la.lwt_inverse_adjoint(t, "db3", mode=la.BoundaryCondition.Zero, level=1)
+    v = float((a * b).sum())
+    assert abs(v - 1974.8783750525286) <= 1e-12 * 1974.9
+
+
+@pytest.mark.parametrize("storage", ["bfloat16", "float16"])
+def test_half_storage_single_axis_matches_fp32_rounded(gpu, oracle, storage):
+    """16-bit storage, fp32 arithmetic: one axis pass equals the fp32 oracle on
+    the 16-bit input, rounded once to the storage type."""
+    import torch
+    import lifted_amd as la
+    tdt = getattr(torch, storage)
+    x = torch.from_numpy(rng(11).standard_normal((16, 300)).astype(np.float32)).to(tdt)
+    x32 = x.float().cpu().numpy()
+    for tf in (F, I, FA, IA):
+        ref = torch.from_numpy(oracle.lwt(x32, 100907, tf, 0, axes=[1], level=[4])).to(tdt).float().numpy()
+        got = la.lwt(x.to(gpu), 100907, tf, axes=[1], level=[4]).float().cpu().numpy()
+        np.testing.assert_array_equal(got, ref)
