@@ -40,7 +40,7 @@ def build(verbose=False):
     for src, device in SOURCES:
         obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
         cmd = [hipcc, "-c"] + COMMON
-        cmd += DEVICE if device else ["-x", "c++", "-D__HIP_PLATFORM_AMD__"]
+        cmd += DEVICE if device else ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
         cmd += [os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd))
