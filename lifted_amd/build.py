@@ -23,6 +23,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 SOURCES = [
     ("lfx_kernels.hip", True),
     ("lfx_api.cpp", False),
+    ("lfx_table.cpp", False),
 ]
 
 

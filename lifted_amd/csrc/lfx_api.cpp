@@ -27,148 +27,18 @@
 #include "../../include/lifted_hip.h"
 #include "lfx_kernels.hpp"
 #include "lfx_program.hpp"
+#include "lfx_tiles.hpp"
+#include "lfx_error.hpp"
+#include "lfx_table.hpp"
+#include "lfx_schedule.hpp"
 
 namespace lfx {
 
+static std::mutex g_mu;  // guards the program cache and the workspaces
 static thread_local std::string g_err;
 
-struct Error {
-  int code;
-  std::string msg;
-};
-[[noreturn]] static void fail(int code, const std::string& m) { throw Error{code, m}; }
 static void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess) fail(LFX_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-// ---------------------------------------------------------------------------
-// wavelet table (generated from the reference's src/lifted-wavelets.hpp)
-
-static std::vector<WaveletDefL> build_table() {
-  std::vector<WaveletDefL> v;
-  WaveletDefL* cur = nullptr;
-  auto add = [&](int kind, int target, int off, int n, int sign, const long double* t, int nt) {
-    StepDefL& s = cur->steps[cur->nsteps++];
-    s = StepDefL{};
-    s.kind = kind;
-    s.target = target;
-    s.offset = off;
-    s.ntaps = n;
-    s.sign = sign;
-    for (int k = 0; k < nt; ++k) s.taps[k] = t[k];
-  };
-  const int S = 0, D = 1;
-#define LFX_WAVELET(nm, idv, w)  \
-  v.emplace_back();              \
-  cur = &v.back();               \
-  *cur = WaveletDefL{};          \
-  cur->id = idv;                 \
-  cur->name = #nm;               \
-  cur->width = w;
-#define LFX_END(nm)
-#define LFX_GENERIC(b, off, ...)                                         \
-  {                                                                      \
-    const long double t_[] = {__VA_ARGS__};                              \
-    const int n_ = int(sizeof(t_) / sizeof(t_[0]));                      \
-    add(kGeneric, b, off, n_, 1, t_, n_);                                \
-  }
-#define LFX_UNIT(b, off, sg) add(kUnit, b, off, 1, sg, nullptr, 0);
-#define LFX_REPEAT(b, off, n, val)   \
-  {                                  \
-    const long double t_[] = {val};  \
-    add(kRepeat, b, off, n, 1, t_, 1); \
-  }
-#define LFX_SCALE(a, b)                 \
-  {                                     \
-    const long double t_[] = {a, b};    \
-    add(kScale, 0, 0, 0, 1, t_, 2);     \
-  }
-#include "wavelet_table.inc"
-#undef LFX_WAVELET
-#undef LFX_END
-#undef LFX_GENERIC
-#undef LFX_UNIT
-#undef LFX_REPEAT
-#undef LFX_SCALE
-  return v;
-}
-
-static std::mutex g_mu;
-static std::vector<WaveletDefL>& table() {
-  static std::vector<WaveletDefL> t = build_table();
-  return t;
-}
-static std::vector<WaveletDefL> g_custom;  // ids 0x80000000 + index
-constexpr uint32_t kCustomBase = 0x80000000u;
-
-static WaveletDefL find_wavelet(uint32_t id) {
-  std::lock_guard<std::mutex> lk(g_mu);
-  if (id >= kCustomBase) {
-    if (id - kCustomBase < g_custom.size()) return g_custom[id - kCustomBase];
-  } else {
-    for (auto& w : table())
-      if (w.id == id) return w;
-  }
-  fail(LFX_EINVAL, "unknown wavelet id " + std::to_string(id));
-}
-
-// Resolve a wavelet for one transform type (see lfx_program.hpp).
-template <class C>
-static DProgram<C> resolve(const WaveletDefL& w, int tf) {
-  DProgram<C> p{};
-  p.width = (int32_t)w.width;
-  const bool fwd_stepping = tf == kForward || tf == kInverseAdjoint;
-  const bool normal = tf == kForward || tf == kInverse;
-  const bool fwd_tf = tf == kForward || tf == kForwardAdjoint;
-  int nc = 0;
-  for (int q = 0; q < w.nsteps; ++q) {
-    const StepDefL& sd = w.steps[fwd_stepping ? q : w.nsteps - 1 - q];
-    DStep& st = p.steps[p.nsteps++];
-    st = DStep{};
-    st.kind = sd.kind;
-    st.coef = nc;
-    if (sd.kind == kScale) {
-      // ScaleStep (lifted-common.hpp:936-964): Forward/ForwardAdjoint multiply by
-      // (s_mul, d_mul = 1/d_div), Inverse/InverseAdjoint by (s_div = 1/s_mul, d_div);
-      // reciprocals in long double, then cast.
-      p.coef[nc++] = fwd_tf ? (C)sd.taps[0] : (C)(1.0L / sd.taps[0]);
-      p.coef[nc++] = fwd_tf ? (C)(1.0L / sd.taps[1]) : (C)sd.taps[1];
-      continue;
-    }
-    const int N = sd.ntaps, o = sd.offset;
-    const int maxo = N - 1 + o;
-    const int n_front = o < 0 ? -o : 0, n_back = maxo < 0 ? 0 : maxo;
-    const int o_r = -maxo;
-    const int maxo_r = N - 1 + o_r;
-    const int n_front_r = o_r < 0 ? -o_r : 0, n_back_r = maxo_r < 0 ? 0 : maxo_r;
-    // updated band: normal transforms update the declared band, adjoints the other
-    st.y_is_d = ((sd.target == 1) == normal) ? 1 : 0;
-    st.adjoint = normal ? 0 : 1;
-    st.off = normal ? o : o_r;
-    st.N = N;
-    st.nf = normal ? n_front : n_front_r;
-    st.nb = normal ? n_back : n_back_r;
-    st.fold_off = o;
-    if (sd.kind == kUnit) {
-      st.sub = ((sd.sign > 0) == fwd_tf) ? 0 : 1;
-    } else {
-      const bool sub = !fwd_tf;
-      if (sd.kind == kGeneric) {
-        for (int k = 0; k < N; ++k) {
-          C c = (C)sd.taps[normal ? k : N - 1 - k];
-          p.coef[nc++] = sub ? -c : c;
-        }
-      } else {
-        C c = (C)sd.taps[0];
-        p.coef[nc++] = sub ? -c : c;
-      }
-    }
-    // reach of the stencil relative to the updated element (band index units)
-    const int lo = -st.off, hi = st.off + N - 1;
-    p.reach_lo = std::max(p.reach_lo, std::max(lo, 0));
-    p.reach_hi = std::max(p.reach_hi, std::max(hi, 0));
-  }
-  return p;
 }
 
 // Device-resident programs, one per (device, wavelet, transform, compute type).
@@ -203,40 +73,6 @@ static const void* device_program(uint32_t wavelet, int tf, int storage) {
   }
   g_dev_progs[key] = d;
   return d;
-}
-
-// max_level (lifted-common.hpp:1099-1108); -1 for width 1 (never terminates there)
-static long long max_level_w(long long width, long long n) {
-  if (width == 0) return 0;
-  if (width == 1) return -1;
-  if (n < width - 1) return 0;
-  long long lvl = 0;
-  while (n >= 2 * (width - 1)) {
-    ++lvl;
-    n = (n + 1) / 2;
-  }
-  return lvl;
-}
-
-static bool has_lift_steps(const WaveletDefL& w) {
-  for (int i = 0; i < w.nsteps; ++i)
-    if (w.steps[i].kind != kScale) return true;
-  return false;
-}
-
-// Bands the reference cannot process: empty bands under a non-Zero BC divide
-// by zero (Periodic) or loop forever (Symmetric, Reflect), as do bands of
-// length 1 under Reflect.
-static void check_bands(int bc, bool steps, long long n, long long L) {
-  if (!steps || bc == kZero) return;
-  long long m = n;
-  for (long long l = 0; l < L; ++l) {
-    const long long nd = m / 2, ns = m - nd;
-    if (nd == 0 || (bc == kReflect && (nd < 2 || ns < 2)))
-      fail(LFX_EINVAL, "level " + std::to_string(l + 1) + " of a length-" + std::to_string(n) +
-                           " axis leaves a band too short for this boundary condition");
-    m = ns;
-  }
 }
 
 struct Plan {
@@ -285,28 +121,86 @@ static Plan make_plan(const lfx_desc* d) {
   return p;
 }
 
-static long long line_limit(int storage) {
-  // longest line the whole-line kernel takes: LDS = 1.5 n elements
-  const size_t cb = compute_bytes(storage);
-  return (long long)(kMaxLdsBytes / (cb * 3 / 2)) - 2;
+// ---------------------------------------------------------------------------
+// workspace: one grow-only device buffer per device. Work that uses it is
+// serialized across streams with an event (not while a stream is capturing).
+
+struct Workspace {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipEvent_t ev = nullptr;
+};
+static std::map<int, Workspace> g_ws;
+
+struct WsLease {
+  void* p = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t ev = nullptr;
+  ~WsLease() {
+    if (ev) (void)hipEventRecord(ev, st);
+  }
+};
+
+static void lease_workspace(size_t bytes, hipStream_t st, WsLease& lease) {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_mu);
+  Workspace& w = g_ws[dev];
+  if (w.bytes < bytes) {
+    if (w.p) {
+      hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+      (void)hipFree(w.p);
+      w.p = nullptr;
+      w.bytes = 0;
+    }
+    const size_t want = std::max(bytes, (size_t)1 << 20);
+    if (hipMalloc(&w.p, want) != hipSuccess) {
+      (void)hipGetLastError();
+      fail(LFX_ENOMEM, "workspace allocation of " + std::to_string(want) + " bytes failed");
+    }
+    w.bytes = want;
+  }
+  if (!w.ev) hip_check(hipEventCreateWithFlags(&w.ev, hipEventDisableTiming), "hipEventCreate");
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  hip_check(hipStreamIsCapturing(st, &cs), "hipStreamIsCapturing");
+  if (cs == hipStreamCaptureStatusNone) {
+    hip_check(hipStreamWaitEvent(st, w.ev, 0), "hipStreamWaitEvent");
+    lease.ev = w.ev;
+  }
+  lease.p = w.p;
+  lease.st = st;
 }
 
-// Build the line map of all lines along `ax` of the box `shp`.
-static LineMap line_map(int ndim, const long long* shp, const long long* sst, const long long* dst, int ax,
-                        long long& nlines) {
-  LineMap m{};
+// LineMap3 over the non-axis dims of box `shp`, with per-array strides.
+static LineMap3 line_map3(int ndim, const long long* shp, int ax, const long long* s0, long long a0,
+                          const long long* s1, long long a1, const long long* s2, long long a2, long long& nlines) {
+  LineMap3 m{};
   nlines = 1;
   for (int k = 0; k < ndim; ++k) {
     if (k == ax) continue;
     m.shape[m.nod] = shp[k];
-    m.sst[m.nod] = sst[k];
-    m.dstr[m.nod] = dst[k];
+    m.str[0][m.nod] = s0 ? s0[k] : 0;
+    m.str[1][m.nod] = s1 ? s1[k] : 0;
+    m.str[2][m.nod] = s2 ? s2[k] : 0;
     ++m.nod;
     nlines *= shp[k];
   }
-  m.s_ax = sst[ax];
-  m.d_ax = dst[ax];
+  m.ax[0] = a0;
+  m.ax[1] = a1;
+  m.ax[2] = a2;
   return m;
+}
+
+// Dense strides of a scratch array holding, per line, `len` contiguous samples.
+static std::vector<long long> dense_strides(int ndim, const long long* shp, int ax, long long len) {
+  std::vector<long long> s(ndim, 0);
+  long long acc = len;
+  for (int k = ndim - 1; k >= 0; --k) {
+    if (k == ax) continue;
+    s[k] = acc;
+    acc *= shp[k];
+  }
+  return s;
 }
 
 static void copy_box(int storage, int ndim, const long long* shp, const void* src, const long long* sst, void* dst,
@@ -323,17 +217,135 @@ static void copy_box(int storage, int ndim, const long long* shp, const void* sr
   hip_check(launch_copy(storage, src, dst, cm, total, st), "copy kernel");
 }
 
-// One 1-D transform of L levels (L == 1 for a single level) of all lines of `shp` along `ax`.
-static void transform_lines(const Plan& p, const void* prog, bool fwd_stepping, const long long* shp, int ax,
-                            long long L, const void* src, const long long* sst, void* dst, const long long* dstr,
-                            hipStream_t st) {
-  long long nlines = 0;
-  LineMap lm = line_map(p.ndim, shp, sst, dstr, ax, nlines);
+struct Tuning {
+  long long tile = 0;  // 0: default tile length
+  int k = 0;           // 0: default levels per pass
+};
+static thread_local Tuning g_tuning;
+
+template <class C>
+static void transform_lines_t(const Plan& p, const WaveletDefL& w, const void* prog, bool fwd, const long long* shp,
+                              int ax, long long L, const void* src, const long long* sst, void* dst,
+                              const long long* dstr, hipStream_t st) {
   const long long n = shp[ax];
-  if (nlines == 0 || n == 0) return;
-  if (n > line_limit(p.storage))
-    fail(LFX_EUNSUPPORTED, "line length " + std::to_string(n) + " exceeds the single-workgroup limit");
-  hip_check(launch_line(p.storage, fwd_stepping, prog, p.bc, n, (int)L, src, dst, lm, nlines, st), "line kernel");
+  long long nlines = 1;
+  for (int k = 0; k < p.ndim; ++k)
+    if (k != ax) nlines *= shp[k];
+  if (nlines == 0 || n == 0 || L == 0) return;
+  const DProgram<C> hp = resolve<C>(w, p.tf);
+  const LineSchedule sc = schedule(hp, fwd, n, L, p.storage, p.bc, g_tuning.tile, g_tuning.k);
+  const bool alias = src == dst;
+  if (sc.passes.empty()) {
+    LineLaunch ll{};
+    ll.storage = p.storage; ll.fwd = fwd; ll.prog = prog; ll.bc = p.bc; ll.n = n; ll.L = (int)L;
+    ll.a0 = src; ll.a1 = dst; ll.a2 = src;
+    ll.lm = line_map3(p.ndim, shp, ax, sst, sst[ax], dstr, dstr[ax], sst, sst[ax], ll.nlines);
+    hip_check(launch_line(ll, st), "line kernel");
+    return;
+  }
+  // workspace: [copy of the input if it aliases the output][scratch A][scratch B]
+  const size_t es = storage_bytes(p.storage), cs = compute_bytes(p.storage);
+  long long mmax = 0;
+  for (size_t i = 1; i < sc.passes.size(); ++i) mmax = std::max(mmax, sc.passes[i].m0);
+  mmax = std::max(mmax, sc.m_line);
+  const size_t copy_bytes = alias ? (size_t)(nlines * n) * es : 0;
+  const size_t copy_pad = (copy_bytes + 255) / 256 * 256;
+  const size_t scr_bytes = (size_t)(nlines * mmax) * cs;
+  const size_t scr_pad = (scr_bytes + 255) / 256 * 256;
+  WsLease lease;
+  lease_workspace(copy_pad + 2 * scr_pad, st, lease);
+  char* ws = (char*)lease.p;
+  void* scr[2] = {ws + copy_pad, ws + copy_pad + scr_pad};
+  const void* in = src;
+  std::vector<long long> in_str(sst, sst + p.ndim);
+  if (alias) {
+    in_str = dense_strides(p.ndim, shp, ax, n);
+    std::vector<long long> s2(in_str);
+    s2[ax] = 1;
+    copy_box(p.storage, p.ndim, shp, src, sst, ws, s2.data(), st);
+    in = ws;
+  }
+  const long long in_ax = alias ? 1 : sst[ax];
+  const size_t P = sc.passes.size();
+  if (fwd) {
+    const void* cur = in;
+    bool cur_scr = false;
+    std::vector<long long> cur_str = in_str;
+    long long cur_ax = in_ax;
+    for (size_t i = 0; i < P; ++i) {
+      const PassPlan& pp = sc.passes[i];
+      const bool last = (i + 1 == P) && sc.L_line == 0;
+      const long long mk = band_len(pp.m0, pp.k);
+      std::vector<long long> ks = dense_strides(p.ndim, shp, ax, mk);
+      TileLaunch tl{};
+      tl.storage = p.storage; tl.fwd = true; tl.prog = prog; tl.bc = p.bc;
+      tl.ta = TileArgs{pp.m0, pp.T, pp.H, pp.ntiles, pp.k, 0};
+      tl.a0 = cur; tl.a1 = dst; tl.a2 = last ? dst : scr[i & 1];
+      tl.scr[0] = cur_scr; tl.scr[1] = false; tl.scr[2] = !last;
+      tl.lm = line_map3(p.ndim, shp, ax, cur_str.data(), cur_ax, dstr, dstr[ax], last ? dstr : ks.data(),
+                        last ? dstr[ax] : 1, tl.nlines);
+      hip_check(launch_tile(tl, st), "tile kernel");
+      cur = scr[i & 1];
+      cur_scr = true;
+      cur_str = ks;
+      cur_ax = 1;
+    }
+    if (sc.L_line > 0) {
+      LineLaunch ll{};
+      ll.storage = p.storage; ll.fwd = true; ll.prog = prog; ll.bc = p.bc; ll.n = sc.m_line; ll.L = (int)sc.L_line;
+      ll.a0 = cur; ll.a1 = dst; ll.a2 = cur;
+      ll.scr[0] = true;
+      ll.lm = line_map3(p.ndim, shp, ax, cur_str.data(), 1, dstr, dstr[ax], cur_str.data(), 1, ll.nlines);
+      hip_check(launch_line(ll, st), "line kernel");
+    }
+    return;
+  }
+  // reverse: whole-line part first (coarsest levels), then tiled passes coarse -> fine
+  const void* s_src = in;
+  bool s_scr = false;
+  std::vector<long long> s_str = in_str;
+  long long s_ax = in_ax;
+  int buf = 0;
+  if (sc.L_line > 0) {
+    std::vector<long long> os = dense_strides(p.ndim, shp, ax, sc.m_line);
+    LineLaunch ll{};
+    ll.storage = p.storage; ll.fwd = false; ll.prog = prog; ll.bc = p.bc; ll.n = sc.m_line; ll.L = (int)sc.L_line;
+    ll.a0 = in; ll.a1 = scr[buf]; ll.a2 = in;
+    ll.scr[1] = true;
+    ll.lm = line_map3(p.ndim, shp, ax, in_str.data(), in_ax, os.data(), 1, in_str.data(), in_ax, ll.nlines);
+    hip_check(launch_line(ll, st), "line kernel");
+    s_src = scr[buf];
+    s_scr = true;
+    s_str = os;
+    s_ax = 1;
+    buf ^= 1;
+  }
+  for (size_t j = P; j-- > 0;) {
+    const PassPlan& pp = sc.passes[j];
+    const bool final_pass = j == 0;
+    std::vector<long long> os = dense_strides(p.ndim, shp, ax, pp.m0);
+    TileLaunch tl{};
+    tl.storage = p.storage; tl.fwd = false; tl.prog = prog; tl.bc = p.bc;
+    tl.ta = TileArgs{pp.m0, pp.T, pp.H, pp.ntiles, pp.k, 0};
+    tl.a0 = in; tl.a1 = final_pass ? dst : scr[buf]; tl.a2 = const_cast<void*>(s_src);
+    tl.scr[0] = false; tl.scr[1] = !final_pass; tl.scr[2] = s_scr;
+    tl.lm = line_map3(p.ndim, shp, ax, in_str.data(), in_ax, final_pass ? dstr : os.data(),
+                      final_pass ? dstr[ax] : 1, s_str.data(), s_ax, tl.nlines);
+    hip_check(launch_tile(tl, st), "tile kernel");
+    s_src = scr[buf];
+    s_scr = true;
+    s_str = os;
+    s_ax = 1;
+    buf ^= 1;
+  }
+}
+
+// One 1-D transform of L levels (L == 1 for a single level) of all lines of `shp` along `ax`.
+static void transform_lines(const Plan& p, const WaveletDefL& w, const void* prog, bool fwd, const long long* shp,
+                            int ax, long long L, const void* src, const long long* sst, void* dst,
+                            const long long* dstr, hipStream_t st) {
+  if (compute_bytes(p.storage) == 8) transform_lines_t<double>(p, w, prog, fwd, shp, ax, L, src, sst, dst, dstr, st);
+  else transform_lines_t<float>(p, w, prog, fwd, shp, ax, L, src, sst, dst, dstr, st);
 }
 
 static void run_plan(const Plan& p, const void* in, void* out, hipStream_t st) {
@@ -370,8 +382,7 @@ static void run_plan(const Plan& p, const void* in, void* out, hipStream_t st) {
         if (ia == 0 && in != out) copy_box(p.storage, p.ndim, p.shape, in, p.sin, out, p.sout, st);
         continue;
       }
-      if (p.flags & LFX_FLAG_PLAIN) fail(LFX_EUNSUPPORTED, "plain per-level path not built yet");
-      transform_lines(p, prog, fwd_stepping, p.shape, p.axes[ia], L[ia], src, sst, out, p.sout, st);
+      transform_lines(p, w, prog, fwd_stepping, p.shape, p.axes[ia], L[ia], src, sst, out, p.sout, st);
     }
     return;
   }
@@ -406,7 +417,7 @@ static void run_plan(const Plan& p, const void* in, void* out, hipStream_t st) {
   for (long long il = 0; il < L; ++il) {
     const std::vector<long long>& shp = fwd_stepping ? shapes[il] : shapes[L - 1 - il];
     for (int a : p.axes) {
-      transform_lines(p, prog, fwd_stepping, shp.data(), a, 1, src, sst, out, p.sout, st);
+      transform_lines(p, w, prog, fwd_stepping, shp.data(), a, 1, src, sst, out, p.sout, st);
       src = out;
       sst = p.sout;
     }
@@ -591,10 +602,7 @@ int lfx_program_create(const lfx_step* steps, int32_t nsteps, uint32_t width, ui
       d.sign = s.sign >= 0 ? 1 : -1;
       for (int k = 0; k < kMaxTaps; ++k) d.taps[k] = (long double)s.taps[k];
     }
-    std::lock_guard<std::mutex> lk(g_mu);
-    w.id = kCustomBase + (uint32_t)g_custom.size();
-    g_custom.push_back(w);
-    *wavelet_id = w.id;
+    *wavelet_id = register_custom(w);
   });
 }
 

@@ -1,0 +1,216 @@
+// Kernel bodies of the lifting DWT: whole-line and tiled multi-level
+// transforms of one line (or one tile of one line) by one workgroup.
+//
+//  * whole-line (line_fwd_body / line_rev_body): the workgroup owns a whole
+//    1-D line and runs all its levels in LDS (the reference's per-line
+//    multi-level drivers, src/lifted-drivers-inl.hpp:75-274, single level when
+//    L == 1, :144-179 / :276-311). The line is read once and written once.
+//      forward-stepping (Forward, InverseAdjoint): deinterleave on load, steps,
+//      write d_l to out[ns_l, ns_l + nd_l), deinterleave s_l in LDS for the next
+//      level, finally s_L -> out[0, ns_L):  [s_L | d_L | ... | d_1].
+//      reverse-stepping (Inverse, ForwardAdjoint): read s_L, per level read d_l,
+//      steps, interleave into the next s; the finest level is interleaved
+//      straight into the output.
+//
+//  * tiled (tile_fwd_body / tile_rev_body): lines longer than one workgroup's
+//    LDS are cut into tiles of T samples; a tile loads its window plus halo and
+//    runs k levels in LDS, tracking which elements are final (lfx_tiles.hpp).
+//    Forward tiles write d_1..d_k of their own range and s_k to a scratch band;
+//    reverse tiles read s_k and d_k..d_1 and write their part of the finer band.
+//
+// The same source also compiles for the host (LFX_CPU_SIM) with one "lane",
+// so the tiling logic can be checked against the oracle without a GPU.
+#pragma once
+#include "lfx_device.hpp"
+#include "lfx_tiles.hpp"
+
+namespace lfx {
+
+using dev::Band;
+using dev::load_as;
+using dev::store_as;
+
+// ---------------------------------------------------------------------------
+// whole-line bodies
+
+template <class C, class SI, class SO>
+__device__ void line_fwd_body(const DProgram<C>* prog, int bc, long long n, int L, const SI* in, long long i_ax,
+                              SO* out, long long o_ax, C* P, C* Q) {
+  const int tid = LFX_TID, nthr = LFX_NTHR;
+  const long long ns0 = n - n / 2;
+  for (long long i = tid; i < n; i += nthr) P[(i & 1) ? ns0 + (i >> 1) : (i >> 1)] = load_as<SI, C>(in, i * i_ax);
+  LFX_SYNC();
+  C* cur = P;
+  C* oth = Q;
+  long long m = n;
+  for (int l = 0; l < L; ++l) {
+    const long long nd = m / 2, ns = m - nd;
+    const Band<C> s{cur, 0, (int)ns}, d{cur + ns, 0, (int)nd};
+    dev::level_steps<C>(prog, bc, s, d, ns, nd, tid, nthr);
+    for (long long i = tid; i < nd; i += nthr) store_as<SO, C>(out, (ns + i) * o_ax, cur[ns + i]);
+    if (l + 1 < L) {
+      const long long ns2 = ns - ns / 2;
+      for (long long i = tid; i < ns; i += nthr) oth[(i & 1) ? ns2 + (i >> 1) : (i >> 1)] = cur[i];
+      LFX_SYNC();
+      C* t = cur;
+      cur = oth;
+      oth = t;
+    }
+    m = ns;
+  }
+  for (long long i = tid; i < m; i += nthr) store_as<SO, C>(out, i * o_ax, cur[i]);
+}
+
+// `in` holds [s_L | d_L | ... | d_1] of a band of length n; s_L may come from a
+// separate array `sin` (scratch of a coarser pass) -- pass sin == in otherwise.
+template <class C, class SI, class SS, class SO>
+__device__ void line_rev_body(const DProgram<C>* prog, int bc, long long n, int L, const SI* in, long long i_ax,
+                              const SS* sin, long long s_ax, SO* out, long long o_ax, C* P, C* Q) {
+  const int tid = LFX_TID, nthr = LFX_NTHR;
+  C* buf[2] = {P, Q};
+  C* cur = buf[(L - 1) & 1];
+  {
+    const long long mL = band_len(n, L - 1);
+    const long long nsL = mL - mL / 2;
+    for (long long i = tid; i < nsL; i += nthr) cur[i] = load_as<SS, C>(sin, i * s_ax);
+  }
+  for (int l = L - 1; l >= 0; --l) {
+    const long long m = band_len(n, l);
+    const long long nd = m / 2, ns = m - nd;
+    for (long long i = tid; i < nd; i += nthr) cur[ns + i] = load_as<SI, C>(in, (ns + i) * i_ax);
+    LFX_SYNC();
+    const Band<C> s{cur, 0, (int)ns}, d{cur + ns, 0, (int)nd};
+    dev::level_steps<C>(prog, bc, s, d, ns, nd, tid, nthr);
+    if (l > 0) {
+      C* nxt = buf[(l - 1) & 1];
+      for (long long j = tid; j < m; j += nthr) nxt[j] = (j & 1) ? cur[ns + (j >> 1)] : cur[j >> 1];
+      LFX_SYNC();
+      cur = nxt;
+    } else {
+      for (long long j = tid; j < m; j += nthr)
+        store_as<SO, C>(out, j * o_ax, (j & 1) ? cur[ns + (j >> 1)] : cur[j >> 1]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// tiled bodies
+
+// All steps of one level on tile windows, each step updating only the
+// elements whose inputs are final; `r` tracks the valid ranges (global).
+template <class C>
+__device__ void tile_level_steps(const DProgram<C>* __restrict__ prog, int bc, const Band<C>& s, const Band<C>& d,
+                                 long long ns, long long nd, Ranges& r, int tid, int nthr) {
+  const int nsteps = prog->nsteps;
+  for (int k = 0; k < nsteps; ++k) {
+    const DStep st = prog->steps[k];
+    const C* cf = prog->coef + st.coef;
+    if (st.kind == kScale) {
+      dev::scale_step<C>(cf, s, d, tid, nthr);
+    } else {
+      step_shrink(st, ns, nd, r);
+      const long long ylo = st.y_is_d ? r.dlo : r.slo, yhi = st.y_is_d ? r.dhi : r.shi;
+      dev::lift_step<C>(st, cf, bc, s, d, ns, nd, ylo, yhi, tid, nthr);
+    }
+    LFX_SYNC();
+  }
+}
+
+// Forward tile: input band `in` (length m0) -> d_r to out[(ns_r + i) * o_ax]
+// for r = 1..k, s_k to sout[i * so_ax].
+template <class C, class SI, class SO, class SK>
+__device__ void tile_fwd_body(const DProgram<C>* prog, int bc, long long m0, int k, long long T, long long H,
+                              long long tile, const SI* in, long long i_ax, SO* out, long long o_ax, SK* sout,
+                              long long so_ax, C* P, C* Q) {
+  const int tid = LFX_TID, nthr = LFX_NTHR;
+  const long long a = tile * T, b = lmin(a + T, m0);
+  const long long wa = lmax(0, a - H), wb = lmin(m0, b + H);
+  long long g0 = wa / 2;
+  int sw = (int)((wb + 1) / 2 - g0), dw = (int)(wb / 2 - g0);
+  for (long long i = tid; i < wb - wa; i += nthr) {
+    const long long gi = wa + i;
+    P[(gi & 1) ? sw + (gi >> 1) - g0 : (gi >> 1) - g0] = load_as<SI, C>(in, gi * i_ax);
+  }
+  LFX_SYNC();
+  Ranges r{g0, g0 + sw, g0, g0 + dw};
+  C* cur = P;
+  C* oth = Q;
+  long long m = m0;
+  for (int lv = 1; lv <= k; ++lv) {
+    const long long nd = m / 2, ns = m - nd;
+    const Band<C> s{cur, g0, sw}, d{cur + sw, g0, dw};
+    tile_level_steps<C>(prog, bc, s, d, ns, nd, r, tid, nthr);
+    const long long ad = a >> lv, bd = (b >= m0) ? nd : (b >> lv);
+    for (long long i = ad + tid; i < bd; i += nthr) store_as<SO, C>(out, (ns + i) * o_ax, cur[sw + (i - g0)]);
+    if (lv < k) {
+      const long long ng0 = g0 / 2;
+      const int nsw = (int)((g0 + sw + 1) / 2 - ng0), ndw = (int)((g0 + sw) / 2 - ng0);
+      for (long long i = tid; i < sw; i += nthr) {
+        const long long gi = g0 + i;
+        oth[(gi & 1) ? nsw + (gi >> 1) - ng0 : (gi >> 1) - ng0] = cur[i];
+      }
+      r = Ranges{(r.slo + 1) / 2, (r.shi + 1) / 2, r.slo / 2, r.shi / 2};
+      LFX_SYNC();
+      C* t = cur;
+      cur = oth;
+      oth = t;
+      g0 = ng0;
+      sw = nsw;
+      dw = ndw;
+    } else {
+      const long long bs = (b >= m0) ? ns : (b >> lv);
+      for (long long i = ad + tid; i < bs; i += nthr) store_as<SK, C>(sout, i * so_ax, cur[i - g0]);
+    }
+    m = ns;
+  }
+}
+
+// Reverse tile: s_k from sin[i * si_ax], d_r from in[(ns_r + i) * i_ax]
+// -> tile [a, b) of the output band (length m0) to out[q * o_ax].
+template <class C, class SI, class SK, class SO>
+__device__ void tile_rev_body(const DProgram<C>* prog, int bc, long long m0, int k, long long T, long long Hk,
+                              long long tile, const SI* in, long long i_ax, const SK* sin, long long si_ax,
+                              SO* out, long long o_ax, C* P, C* Q) {
+  const int tid = LFX_TID, nthr = LFX_NTHR;
+  const long long a = tile * T, b = lmin(a + T, m0);
+  // level r lives in buf[r & 1]: level 1 (largest window) in P
+  C* buf[2] = {Q, P};
+  long long m = band_len(m0, k - 1);
+  long long ns = m - m / 2, nd = m / 2;
+  long long gg = lmax(0, (a >> k) - Hk);
+  const long long ghi = ((b + (1ll << k) - 1) >> k) + Hk;
+  int sw = (int)lmax(0, lmin(ghi, ns) - gg), dw = (int)lmax(0, lmin(ghi, nd) - gg);
+  C* cur = buf[k & 1];
+  for (long long i = tid; i < sw; i += nthr) cur[i] = load_as<SK, C>(sin, (gg + i) * si_ax);
+  for (long long i = tid; i < dw; i += nthr) cur[sw + i] = load_as<SI, C>(in, (ns + gg + i) * i_ax);
+  Ranges r{gg, gg + sw, gg, gg + dw};
+  for (int lv = k; lv >= 1; --lv) {
+    LFX_SYNC();
+    const Band<C> s{cur, gg, sw}, d{cur + sw, gg, dw};
+    tile_level_steps<C>(prog, bc, s, d, ns, nd, r, tid, nthr);
+    const long long vlo = lmax(2 * r.slo, 2 * r.dlo + 1) - 1;
+    const long long vhi = lmin(2 * r.shi, 2 * r.dhi + 1);
+    const long long wlen = lmin(2 * (long long)sw, 2 * (long long)dw + 1);
+    const long long lo = lmax(vlo, 2 * gg), hi = lmin(vhi, 2 * gg + wlen);
+    if (lv == 1) {
+      for (long long q = a + tid; q < b; q += nthr) {
+        const long long j = (q >> 1) - gg;
+        store_as<SO, C>(out, q * o_ax, (q & 1) ? cur[sw + j] : cur[j]);
+      }
+      break;
+    }
+    C* nxt = buf[(lv - 1) & 1];
+    for (long long q = tid; q < wlen; q += nthr) nxt[q] = (q & 1) ? cur[sw + (q >> 1)] : cur[q >> 1];
+    m = band_len(m0, lv - 2);
+    ns = m - m / 2;
+    nd = m / 2;
+    gg = 2 * gg;
+    sw = (int)wlen;
+    dw = (int)lmax(0, lmin(gg + sw, nd) - gg);
+    for (long long i = tid; i < dw; i += nthr) nxt[sw + i] = load_as<SI, C>(in, (ns + gg + i) * i_ax);
+    r = Ranges{lo, hi, gg, gg + dw};
+    cur = nxt;
+  }
+}
+
+}  // namespace lfx

@@ -1,0 +1,90 @@
+// Types shared by the host planner, the kernels and the CPU simulation build
+// (no HIP dependency).
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+namespace lfx {
+
+enum Storage : int32_t { kF32 = 0, kF64 = 1, kBF16 = 2, kF16 = 3 };
+
+inline size_t storage_bytes(int s) { return s == kF64 ? 8 : (s == kF32 ? 4 : 2); }
+inline size_t compute_bytes(int s) { return s == kF64 ? 8 : 4; }
+
+constexpr int kThreads = 256;
+constexpr int kMaxDims = 8;
+// LDS one workgroup may use (gfx950: 160 KiB per CU)
+constexpr size_t kMaxLdsBytes = 160 * 1024;
+
+// Base offsets of a line in three arrays: the line index is decomposed over
+// the non-transformed dims (last fastest); str[a][k] is array a's stride of
+// dim k, ax[a] its element stride along the transform axis.
+struct LineMap3 {
+  int32_t nod;
+  int32_t pad;
+  long long shape[kMaxDims];
+  long long str[3][kMaxDims];
+  long long ax[3];
+};
+
+struct CopyMap {
+  int32_t nd;
+  int32_t pad;
+  long long shape[kMaxDims];
+  long long sst[kMaxDims];
+  long long dstr[kMaxDims];
+};
+
+// LDS bytes of the whole-line kernels for a line of n samples
+inline size_t line_lds_bytes(long long n, size_t csize) { return (size_t)(n + (n - n / 2)) * csize; }
+
+// Whole-line launch: arrays 0 (input), 1 (output), 2 (reverse: s_L source).
+struct LineLaunch {
+  int storage;
+  bool fwd;
+  const void* prog;
+  int bc;
+  long long n;  // band length the launch starts from
+  int L;        // levels
+  const void* a0;
+  void* a1;
+  const void* a2;
+  bool scr[3];  // array holds compute-type scratch instead of storage type
+  LineMap3 lm;
+  long long nlines;
+};
+
+struct TileArgs {
+  long long m0;      // band length at the pass's finest level
+  long long T;       // tile length at the finest level (multiple of 2^k)
+  long long H;       // forward: halo (finest level units); reverse: halo at level k
+  long long ntiles;  // tiles per line
+  int k;             // levels in the pass
+  int pad;
+};
+
+// Reverse tile: LDS window (elements) of the finest level.
+inline long long tile_rev_window(const TileArgs& t) { return (1ll << t.k) * ((t.T >> t.k) + 2 + 2 * t.H); }
+inline long long tile_fwd_window(const TileArgs& t) { return t.T + 2 * t.H + 2; }
+inline size_t tile_lds_bytes(const TileArgs& t, bool fwd, size_t csize) {
+  const long long W = fwd ? tile_fwd_window(t) : tile_rev_window(t);
+  return (size_t)(W + W / 2 + 2) * csize;
+}
+
+// Tiled launch. Forward: arrays 0 input band, 1 output (d regions), 2 s_k out.
+// Reverse: 0 input (d regions), 1 output band, 2 s_k in.
+struct TileLaunch {
+  int storage;
+  bool fwd;
+  const void* prog;
+  int bc;
+  TileArgs ta;
+  const void* a0;
+  void* a1;
+  void* a2;
+  bool scr[3];
+  LineMap3 lm;
+  long long nlines;
+};
+
+}  // namespace lfx

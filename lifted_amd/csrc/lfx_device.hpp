@@ -22,8 +22,15 @@
 // Accumulation: y = fma(c[k], x[k], y) for k = 0..N-1 (steps:97-102); Repeat
 // sums its taps first and applies one fma (steps:116-122); Unit is y +/- x.
 #pragma once
-#include <hip/hip_runtime.h>
 #include <cstdint>
+#ifdef LFX_CPU_SIM
+#include "lfx_cpu_sim.hpp"
+#else
+#include <hip/hip_runtime.h>
+#define LFX_TID ((int)threadIdx.x)
+#define LFX_NTHR ((int)blockDim.x)
+#define LFX_SYNC __syncthreads
+#endif
 
 #include "lfx_program.hpp"
 
@@ -253,7 +260,7 @@ __device__ void level_steps(const DProgram<C>* __restrict__ prog, int bc, const 
       const long long yhi = ylo + (st.y_is_d ? d.len : s.len);
       lift_step<C>(st, cf, bc, s, d, ns, nd, ylo, yhi, tid, nthr);
     }
-    __syncthreads();
+    LFX_SYNC();
   }
 }
 
@@ -267,12 +274,14 @@ template <>
 __device__ __forceinline__ float load_as<bf16_t, float>(const bf16_t* p, long long i) {
   return __uint_as_float(((uint32_t)p[i].v) << 16);
 }
+#ifndef LFX_CPU_SIM
 template <>
 __device__ __forceinline__ float load_as<f16_t, float>(const f16_t* p, long long i) {
   _Float16 h;
   __builtin_memcpy(&h, &p[i].v, 2);
   return (float)h;
 }
+#endif
 
 template <class S, class C>
 __device__ __forceinline__ void store_as(S* p, long long i, C v) { p[i] = (S)v; }
@@ -286,11 +295,13 @@ __device__ __forceinline__ void store_as<bf16_t, float>(bf16_t* p, long long i, 
     p[i].v = (uint16_t)(u >> 16);
   }
 }
+#ifndef LFX_CPU_SIM
 template <>
 __device__ __forceinline__ void store_as<f16_t, float>(f16_t* p, long long i, float v) {
   _Float16 h = (_Float16)v;
   __builtin_memcpy(&p[i].v, &h, 2);
 }
+#endif
 
 }  // namespace dev
 }  // namespace lfx

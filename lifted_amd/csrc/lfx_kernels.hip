@@ -1,113 +1,90 @@
-// gfx950 kernels of the lifting DWT.
-//
-// k_line_fwd / k_line_rev: one workgroup owns one whole 1-D line, stages it in
-// LDS and runs every level there (the reference's per-line multi-level drivers,
-// src/lifted-drivers-inl.hpp:75-274; single-level when L == 1, :144-179 /
-// :276-311). The line is read once and written once; intermediate bands never
-// touch HBM.
-//   forward-stepping (Forward, InverseAdjoint): deinterleave on load, all steps,
-//   write d_l to out[ns_l, ns_l + nd_l), deinterleave s_l in LDS for the next
-//   level, finally write s_L to out[0, ns_L)  ->  [s_L | d_L | ... | d_1].
-//   reverse-stepping (Inverse, ForwardAdjoint): read s_L, then per level read
-//   d_l, run the steps, interleave into the next level's s; the finest level
-//   is interleaved straight into the output.
-// LDS: a band buffer of n elements plus one of ceil(n/2) (levels alternate).
+// gfx950 kernels of the lifting DWT: thin __global__ wrappers around the
+// bodies in lfx_bodies.hpp (blockIdx -> line / tile, line base offsets from a
+// LineMap3) and their launchers.
 #include <hip/hip_runtime.h>
 
-#include "lfx_device.hpp"
+#include <mutex>
+#include <set>
+#include <type_traits>
+#include <utility>
+
+#include "lfx_bodies.hpp"
 #include "lfx_kernels.hpp"
 
 namespace lfx {
 
-using dev::Band;
 using dev::bf16_t;
 using dev::f16_t;
-using dev::load_as;
-using dev::store_as;
 
-__device__ __forceinline__ void line_offsets(const LineMap& m, long long line, long long& so, long long& dof) {
-  so = 0;
-  dof = 0;
+// ---------------------------------------------------------------------------
+// ND wrappers: blockIdx -> (line, tile); a LineMap3 gives the line's base
+// offsets in the three arrays.
+
+__device__ __forceinline__ void line_bases(const LineMap3& m, long long line, long long* off) {
+  off[0] = off[1] = off[2] = 0;
   for (int k = m.nod - 1; k >= 0; --k) {
     const long long q = line % m.shape[k];
     line /= m.shape[k];
-    so += q * m.sst[k];
-    dof += q * m.dstr[k];
+    off[0] += q * m.str[0][k];
+    off[1] += q * m.str[1][k];
+    off[2] += q * m.str[2][k];
   }
 }
 
-__device__ __forceinline__ long long level_len(long long n, int l) { return (n + (1ll << l) - 1) >> l; }
-
-template <class C, class S>
-__global__ void __launch_bounds__(kLineThreads) k_line_fwd(const DProgram<C>* prog, int bc, long long n, int L,
-                                                           const S* src, S* dst, LineMap lm, long long line0) {
+template <class C>
+__device__ __forceinline__ C* lds_base() {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  C* P = reinterpret_cast<C*>(smem_raw);
-  C* Q = P + n;
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  long long so, dof;
-  line_offsets(lm, line0 + blockIdx.x, so, dof);
-  const S* in = src + so;
-  S* out = dst + dof;
-  const long long ns0 = n - n / 2;
-  for (long long i = tid; i < n; i += nthr) P[(i & 1) ? ns0 + (i >> 1) : (i >> 1)] = load_as<S, C>(in, i * lm.s_ax);
-  __syncthreads();
-  C* cur = P;
-  C* oth = Q;
-  long long m = n;
-  for (int l = 0; l < L; ++l) {
-    const long long nd = m / 2, ns = m - nd;
-    const Band<C> s{cur, 0, (int)ns}, d{cur + ns, 0, (int)nd};
-    dev::level_steps<C>(prog, bc, s, d, ns, nd, tid, nthr);
-    for (long long i = tid; i < nd; i += nthr) store_as<S, C>(out, (ns + i) * lm.d_ax, cur[ns + i]);
-    if (l + 1 < L) {
-      const long long ns2 = ns - ns / 2;
-      for (long long i = tid; i < ns; i += nthr) oth[(i & 1) ? ns2 + (i >> 1) : (i >> 1)] = cur[i];
-      __syncthreads();
-      C* t = cur;
-      cur = oth;
-      oth = t;
-    }
-    m = ns;
-  }
-  for (long long i = tid; i < m; i += nthr) store_as<S, C>(out, i * lm.d_ax, cur[i]);
+  return reinterpret_cast<C*>(smem_raw);
 }
 
-template <class C, class S>
-__global__ void __launch_bounds__(kLineThreads) k_line_rev(const DProgram<C>* prog, int bc, long long n, int L,
-                                                           const S* src, S* dst, LineMap lm, long long line0) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  C* buf[2];
-  buf[0] = reinterpret_cast<C*>(smem_raw);
-  buf[1] = buf[0] + n;
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  long long so, dof;
-  line_offsets(lm, line0 + blockIdx.x, so, dof);
-  const S* in = src + so;
-  S* out = dst + dof;
-  C* cur = buf[(L - 1) & 1];
-  {
-    const long long mL = level_len(n, L - 1);
-    const long long nsL = mL - mL / 2;
-    for (long long i = tid; i < nsL; i += nthr) cur[i] = load_as<S, C>(in, i * lm.s_ax);
-  }
-  for (int l = L - 1; l >= 0; --l) {
-    const long long m = level_len(n, l);
-    const long long nd = m / 2, ns = m - nd;
-    for (long long i = tid; i < nd; i += nthr) cur[ns + i] = load_as<S, C>(in, (ns + i) * lm.s_ax);
-    __syncthreads();
-    const Band<C> s{cur, 0, (int)ns}, d{cur + ns, 0, (int)nd};
-    dev::level_steps<C>(prog, bc, s, d, ns, nd, tid, nthr);
-    if (l > 0) {
-      C* nxt = buf[(l - 1) & 1];
-      for (long long j = tid; j < m; j += nthr) nxt[j] = (j & 1) ? cur[ns + (j >> 1)] : cur[j >> 1];
-      __syncthreads();
-      cur = nxt;
-    } else {
-      for (long long j = tid; j < m; j += nthr)
-        store_as<S, C>(out, j * lm.d_ax, (j & 1) ? cur[ns + (j >> 1)] : cur[j >> 1]);
-    }
-  }
+// array 0: input, array 1: output, array 2: separate s_L source (reverse only)
+template <class C, class SI, class SO>
+__global__ void __launch_bounds__(kThreads) k_line_fwd(const DProgram<C>* prog, int bc, long long n, int L,
+                                                       const SI* src, SO* dst, LineMap3 lm, long long cap0,
+                                                       long long line0) {
+  long long off[3];
+  line_bases(lm, line0 + blockIdx.x, off);
+  C* P = lds_base<C>();
+  line_fwd_body<C, SI, SO>(prog, bc, n, L, src + off[0], lm.ax[0], dst + off[1], lm.ax[1], P, P + cap0);
+}
+
+template <class C, class SI, class SS, class SO>
+__global__ void __launch_bounds__(kThreads) k_line_rev(const DProgram<C>* prog, int bc, long long n, int L,
+                                                       const SI* src, const SS* ssrc, SO* dst, LineMap3 lm,
+                                                       long long cap0, long long line0) {
+  long long off[3];
+  line_bases(lm, line0 + blockIdx.x, off);
+  C* P = lds_base<C>();
+  line_rev_body<C, SI, SS, SO>(prog, bc, n, L, src + off[0], lm.ax[0], ssrc + off[2], lm.ax[2], dst + off[1],
+                               lm.ax[1], P, P + cap0);
+}
+
+// forward tiles: array 0 = input band, 1 = output (d regions), 2 = s_k out
+template <class C, class SI, class SO, class SK>
+__global__ void __launch_bounds__(kThreads) k_tile_fwd(const DProgram<C>* prog, int bc, TileArgs ta, const SI* src,
+                                                       SO* dst, SK* sdst, LineMap3 lm, long long cap0,
+                                                       long long blk0) {
+  const long long blk = blk0 + blockIdx.x;
+  const long long line = blk / ta.ntiles, tile = blk % ta.ntiles;
+  long long off[3];
+  line_bases(lm, line, off);
+  C* P = lds_base<C>();
+  tile_fwd_body<C, SI, SO, SK>(prog, bc, ta.m0, ta.k, ta.T, ta.H, tile, src + off[0], lm.ax[0], dst + off[1],
+                               lm.ax[1], sdst + off[2], lm.ax[2], P, P + cap0);
+}
+
+// reverse tiles: array 0 = input (d regions), 1 = output band, 2 = s_k in
+template <class C, class SI, class SK, class SO>
+__global__ void __launch_bounds__(kThreads) k_tile_rev(const DProgram<C>* prog, int bc, TileArgs ta, const SI* src,
+                                                       const SK* ssrc, SO* dst, LineMap3 lm, long long cap0,
+                                                       long long blk0) {
+  const long long blk = blk0 + blockIdx.x;
+  const long long line = blk / ta.ntiles, tile = blk % ta.ntiles;
+  long long off[3];
+  line_bases(lm, line, off);
+  C* P = lds_base<C>();
+  tile_rev_body<C, SI, SK, SO>(prog, bc, ta.m0, ta.k, ta.T, ta.H, tile, src + off[0], lm.ax[0], ssrc + off[2],
+                               lm.ax[2], dst + off[1], lm.ax[1], P, P + cap0);
 }
 
 // Strided elementwise copy over an ND box (used by the level-by-level inverse,
@@ -130,39 +107,167 @@ __global__ void k_copy_nd(const S* src, S* dst, CopyMap cm, long long total) {
 // ---------------------------------------------------------------------------
 // launchers
 
-template <class C, class S>
-static hipError_t launch_line_t(bool fwd, const void* prog, int bc, long long n, int L, const void* src, void* dst,
-                                const LineMap& lm, long long nlines, hipStream_t st) {
-  const size_t lds = line_lds_bytes(n, sizeof(C));
-  auto kf = fwd ? k_line_fwd<C, S> : k_line_rev<C, S>;
-  static bool attr_set[2] = {false, false};
-  if (!attr_set[fwd ? 1 : 0]) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kf),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLdsBytes);
-    if (e != hipSuccess) return e;
-    attr_set[fwd ? 1 : 0] = true;
-  }
+template <class K>
+static hipError_t ensure_lds(K kern) {
+  // raise the dynamic-LDS limit once per kernel (and device)
+  static std::mutex mu;
+  static std::set<std::pair<int, const void*>> done;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const void* key = reinterpret_cast<const void*>(kern);
+  std::lock_guard<std::mutex> lk(mu);
+  if (done.count({dev, key})) return hipSuccess;
+  e = hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLdsBytes);
+  if (e == hipSuccess) done.insert({dev, key});
+  return e;
+}
+
+template <class F>
+static hipError_t grid_loop(long long nblocks, F&& launch) {
   const long long kMaxGrid = 1ll << 30;
-  for (long long l0 = 0; l0 < nlines; l0 += kMaxGrid) {
-    const long long cnt = nlines - l0 < kMaxGrid ? nlines - l0 : kMaxGrid;
-    hipLaunchKernelGGL(kf, dim3((unsigned)cnt), dim3(kLineThreads), lds, st,
-                       reinterpret_cast<const DProgram<C>*>(prog), bc, n, L, reinterpret_cast<const S*>(src),
-                       reinterpret_cast<S*>(dst), lm, l0);
+  for (long long b0 = 0; b0 < nblocks; b0 += kMaxGrid) {
+    const long long cnt = nblocks - b0 < kMaxGrid ? nblocks - b0 : kMaxGrid;
+    launch(b0, (unsigned)cnt);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
 }
 
-hipError_t launch_line(int storage, bool fwd, const void* prog, int bc, long long n, int L, const void* src,
-                       void* dst, const LineMap& lm, long long nlines, hipStream_t st) {
-  switch (storage) {
-    case kF32: return launch_line_t<float, float>(fwd, prog, bc, n, L, src, dst, lm, nlines, st);
-    case kF64: return launch_line_t<double, double>(fwd, prog, bc, n, L, src, dst, lm, nlines, st);
-    case kBF16: return launch_line_t<float, bf16_t>(fwd, prog, bc, n, L, src, dst, lm, nlines, st);
-    case kF16: return launch_line_t<float, f16_t>(fwd, prog, bc, n, L, src, dst, lm, nlines, st);
-  }
+// storage enum -> element type of the storage arrays
+template <int S> struct StoreT;
+template <> struct StoreT<kF32> { using T = float; using C = float; };
+template <> struct StoreT<kF64> { using T = double; using C = double; };
+template <> struct StoreT<kBF16> { using T = bf16_t; using C = float; };
+template <> struct StoreT<kF16> { using T = f16_t; using C = float; };
+
+// Array kinds of a launch: storage-typed (user arrays) or compute-typed (scratch).
+template <int S, bool SCRATCH>
+using ArrT = typename std::conditional<SCRATCH, typename StoreT<S>::C, typename StoreT<S>::T>::type;
+
+template <int S, bool I_SCR, bool O_SCR>
+static hipError_t line_fwd_t(const LineLaunch& ll, hipStream_t st) {
+  using C = typename StoreT<S>::C;
+  using SI = ArrT<S, I_SCR>;
+  using SO = ArrT<S, O_SCR>;
+  auto kf = k_line_fwd<C, SI, SO>;
+  hipError_t e = ensure_lds(kf);
+  if (e != hipSuccess) return e;
+  const long long cap0 = ll.n;
+  const size_t lds = line_lds_bytes(ll.n, sizeof(C));
+  return grid_loop(ll.nlines, [&](long long b0, unsigned cnt) {
+    hipLaunchKernelGGL(kf, dim3(cnt), dim3(kThreads), lds, st, reinterpret_cast<const DProgram<C>*>(ll.prog), ll.bc,
+                       ll.n, ll.L, reinterpret_cast<const SI*>(ll.a0), reinterpret_cast<SO*>(ll.a1), ll.lm, cap0, b0);
+  });
+}
+
+template <int S, bool I_SCR, bool S_SCR, bool O_SCR>
+static hipError_t line_rev_t(const LineLaunch& ll, hipStream_t st) {
+  using C = typename StoreT<S>::C;
+  using SI = ArrT<S, I_SCR>;
+  using SS = ArrT<S, S_SCR>;
+  using SO = ArrT<S, O_SCR>;
+  auto kf = k_line_rev<C, SI, SS, SO>;
+  hipError_t e = ensure_lds(kf);
+  if (e != hipSuccess) return e;
+  const long long cap0 = ll.n;
+  const size_t lds = line_lds_bytes(ll.n, sizeof(C));
+  return grid_loop(ll.nlines, [&](long long b0, unsigned cnt) {
+    hipLaunchKernelGGL(kf, dim3(cnt), dim3(kThreads), lds, st, reinterpret_cast<const DProgram<C>*>(ll.prog), ll.bc,
+                       ll.n, ll.L, reinterpret_cast<const SI*>(ll.a0), reinterpret_cast<const SS*>(ll.a2),
+                       reinterpret_cast<SO*>(ll.a1), ll.lm, cap0, b0);
+  });
+}
+
+template <int S, bool I_SCR, bool O_SCR, bool K_SCR>
+static hipError_t tile_fwd_t(const TileLaunch& tl, hipStream_t st) {
+  using C = typename StoreT<S>::C;
+  using SI = ArrT<S, I_SCR>;
+  using SO = ArrT<S, O_SCR>;
+  using SK = ArrT<S, K_SCR>;
+  auto kf = k_tile_fwd<C, SI, SO, SK>;
+  hipError_t e = ensure_lds(kf);
+  if (e != hipSuccess) return e;
+  const long long cap0 = tile_fwd_window(tl.ta);
+  const size_t lds = tile_lds_bytes(tl.ta, true, sizeof(C));
+  return grid_loop(tl.nlines * tl.ta.ntiles, [&](long long b0, unsigned cnt) {
+    hipLaunchKernelGGL(kf, dim3(cnt), dim3(kThreads), lds, st, reinterpret_cast<const DProgram<C>*>(tl.prog), tl.bc,
+                       tl.ta, reinterpret_cast<const SI*>(tl.a0), reinterpret_cast<SO*>(tl.a1),
+                       reinterpret_cast<SK*>(tl.a2), tl.lm, cap0, b0);
+  });
+}
+
+template <int S, bool I_SCR, bool K_SCR, bool O_SCR>
+static hipError_t tile_rev_t(const TileLaunch& tl, hipStream_t st) {
+  using C = typename StoreT<S>::C;
+  using SI = ArrT<S, I_SCR>;
+  using SK = ArrT<S, K_SCR>;
+  using SO = ArrT<S, O_SCR>;
+  auto kf = k_tile_rev<C, SI, SK, SO>;
+  hipError_t e = ensure_lds(kf);
+  if (e != hipSuccess) return e;
+  const long long cap0 = tile_rev_window(tl.ta);
+  const size_t lds = tile_lds_bytes(tl.ta, false, sizeof(C));
+  return grid_loop(tl.nlines * tl.ta.ntiles, [&](long long b0, unsigned cnt) {
+    hipLaunchKernelGGL(kf, dim3(cnt), dim3(kThreads), lds, st, reinterpret_cast<const DProgram<C>*>(tl.prog), tl.bc,
+                       tl.ta, reinterpret_cast<const SI*>(tl.a0), reinterpret_cast<const SK*>(tl.a2),
+                       reinterpret_cast<SO*>(tl.a1), tl.lm, cap0, b0);
+  });
+}
+
+#define LFX_DISPATCH_STORAGE(S, CALL) \
+  switch (S) {                        \
+    case kF32: return CALL(kF32);     \
+    case kF64: return CALL(kF64);     \
+    case kBF16: return CALL(kBF16);   \
+    case kF16: return CALL(kF16);     \
+  }                                   \
   return hipErrorInvalidValue;
+
+hipError_t launch_line(const LineLaunch& ll, hipStream_t st) {
+  const int code = (ll.scr[0] ? 1 : 0) | (ll.scr[1] ? 2 : 0) | (ll.scr[2] ? 4 : 0);
+  if (ll.fwd) {
+#define CALL_F(S)                                          \
+  (code & 3) == 0 ? line_fwd_t<S, false, false>(ll, st)    \
+  : (code & 3) == 1 ? line_fwd_t<S, true, false>(ll, st)   \
+  : (code & 3) == 2 ? line_fwd_t<S, false, true>(ll, st)   \
+                    : line_fwd_t<S, true, true>(ll, st)
+    LFX_DISPATCH_STORAGE(ll.storage, CALL_F)
+#undef CALL_F
+  }
+#define CALL_R(S)                                                   \
+  code == 0 ? line_rev_t<S, false, false, false>(ll, st)            \
+  : code == 4 ? line_rev_t<S, false, true, false>(ll, st)           \
+  : code == 2 ? line_rev_t<S, false, false, true>(ll, st)           \
+  : code == 6 ? line_rev_t<S, false, true, true>(ll, st)            \
+              : hipErrorInvalidValue
+  LFX_DISPATCH_STORAGE(ll.storage, CALL_R)
+#undef CALL_R
+}
+
+hipError_t launch_tile(const TileLaunch& tl, hipStream_t st) {
+  const int code = (tl.scr[0] ? 1 : 0) | (tl.scr[1] ? 2 : 0) | (tl.scr[2] ? 4 : 0);
+  if (tl.fwd) {
+    // input: user or scratch; output: user; s_k: user or scratch
+#define CALL_F(S)                                                    \
+  code == 0 ? tile_fwd_t<S, false, false, false>(tl, st)             \
+  : code == 1 ? tile_fwd_t<S, true, false, false>(tl, st)            \
+  : code == 4 ? tile_fwd_t<S, false, false, true>(tl, st)            \
+  : code == 5 ? tile_fwd_t<S, true, false, true>(tl, st)             \
+              : hipErrorInvalidValue
+    LFX_DISPATCH_STORAGE(tl.storage, CALL_F)
+#undef CALL_F
+  }
+  // input: user; output band: user or scratch; s_k in: user or scratch
+#define CALL_R(S)                                                    \
+  code == 0 ? tile_rev_t<S, false, false, false>(tl, st)             \
+  : code == 2 ? tile_rev_t<S, false, false, true>(tl, st)            \
+  : code == 4 ? tile_rev_t<S, false, true, false>(tl, st)            \
+  : code == 6 ? tile_rev_t<S, false, true, true>(tl, st)             \
+              : hipErrorInvalidValue
+  LFX_DISPATCH_STORAGE(tl.storage, CALL_R)
+#undef CALL_R
 }
 
 hipError_t launch_copy(int storage, const void* src, void* dst, const CopyMap& cm, long long total,

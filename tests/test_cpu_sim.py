@@ -1,0 +1,95 @@
+"""The GPU kernel bodies compiled for the host (one lane, AddressSanitizer on
+exact-size LDS buffers) against the oracle.
+
+This checks the tiling logic of long lines -- windows, halos, valid-range
+tracking, boundary folds at tile edges, multi-pass schedules -- without a GPU:
+the line-length limit is lowered so that short lines take the tiled path.
+Results must be bit-identical to the oracle (same operations, same order).
+"""
+import itertools
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import MOCK_WAVELETS, WAVELETS, WIDTHS, oracle_steps
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SIM = os.path.join(HERE, "cpu_sim", "build", "lfx_sim")
+
+
+@pytest.fixture(scope="module")
+def sim():
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "cpu_sim")], check=True)
+    return SIM
+
+
+def run_sim(sim, w, tf, bc, x, L, limit, tile, k, steps=None):
+    dt = 1 if x.dtype == np.float64 else 0
+    args = [sim, "custom" if steps else str(w), str(tf), str(bc), str(dt), str(len(x)), str(L), str(limit),
+            str(tile), str(k)]
+    if steps:
+        args.append(steps)
+    r = subprocess.run(args, input=x.tobytes(), capture_output=True)
+    assert r.returncode == 0, r.stderr.decode()[-3000:]
+    return np.frombuffer(r.stdout, dtype=x.dtype), r.stderr.decode()
+
+
+def ok_bands(bc, n, L):
+    m = n
+    for _ in range(L):
+        nd, ns = m // 2, m - m // 2
+        if bc != 0 and (nd == 0 or (bc == 4 and (nd < 2 or ns < 2))):
+            return False
+        m = ns
+    return True
+
+
+@pytest.mark.parametrize("w", [w for w in WAVELETS if w != 0])
+def test_tiled_all_wavelets(sim, oracle, w):
+    g = np.random.default_rng(w)
+    for bc, tf, n in itertools.product((0, 1, 3, 4), (0, 1, 2, 3), (3001, 2048)):
+        L = min(7, max(1, oracle.max_level(w, n)))
+        if not ok_bands(bc, n, L):
+            continue
+        x = g.standard_normal(n)
+        ref = oracle.lwt(x, w, tf, bc, level=[L])
+        got, info = run_sim(sim, w, tf, bc, x, L, limit=200, tile=512, k=0)
+        assert "pass" in info, info
+        np.testing.assert_array_equal(got, ref, err_msg=f"w={w} bc={bc} tf={tf} n={n}\n{info}")
+
+
+@pytest.mark.parametrize("tile,k,limit", [(128, 1, 100), (256, 2, 60), (1024, 4, 300), (64, 0, 50)])
+def test_tiled_shapes_cdf97_float32(sim, oracle, tile, k, limit):
+    g = np.random.default_rng(tile + k)
+    for bc, tf, n in itertools.product((0, 1, 3, 4), (0, 1, 2, 3), (1777, 4096, 999)):
+        L = 6
+        x = g.standard_normal(n).astype(np.float32)
+        ref = oracle.lwt(x, 100907, tf, bc, level=[L])
+        got, info = run_sim(sim, 100907, tf, bc, x, L, limit=limit, tile=tile, k=k)
+        np.testing.assert_array_equal(got, ref, err_msg=f"bc={bc} tf={tf} n={n}\n{info}")
+
+
+def _spec(steps):
+    parts = []
+    for s in oracle_steps(steps):
+        tgt = 1 if s["target"] in ("d", 1) else 0
+        taps = s.get("taps", [])
+        parts.append(",".join(str(v) for v in [s["kind"], tgt, s["offset"], s.get("sign", 1), s["ntaps"]] +
+                              [repr(float(t)) for t in taps]))
+    return ";".join(parts)
+
+
+@pytest.mark.parametrize("name", list(MOCK_WAVELETS))
+def test_tiled_mock_wavelets(sim, oracle, name):
+    """The reference's mock step programs (large positive / negative offsets
+    exercise folds that land next to tile edges)."""
+    st = oracle_steps(MOCK_WAVELETS[name])
+    spec = _spec(MOCK_WAVELETS[name])
+    g = np.random.default_rng(5)
+    for bc, tf, n in itertools.product((0, 1, 3, 4), (0, 1, 2, 3), (1531, 1024)):
+        x = g.standard_normal(n)
+        ref = oracle.lwt(x, 0, tf, bc, level=[3], steps=st)
+        got, info = run_sim(sim, 0, tf, bc, x, 3, limit=100, tile=256, k=0, steps=spec)
+        np.testing.assert_array_equal(got, ref, err_msg=f"{name} bc={bc} tf={tf} n={n}\n{info}")
