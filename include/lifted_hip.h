@@ -109,6 +109,38 @@ int lfx_program_create(const lfx_step* steps, int32_t nsteps, uint32_t width, ui
  * fold_off, c0..c15] (25 values). Returns the step count. */
 int lfx_program_dump(uint32_t wavelet, uint32_t transform, int32_t storage, double* out, int32_t cap_steps);
 
+/* ---- batched transforms of many 1-D signals ----
+ * Signals are packed back to back (any order) in one input and one output
+ * array of the plan's storage type; each has its own length, wavelet and
+ * level count. A plan fixes the transform type, boundary condition and the
+ * signal list, owns its device scratch, and runs as a few stage launches that
+ * can be captured in a hipGraph. This is the reference's per-line driver
+ * (src/lifted-drivers-inl.hpp:75-274) applied to many lines at once; the
+ * reference has no batched entry point. */
+typedef struct {
+  int64_t offset;   /* element offset of the signal in the packed arrays */
+  int64_t length;   /* samples */
+  uint32_t wavelet; /* lifted::Wavelet value or custom program id */
+  int32_t levels;   /* 0 = max_level */
+} lfx_signal;
+
+typedef struct lfx_batch_plan_s* lfx_batch_plan;
+
+int lfx_batch_plan_create(const lfx_signal* signals, int32_t nsignals, uint32_t transform, uint32_t bc,
+                          int32_t storage, uint32_t flags, lfx_batch_plan* plan);
+/* All stages, stream-ordered; in and out must not alias. */
+int lfx_batch_execute(lfx_batch_plan plan, const void* in_dev, void* out_dev, void* stream);
+int lfx_batch_num_stages(lfx_batch_plan plan);
+int lfx_batch_execute_stage(lfx_batch_plan plan, int32_t stage, const void* in_dev, void* out_dev, void* stream);
+/* Work items, algorithmic bytes (owned samples read + written) and LDS bytes of a stage. */
+int lfx_batch_stage_info(lfx_batch_plan plan, int32_t stage, int64_t* nitems, double* alg_bytes, int64_t* lds_bytes);
+void lfx_batch_plan_destroy(lfx_batch_plan plan);
+
+/* Tile length (samples at a pass's finest level, 0 = default) and levels per
+ * tiled pass (0 = default) used by plans and transforms created afterwards on
+ * this thread; set by the autotuner. */
+int lfx_set_tuning(int64_t tile, int32_t levels_per_pass);
+
 /* Message of the last error on this thread. */
 const char* lfx_last_error(void);
 

@@ -1,0 +1,105 @@
+"""Batched transforms of many 1-D signals of uneven length, wavelet and level
+count, packed back to back in one tensor (see lfx_batch_* in the C ABI)."""
+import ctypes
+
+from . import _lib
+from . import _wavelet_id, _storage_torch, LFX_FLAG_PLAIN
+
+
+class LfxSignal(ctypes.Structure):
+    _fields_ = [("offset", ctypes.c_int64), ("length", ctypes.c_int64), ("wavelet", ctypes.c_uint32),
+                ("levels", ctypes.c_int32)]
+
+
+def _bind():
+    L = _lib.lib()
+    if not hasattr(L, "_batch_bound"):
+        vp = ctypes.c_void_p
+        L.lfx_batch_plan_create.argtypes = [ctypes.POINTER(LfxSignal), ctypes.c_int32, ctypes.c_uint32,
+                                            ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32,
+                                            ctypes.POINTER(vp)]
+        L.lfx_batch_execute.argtypes = [vp, vp, vp, vp]
+        L.lfx_batch_num_stages.argtypes = [vp]
+        L.lfx_batch_execute_stage.argtypes = [vp, ctypes.c_int32, vp, vp, vp]
+        L.lfx_batch_stage_info.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64),
+                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
+        L.lfx_batch_plan_destroy.argtypes = [vp]
+        L.lfx_batch_plan_destroy.restype = None
+        L.lfx_set_tuning.argtypes = [ctypes.c_int64, ctypes.c_int32]
+        L._batch_bound = True
+    return L
+
+
+def set_tuning(tile=0, levels_per_pass=0):
+    """Tile length / levels per tiled pass for plans created afterwards (0 = default)."""
+    _lib.check(_bind().lfx_set_tuning(int(tile), int(levels_per_pass)))
+
+
+class BatchPlan:
+    """Plan for one transform type over a fixed list of packed signals.
+
+    signals: iterable of (offset, length, wavelet, levels); levels 0 = max level.
+    dtype: torch dtype of the packed arrays (float32, float64, bfloat16, float16).
+    """
+
+    def __init__(self, signals, transform, mode=0, dtype=None, plain=False):
+        import torch
+        L = _bind()
+        sigs = list(signals)
+        arr = (LfxSignal * max(len(sigs), 1))()
+        for i, (off, n, w, lv) in enumerate(sigs):
+            arr[i].offset, arr[i].length, arr[i].wavelet, arr[i].levels = int(off), int(n), _wavelet_id(w), int(lv)
+        self.dtype = dtype or torch.float32
+        storage = _storage_torch(self.dtype)
+        h = ctypes.c_void_p()
+        _lib.check(L.lfx_batch_plan_create(arr, len(sigs), int(transform), int(mode), storage,
+                                           LFX_FLAG_PLAIN if plain else 0, ctypes.byref(h)))
+        self._h = h
+        self.signals = sigs
+        self.total = max((o + n for o, n, _, _ in sigs), default=0)
+        self.num_stages = L.lfx_batch_num_stages(h)
+
+    def _check_args(self, x, out):
+        if x.dtype != self.dtype or out.dtype != self.dtype:
+            raise ValueError("array dtype does not match the plan")
+        if not (x.is_cuda and out.is_cuda and x.is_contiguous() and out.is_contiguous()):
+            raise ValueError("packed arrays must be contiguous device tensors")
+        if x.numel() < self.total or out.numel() < self.total:
+            raise ValueError("packed arrays are shorter than the signal list")
+        if x.data_ptr() == out.data_ptr():
+            raise ValueError("input and output must not alias")
+
+    def execute(self, x, out=None):
+        import torch
+        if out is None:
+            out = torch.empty_like(x)
+        self._check_args(x, out)
+        st = torch.cuda.current_stream(x.device).cuda_stream
+        _lib.check(_bind().lfx_batch_execute(self._h, x.data_ptr(), out.data_ptr(), st))
+        return out
+
+    def execute_stage(self, stage, x, out):
+        import torch
+        self._check_args(x, out)
+        st = torch.cuda.current_stream(x.device).cuda_stream
+        _lib.check(_bind().lfx_batch_execute_stage(self._h, int(stage), x.data_ptr(), out.data_ptr(), st))
+        return out
+
+    def stage_info(self, stage):
+        n = ctypes.c_int64()
+        b = ctypes.c_double()
+        l = ctypes.c_int64()
+        _lib.check(_bind().lfx_batch_stage_info(self._h, int(stage), ctypes.byref(n), ctypes.byref(b),
+                                                ctypes.byref(l)))
+        return {"items": n.value, "alg_bytes": b.value, "lds_bytes": l.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _bind().lfx_batch_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

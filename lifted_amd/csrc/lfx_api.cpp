@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -521,6 +522,244 @@ static int lwt_overload(uint32_t wavelet, uint32_t transform, uint32_t bc, size_
   });
 }
 
+// ---------------------------------------------------------------------------
+// batched plans: many 1-D signals of uneven length, wavelet and level count,
+// packed back to back in one input and one output array.
+//
+// Each signal's levels are cut into groups (tiled passes of k levels while the
+// band exceeds one workgroup, then one whole-line group; with LFX_FLAG_PLAIN
+// every group is a single level). Group g of every signal runs in stage g
+// (forward: fine -> coarse; reverse: coarse -> fine); a stage is one launch
+// whose items are tiles or whole lines of any signal, longest first.
+
+struct BatchStage {
+  long long item0 = 0, nitems = 0;
+  size_t lds = 0;
+  double alg_bytes = 0;  // bytes the stage must read and write (owned ranges)
+};
+
+struct BatchPlanImpl {
+  int dev = 0;
+  int tf = 0, bc = 0, storage = 0;
+  bool fwd = true;
+  long long total = 0;  // samples
+  std::vector<BatchStage> stages;
+  BatchItem* d_items = nullptr;
+  void* d_scr = nullptr;
+  size_t scr_bytes = 0;
+};
+
+struct Group {
+  bool tile;
+  long long m0;  // band length entering the group (fine side)
+  int k;         // levels
+  long long T, H, ntiles;
+};
+
+template <class C>
+static std::vector<Group> signal_groups(const WaveletDefL& w, int tf, int storage, int bc, long long n, long long L,
+                                        bool plain) {
+  const DProgram<C> hp = resolve<C>(w, tf);
+  const bool fwd = tf == kForward || tf == kInverseAdjoint;
+  std::vector<Group> g;
+  const LineSchedule sc = schedule(hp, fwd, n, L, storage, bc, g_tuning.tile, plain ? 1 : g_tuning.k);
+  for (auto& p : sc.passes) g.push_back(Group{true, p.m0, p.k, p.T, p.H, p.ntiles});
+  if (sc.L_line > 0) {
+    if (plain) {
+      long long m = sc.m_line;
+      for (long long l = 0; l < sc.L_line; ++l) {
+        g.push_back(Group{false, m, 1, 0, 0, 1});
+        m = band_len(m, 1);
+      }
+    } else {
+      g.push_back(Group{false, sc.m_line, (int)sc.L_line, 0, 0, 1});
+    }
+  }
+  return g;  // fine -> coarse
+}
+
+static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int bc, int storage, uint32_t flags) {
+  if (tf < 0 || tf > 3) fail(LFX_EINVAL, "unknown transform");
+  if (bc < 0 || bc > 4) fail(LFX_EINVAL, "Unknown Boundary Condition");
+  if (storage < 0 || storage > 3) fail(LFX_EINVAL, "unknown storage type");
+  if (nsig < 0 || (nsig > 0 && !sigs)) fail(LFX_EINVAL, "bad signal list");
+  auto plan = std::make_unique<BatchPlanImpl>();
+  hip_check(hipGetDevice(&plan->dev), "hipGetDevice");
+  plan->tf = tf;
+  plan->bc = bc;
+  plan->storage = storage;
+  plan->fwd = tf == kForward || tf == kInverseAdjoint;
+  const bool plain = (flags & LFX_FLAG_PLAIN) != 0;
+  const size_t es = storage_bytes(storage), cs = compute_bytes(storage);
+  std::vector<std::vector<BatchItem>> per_stage;
+  std::vector<std::vector<double>> cost;
+  std::vector<size_t> lds;
+  std::vector<double> bytes;
+  long long scr_elems = 0;
+  for (int i = 0; i < nsig; ++i) {
+    const lfx_signal& sg = sigs[i];
+    if (sg.length < 0 || sg.offset < 0 || sg.levels < 0) fail(LFX_EINVAL, "bad signal " + std::to_string(i));
+    const WaveletDefL w = find_wavelet(sg.wavelet);
+    long long L = sg.levels;
+    if (L == 0) {
+      L = max_level_w(w.width, sg.length);
+      if (L < 0) fail(LFX_EINVAL, "level 0 (max level) is undefined for a width-1 wavelet");
+    }
+    check_bands(bc, has_lift_steps(w), sg.length, L);
+    plan->total += sg.length;
+    if (sg.length == 0 || L == 0) {
+      if (L == 0) fail(LFX_EINVAL, "batched signals need at least one level");
+      continue;
+    }
+    const void* prog = device_program(sg.wavelet, tf, storage);
+    std::vector<Group> gs = cs == 8 ? signal_groups<double>(w, tf, storage, bc, sg.length, L, plain)
+                                    : signal_groups<float>(w, tf, storage, bc, sg.length, L, plain);
+    const int G = (int)gs.size();
+    // where each group's output band goes: user output or a scratch region
+    // forward: group g writes d's to out, s to (last ? out : next input)
+    // reverse: group g (coarse -> fine order) writes its band to
+    //          (finest or next is a whole-line group) ? out : scratch
+    std::vector<long long> scr_off(G, -1);
+    auto add_item = [&](int stage, const BatchItem& it, double c, size_t l, double b) {
+      if ((int)per_stage.size() <= stage) {
+        per_stage.resize(stage + 1);
+        cost.resize(stage + 1);
+        lds.resize(stage + 1, 0);
+        bytes.resize(stage + 1, 0.0);
+      }
+      per_stage[stage].push_back(it);
+      cost[stage].push_back(c);
+      lds[stage] = std::max(lds[stage], l);
+      bytes[stage] += b;
+    };
+    auto arr_bytes = [&](int arr) { return arr == kArrScr ? (double)cs : (double)es; };
+    if (plan->fwd) {
+      int in_arr = kArrIn;
+      long long in_off = sg.offset;
+      for (int g = 0; g < G; ++g) {
+        const Group& gr = gs[g];
+        const bool last = g + 1 == G;
+        const long long mk = band_len(gr.m0, gr.k);
+        int s_arr;
+        long long s_off;
+        if (last || !gs[g + 1].tile) {
+          if (!last && gr.tile) {  // tile group followed by a whole-line group: s via scratch
+            s_arr = kArrScr;
+            s_off = scr_elems;
+            scr_elems += mk;
+          } else {
+            s_arr = kArrOut;
+            s_off = sg.offset;
+          }
+        } else {
+          s_arr = kArrScr;
+          s_off = scr_elems;
+          scr_elems += mk;
+        }
+        if (gr.tile) {
+          const TileArgs ta{gr.m0, gr.T, gr.H, gr.ntiles, gr.k, 0};
+          for (long long t = 0; t < gr.ntiles; ++t) {
+            BatchItem it{};
+            it.prog = prog; it.kind = kItemTile; it.m0 = gr.m0; it.T = gr.T; it.H = gr.H; it.tile = t; it.k = gr.k;
+            it.arr[0] = in_arr; it.off[0] = in_off;
+            it.arr[1] = kArrOut; it.off[1] = sg.offset;
+            it.arr[2] = s_arr; it.off[2] = s_off;
+            const long long own = std::min(gr.T, gr.m0 - t * gr.T);
+            add_item(g, it, (double)(gr.T + 2 * gr.H), tile_lds_bytes(ta, true, cs),
+                     own * (arr_bytes(in_arr) + (double)es));
+          }
+        } else {
+          BatchItem it{};
+          it.prog = prog; it.kind = kItemLine; it.m0 = gr.m0; it.k = gr.k;
+          it.arr[0] = in_arr; it.off[0] = in_off;
+          it.arr[1] = kArrOut; it.off[1] = sg.offset;
+          it.arr[2] = in_arr; it.off[2] = in_off;
+          add_item(g, it, (double)gr.m0, line_lds_bytes(gr.m0, cs), gr.m0 * (arr_bytes(in_arr) + (double)es));
+        }
+        in_arr = s_arr;
+        in_off = s_off;
+      }
+    } else {
+      // coarse -> fine: groups in reverse order
+      int s_arr = kArrIn;
+      long long s_off = sg.offset;
+      for (int gi = 0; gi < G; ++gi) {
+        const int g = G - 1 - gi;
+        const Group& gr = gs[g];
+        const bool finest = g == 0;
+        int o_arr;
+        long long o_off;
+        if (finest || !gs[g - 1].tile) {
+          o_arr = kArrOut;
+          o_off = sg.offset;
+        } else {
+          o_arr = kArrScr;
+          o_off = scr_elems;
+          scr_elems += gr.m0;
+        }
+        if (gr.tile) {
+          const TileArgs ta{gr.m0, gr.T, gr.H, gr.ntiles, gr.k, 0};
+          for (long long t = 0; t < gr.ntiles; ++t) {
+            BatchItem it{};
+            it.prog = prog; it.kind = kItemTile; it.m0 = gr.m0; it.T = gr.T; it.H = gr.H; it.tile = t; it.k = gr.k;
+            it.arr[0] = kArrIn; it.off[0] = sg.offset;
+            it.arr[1] = o_arr; it.off[1] = o_off;
+            it.arr[2] = s_arr; it.off[2] = s_off;
+            const long long own = std::min(gr.T, gr.m0 - t * gr.T);
+            add_item(gi, it, (double)tile_rev_window(ta), tile_lds_bytes(ta, false, cs),
+                     own * ((double)es + arr_bytes(o_arr)));
+          }
+        } else {
+          BatchItem it{};
+          it.prog = prog; it.kind = kItemLine; it.m0 = gr.m0; it.k = gr.k;
+          it.arr[0] = kArrIn; it.off[0] = sg.offset;
+          it.arr[1] = o_arr; it.off[1] = o_off;
+          it.arr[2] = s_arr; it.off[2] = s_off;
+          add_item(gi, it, (double)gr.m0, line_lds_bytes(gr.m0, cs), gr.m0 * ((double)es + arr_bytes(o_arr)));
+        }
+        s_arr = o_arr;
+        s_off = o_off;
+      }
+    }
+  }
+  // order items of each stage by decreasing cost, upload
+  std::vector<BatchItem> all;
+  for (size_t st = 0; st < per_stage.size(); ++st) {
+    std::vector<size_t> idx(per_stage[st].size());
+    for (size_t j = 0; j < idx.size(); ++j) idx[j] = j;
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return cost[st][a] > cost[st][b]; });
+    BatchStage bs;
+    bs.item0 = (long long)all.size();
+    bs.nitems = (long long)idx.size();
+    bs.lds = lds[st];
+    bs.alg_bytes = bytes[st];
+    for (size_t j : idx) all.push_back(per_stage[st][j]);
+    plan->stages.push_back(bs);
+  }
+  if (!all.empty()) {
+    hip_check(hipMalloc(&plan->d_items, all.size() * sizeof(BatchItem)), "hipMalloc(items)");
+    hip_check(hipMemcpy(plan->d_items, all.data(), all.size() * sizeof(BatchItem), hipMemcpyHostToDevice),
+              "hipMemcpy(items)");
+  }
+  plan->scr_bytes = (size_t)std::max<long long>(scr_elems, 1) * cs;
+  if (hipMalloc(&plan->d_scr, plan->scr_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    if (plan->d_items) (void)hipFree(plan->d_items);
+    fail(LFX_ENOMEM, "batch scratch allocation failed");
+  }
+  return plan.release();
+}
+
+static void batch_stage(BatchPlanImpl* p, int st, const void* in, void* out, hipStream_t s) {
+  if (st < 0 || st >= (int)p->stages.size()) fail(LFX_EINVAL, "stage out of range");
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev != p->dev) fail(LFX_EINVAL, "plan was created on another device");
+  const BatchStage& bs = p->stages[st];
+  hip_check(launch_batch(p->storage, p->fwd, p->bc, p->d_items + bs.item0, bs.nitems, bs.lds, in, out, p->d_scr, s),
+            "batch kernel");
+}
+
 }  // namespace lfx
 
 using namespace lfx;
@@ -626,6 +865,65 @@ int lfx_program_dump(uint32_t wavelet, uint32_t transform, int32_t storage, doub
     else dump(resolve<float>(w, (int)transform));
   });
   return rc == LFX_OK ? n : rc;
+}
+
+int lfx_batch_plan_create(const lfx_signal* signals, int32_t nsignals, uint32_t transform, uint32_t bc,
+                          int32_t storage, uint32_t flags, lfx_batch_plan* plan) {
+  return guarded([&] {
+    if (!plan) fail(LFX_EINVAL, "null plan pointer");
+    *plan = reinterpret_cast<lfx_batch_plan>(batch_create(signals, nsignals, (int)transform, (int)bc, storage, flags));
+  });
+}
+
+int lfx_batch_execute(lfx_batch_plan plan, const void* in_dev, void* out_dev, void* stream) {
+  return guarded([&] {
+    BatchPlanImpl* p = reinterpret_cast<BatchPlanImpl*>(plan);
+    if (!p) fail(LFX_EINVAL, "null plan");
+    if (in_dev == out_dev) fail(LFX_EINVAL, "batched transforms need distinct input and output arrays");
+    for (int s = 0; s < (int)p->stages.size(); ++s) batch_stage(p, s, in_dev, out_dev, (hipStream_t)stream);
+  });
+}
+
+int lfx_batch_num_stages(lfx_batch_plan plan) {
+  BatchPlanImpl* p = reinterpret_cast<BatchPlanImpl*>(plan);
+  return p ? (int)p->stages.size() : LFX_EINVAL;
+}
+
+int lfx_batch_execute_stage(lfx_batch_plan plan, int32_t stage, const void* in_dev, void* out_dev, void* stream) {
+  return guarded([&] {
+    BatchPlanImpl* p = reinterpret_cast<BatchPlanImpl*>(plan);
+    if (!p) fail(LFX_EINVAL, "null plan");
+    if (in_dev == out_dev) fail(LFX_EINVAL, "batched transforms need distinct input and output arrays");
+    batch_stage(p, stage, in_dev, out_dev, (hipStream_t)stream);
+  });
+}
+
+int lfx_batch_stage_info(lfx_batch_plan plan, int32_t stage, int64_t* nitems, double* alg_bytes, int64_t* lds_bytes) {
+  return guarded([&] {
+    BatchPlanImpl* p = reinterpret_cast<BatchPlanImpl*>(plan);
+    if (!p) fail(LFX_EINVAL, "null plan");
+    if (stage < 0 || stage >= (int)p->stages.size()) fail(LFX_EINVAL, "stage out of range");
+    if (nitems) *nitems = p->stages[stage].nitems;
+    if (alg_bytes) *alg_bytes = p->stages[stage].alg_bytes;
+    if (lds_bytes) *lds_bytes = (int64_t)p->stages[stage].lds;
+  });
+}
+
+void lfx_batch_plan_destroy(lfx_batch_plan plan) {
+  BatchPlanImpl* p = reinterpret_cast<BatchPlanImpl*>(plan);
+  if (!p) return;
+  if (p->d_items) (void)hipFree(p->d_items);
+  if (p->d_scr) (void)hipFree(p->d_scr);
+  delete p;
+}
+
+int lfx_set_tuning(int64_t tile, int32_t levels_per_pass) {
+  return guarded([&] {
+    if (tile < 0 || levels_per_pass < 0 || levels_per_pass > 12) fail(LFX_EINVAL, "bad tuning values");
+    if (tile > 0 && (tile < 64 || tile > (1 << 16))) fail(LFX_EINVAL, "tile must be in [64, 65536]");
+    g_tuning.tile = tile;
+    g_tuning.k = levels_per_pass;
+  });
 }
 
 }  // extern "C"

@@ -4,6 +4,12 @@
 #include <cstddef>
 #include <cstdint>
 
+#if defined(__HIPCC__) && !defined(LFX_CPU_SIM)
+#define LFX_HDI __host__ __device__ inline
+#else
+#define LFX_HDI inline
+#endif
+
 namespace lfx {
 
 enum Storage : int32_t { kF32 = 0, kF64 = 1, kBF16 = 2, kF16 = 3 };
@@ -64,8 +70,8 @@ struct TileArgs {
 };
 
 // Reverse tile: LDS window (elements) of the finest level.
-inline long long tile_rev_window(const TileArgs& t) { return (1ll << t.k) * ((t.T >> t.k) + 2 + 2 * t.H); }
-inline long long tile_fwd_window(const TileArgs& t) { return t.T + 2 * t.H + 2; }
+LFX_HDI long long tile_rev_window(const TileArgs& t) { return (1ll << t.k) * ((t.T >> t.k) + 2 + 2 * t.H); }
+LFX_HDI long long tile_fwd_window(const TileArgs& t) { return t.T + 2 * t.H + 2; }
 inline size_t tile_lds_bytes(const TileArgs& t, bool fwd, size_t csize) {
   const long long W = fwd ? tile_fwd_window(t) : tile_rev_window(t);
   return (size_t)(W + W / 2 + 2) * csize;
@@ -85,6 +91,23 @@ struct TileLaunch {
   bool scr[3];
   LineMap3 lm;
   long long nlines;
+};
+
+// One work item of a batched stage (see lfx_batch_*). Each of the item's
+// three logical arrays (as in LineLaunch / TileLaunch) lives in one of the
+// user input, the user output or the plan's scratch, at an element offset.
+enum BatchArr : int32_t { kArrIn = 0, kArrOut = 1, kArrScr = 2 };
+enum BatchKind : int32_t { kItemLine = 0, kItemTile = 1 };
+struct BatchItem {
+  const void* prog;  // device program
+  long long m0;      // band length
+  long long T, H;    // tile geometry (tile items)
+  long long tile;    // tile index (tile items)
+  long long off[3];  // element offsets of the three logical arrays
+  int32_t kind;      // BatchKind
+  int32_t k;         // levels of this item
+  int32_t arr[3];    // BatchArr of the three logical arrays
+  int32_t pad;
 };
 
 }  // namespace lfx

@@ -270,6 +270,91 @@ hipError_t launch_tile(const TileLaunch& tl, hipStream_t st) {
 #undef CALL_R
 }
 
+// ---------------------------------------------------------------------------
+// batched stages: one workgroup per item; items of one stage may belong to
+// different signals, wavelets and level counts (programs are per item).
+
+template <class C, class S>
+struct ArrSel {
+  const S* in;
+  S* out;
+  C* scr;
+};
+
+// Dispatch the item's array kinds to the body with matching element types.
+template <class C, class S, bool FWD>
+__device__ void batch_item(const BatchItem& it, int bc, const ArrSel<C, S>& a, C* P) {
+  const DProgram<C>* prog = reinterpret_cast<const DProgram<C>*>(it.prog);
+  const bool s0 = it.arr[0] == kArrScr, s1 = it.arr[1] == kArrScr, s2 = it.arr[2] == kArrScr;
+  auto user_in = [&](int i) -> const S* { return (it.arr[i] == kArrIn ? a.in : a.out) + it.off[i]; };
+  auto user_out = [&](int i) -> S* { return a.out + it.off[i]; };
+  auto scr = [&](int i) -> C* { return a.scr + it.off[i]; };
+  if (it.kind == kItemLine) {
+    C* Q = P + it.m0;
+    if (FWD) {
+      if (s0) line_fwd_body<C, C, S>(prog, bc, it.m0, it.k, scr(0), 1, user_out(1), 1, P, Q);
+      else line_fwd_body<C, S, S>(prog, bc, it.m0, it.k, user_in(0), 1, user_out(1), 1, P, Q);
+    } else {
+      if (s1) line_rev_body<C, S, S, C>(prog, bc, it.m0, it.k, user_in(0), 1, user_in(2), 1, scr(1), 1, P, Q);
+      else line_rev_body<C, S, S, S>(prog, bc, it.m0, it.k, user_in(0), 1, user_in(2), 1, user_out(1), 1, P, Q);
+    }
+    return;
+  }
+  const TileArgs ta{it.m0, it.T, it.H, 0, it.k, 0};
+  if (FWD) {
+    C* Q = P + tile_fwd_window(ta);
+    if (s0) {
+      if (s2) tile_fwd_body<C, C, S, C>(prog, bc, it.m0, it.k, it.T, it.H, it.tile, scr(0), 1, user_out(1), 1, scr(2), 1, P, Q);
+      else tile_fwd_body<C, C, S, S>(prog, bc, it.m0, it.k, it.T, it.H, it.tile, scr(0), 1, user_out(1), 1, user_out(2), 1, P, Q);
+    } else {
+      if (s2) tile_fwd_body<C, S, S, C>(prog, bc, it.m0, it.k, it.T, it.H, it.tile, user_in(0), 1, user_out(1), 1, scr(2), 1, P, Q);
+      else tile_fwd_body<C, S, S, S>(prog, bc, it.m0, it.k, it.T, it.H, it.tile, user_in(0), 1, user_out(1), 1, user_out(2), 1, P, Q);
+    }
+  } else {
+    C* Q = P + tile_rev_window(ta);
+    if (s2) {
+      if (s1) tile_rev_body<C, S, C, C>(prog, bc, it.m0, it.k, it.T, it.H, it.tile, user_in(0), 1, scr(2), 1, scr(1), 1, P, Q);
+      else tile_rev_body<C, S, C, S>(prog, bc, it.m0, it.k, it.T, it.H, it.tile, user_in(0), 1, scr(2), 1, user_out(1), 1, P, Q);
+    } else {
+      if (s1) tile_rev_body<C, S, S, C>(prog, bc, it.m0, it.k, it.T, it.H, it.tile, user_in(0), 1, user_in(2), 1, scr(1), 1, P, Q);
+      else tile_rev_body<C, S, S, S>(prog, bc, it.m0, it.k, it.T, it.H, it.tile, user_in(0), 1, user_in(2), 1, user_out(1), 1, P, Q);
+    }
+  }
+}
+
+template <class C, class S, bool FWD>
+__global__ void __launch_bounds__(kThreads) k_batch(const BatchItem* items, long long nitems, int bc, const S* in,
+                                                    S* out, C* scr, long long item0) {
+  const long long i = item0 + blockIdx.x;
+  if (i >= nitems) return;
+  const BatchItem it = items[i];
+  const ArrSel<C, S> a{in, out, scr};
+  batch_item<C, S, FWD>(it, bc, a, lds_base<C>());
+}
+
+template <int SI, bool FWD>
+static hipError_t batch_t(int bc, const BatchItem* items, long long nitems, size_t lds, const void* in, void* out,
+                          void* scr, hipStream_t st) {
+  using C = typename StoreT<SI>::C;
+  using S = typename StoreT<SI>::T;
+  auto kf = k_batch<C, S, FWD>;
+  hipError_t e = ensure_lds(kf);
+  if (e != hipSuccess) return e;
+  return grid_loop(nitems, [&](long long b0, unsigned cnt) {
+    hipLaunchKernelGGL(kf, dim3(cnt), dim3(kThreads), lds, st, items, nitems, bc, reinterpret_cast<const S*>(in),
+                       reinterpret_cast<S*>(out), reinterpret_cast<C*>(scr), b0);
+  });
+}
+
+hipError_t launch_batch(int storage, bool fwd, int bc, const BatchItem* items, long long nitems, size_t lds_bytes,
+                        const void* in, void* out, void* scr, hipStream_t st) {
+  if (nitems <= 0) return hipSuccess;
+#define CALL_B(S) fwd ? batch_t<S, true>(bc, items, nitems, lds_bytes, in, out, scr, st) \
+                      : batch_t<S, false>(bc, items, nitems, lds_bytes, in, out, scr, st)
+  LFX_DISPATCH_STORAGE(storage, CALL_B)
+#undef CALL_B
+}
+
 hipError_t launch_copy(int storage, const void* src, void* dst, const CopyMap& cm, long long total,
                        hipStream_t st) {
   if (total <= 0) return hipSuccess;

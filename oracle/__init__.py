@@ -55,6 +55,10 @@ def _load():
         lib.oracle_wavelet_dump.argtypes = [ctypes.c_uint32, ctypes.c_int32,
                                             ctypes.POINTER(ctypes.c_double), ctypes.c_int32]
         lib.oracle_wavelet_dump.restype = ctypes.c_int
+        lib.oracle_batch.argtypes = [i64p, i64p, ctypes.POINTER(ctypes.c_uint32), i64p, ctypes.c_int64,
+                                     ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_int32]
+        lib.oracle_batch.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -126,3 +130,20 @@ def wavelet_dump(wavelet, dtype):
     if n < 0:
         raise ValueError("unknown wavelet")
     return np.array(buf[: 21 * n]).reshape(n, 21)
+
+
+def batch(x, signals, transform, bc=ZERO, out=None, n_threads=1):
+    """Packed 1-D signals [(offset, length, wavelet, levels)] of a flat array."""
+    lib = _load()
+    if out is None:
+        out = np.empty_like(x)
+    n = len(signals)
+    offs = _arr([s[0] for s in signals])
+    lens = _arr([s[1] for s in signals])
+    wv = (ctypes.c_uint32 * max(n, 1))(*[int(s[2]) for s in signals])
+    lv = _arr([s[3] for s in signals])
+    rc = lib.oracle_batch(offs, lens, wv, lv, n, int(transform), int(bc), 0 if x.dtype == np.float32 else 1,
+                          ctypes.c_void_p(x.ctypes.data), ctypes.c_void_p(out.ctypes.data), int(n_threads))
+    if rc != 0:
+        raise ValueError(lib.oracle_last_error().decode())
+    return out

@@ -33,6 +33,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -690,6 +692,58 @@ int oracle_lwt_custom(const oracle_step* steps, int32_t nsteps, uint32_t width, 
   }
   return oracle_run(w, transform, bc, ndim, shape, stride_in, stride_out, naxes, axes, separable, levels,
                     dtype, in, out, n_threads);
+}
+
+// Batch of packed 1-D signals (offset, length, wavelet, levels), each the
+// reference's per-line multi-level driver; signals are handed to n_threads
+// threads dynamically (the reference always gives one line to one thread).
+int oracle_batch(const int64_t* offsets, const int64_t* lengths, const uint32_t* wavelets, const int64_t* levels,
+                 int64_t nsig, uint32_t transform, uint32_t bc, int32_t dtype, const void* in, void* out,
+                 int32_t n_threads) {
+  try {
+    if (transform > 3 || bc > 4) throw std::invalid_argument("bad transform / bc");
+    std::atomic<int64_t> next{0};
+    std::string err;
+    std::mutex emu;
+    auto work = [&](size_t, size_t, int) {
+      ora::LineWork<float> wf;
+      ora::LineWork<double> wd;
+      for (;;) {
+        const int64_t i = next.fetch_add(1);
+        if (i >= nsig) break;
+        const ora::WaveletL* w = ora::find(wavelets[i]);
+        if (!w) { std::lock_guard<std::mutex> lk(emu); err = "unknown wavelet"; continue; }
+        const size_t n = size_t(lengths[i]);
+        long L = long(levels[i]);
+        if (L == 0) L = ora::max_level(w->width, n);
+        if (L <= 0) continue;
+        const bool fs = ora::forward_stepping(int(transform));
+        if (dtype == 0) {
+          std::vector<ora::Step<float>> st;
+          for (auto& sd : w->steps) st.push_back(ora::make_step<float>(sd));
+          const float* a = (const float*)in + offsets[i];
+          float* b = (float*)out + offsets[i];
+          if (fs) ora::line_fwd_multi(st, int(transform), int(bc), a, b, n, size_t(L), wf);
+          else ora::line_rev_multi(st, int(transform), int(bc), a, b, n, size_t(L), wf);
+        } else {
+          std::vector<ora::Step<double>> st;
+          for (auto& sd : w->steps) st.push_back(ora::make_step<double>(sd));
+          const double* a = (const double*)in + offsets[i];
+          double* b = (double*)out + offsets[i];
+          if (fs) ora::line_fwd_multi(st, int(transform), int(bc), a, b, n, size_t(L), wd);
+          else ora::line_rev_multi(st, int(transform), int(bc), a, b, n, size_t(L), wd);
+        }
+      }
+    };
+    ora::parallel_for(size_t(std::max<int32_t>(n_threads, 1)), n_threads, [&](size_t lo, size_t hi, int t) {
+      for (size_t k = lo; k < hi; ++k) work(0, 0, t);
+    });
+    if (!err.empty()) throw std::invalid_argument(err);
+    return 0;
+  } catch (const std::exception& e) {
+    ora::g_err = e.what();
+    return -1;
+  }
 }
 
 }  // extern "C"

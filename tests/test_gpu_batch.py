@@ -1,0 +1,74 @@
+"""Batched plans (many packed 1-D signals of uneven length / wavelet / level
+count) against the oracle, per signal, fused and plain (one level per stage)."""
+import numpy as np
+import pytest
+
+from helpers import WAVELETS
+
+pytestmark = pytest.mark.gpu
+
+
+def make_signals(seed, nsig, lo, hi, oracle):
+    g = np.random.default_rng(seed)
+    sigs, off = [], 0
+    for _ in range(nsig):
+        n = int(round(2 ** g.uniform(np.log2(lo), np.log2(hi))))
+        w = int(g.choice([w for w in WAVELETS if w != 0]))
+        L = int(g.integers(1, max(2, oracle.max_level(w, n)) + 1))
+        sigs.append((off, n, w, L))
+        off += n
+    return sigs, off
+
+
+@pytest.mark.parametrize("plain", [False, True])
+@pytest.mark.parametrize("tf", [0, 1, 2, 3])
+def test_batch_matches_oracle(gpu, oracle, tf, plain):
+    import torch
+    from lifted_amd.batch import BatchPlan
+    sigs, total = make_signals(10 + tf, 40, 16, 300000, oracle)
+    for bc in (0, 3):
+        x = np.random.default_rng(1).standard_normal(total).astype(np.float32)
+        plan = BatchPlan(sigs, tf, mode=bc, dtype=torch.float32, plain=plain)
+        y = plan.execute(torch.from_numpy(x).to(gpu)).cpu().numpy()
+        for off, n, w, L in sigs:
+            ref = oracle.lwt(x[off:off + n], w, tf, bc, level=[L])
+            got = y[off:off + n]
+            scale = max(np.abs(ref).max(), 1e-30)
+            assert np.abs(got - ref).max() <= 4 * np.finfo(np.float32).eps * scale, (off, n, w, L, tf, bc)
+
+
+def test_batch_fp64_and_graph_replay(gpu, oracle):
+    """fp64 batch; a captured graph of forward + forward-adjoint replays
+    identically to eager execution (several replays, eager work in between)."""
+    import torch
+    from lifted_amd.batch import BatchPlan
+    sigs, total = make_signals(99, 24, 500, 100000, oracle)
+    x = torch.from_numpy(np.random.default_rng(2).standard_normal(total)).to(gpu)
+    fp = BatchPlan(sigs, 0, dtype=torch.float64)
+    fa = BatchPlan(sigs, 2, dtype=torch.float64)
+    y = torch.empty_like(x)
+    z = torch.empty_like(x)
+    fp.execute(x, y)
+    fa.execute(y, z)
+    ye, ze = y.clone(), z.clone()
+    xn = x.cpu().numpy()
+    for off, n, w, L in sigs:
+        r = oracle.lwt(xn[off:off + n], w, 0, 0, level=[L])
+        assert np.abs(ye.cpu().numpy()[off:off + n] - r).max() <= 1e-12 * np.abs(r).max()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    y.zero_()
+    z.zero_()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            fp.execute(x, y)
+            fa.execute(y, z)
+    torch.cuda.current_stream().wait_stream(s)
+    for _ in range(3):
+        y.zero_()
+        z.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(y, ye) and torch.equal(z, ze)
