@@ -532,10 +532,15 @@ static int lwt_overload(uint32_t wavelet, uint32_t transform, uint32_t bc, size_
 // (forward: fine -> coarse; reverse: coarse -> fine); a stage is one launch
 // whose items are tiles or whole lines of any signal, longest first.
 
-struct BatchStage {
+struct BatchLaunch {
   long long item0 = 0, nitems = 0;
   size_t lds = 0;
+};
+struct BatchStage {
+  long long nitems = 0;
+  size_t lds = 0;
   double alg_bytes = 0;  // bytes the stage must read and write (owned ranges)
+  std::vector<BatchLaunch> launches;
 };
 
 struct BatchPlanImpl {
@@ -594,6 +599,7 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
   std::vector<std::vector<BatchItem>> per_stage;
   std::vector<std::vector<double>> cost;
   std::vector<size_t> lds;
+  std::vector<std::vector<size_t>> item_lds;
   std::vector<double> bytes;
   long long scr_elems = 0;
   for (int i = 0; i < nsig; ++i) {
@@ -625,9 +631,11 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
         per_stage.resize(stage + 1);
         cost.resize(stage + 1);
         lds.resize(stage + 1, 0);
+        item_lds.resize(stage + 1);
         bytes.resize(stage + 1, 0.0);
       }
       per_stage[stage].push_back(it);
+      item_lds[stage].push_back(l);
       cost[stage].push_back(c);
       lds[stage] = std::max(lds[stage], l);
       bytes[stage] += b;
@@ -722,18 +730,34 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
       }
     }
   }
-  // order items of each stage by decreasing cost, upload
+  // each stage: items grouped into LDS classes (one launch per class, so small
+  // tiles are not launched with the LDS reservation of the longest line), each
+  // class ordered by decreasing cost; upload
   std::vector<BatchItem> all;
+  const size_t kClasses[] = {40 * 1024, 80 * 1024, kMaxLdsBytes};
   for (size_t st = 0; st < per_stage.size(); ++st) {
-    std::vector<size_t> idx(per_stage[st].size());
-    for (size_t j = 0; j < idx.size(); ++j) idx[j] = j;
-    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return cost[st][a] > cost[st][b]; });
     BatchStage bs;
-    bs.item0 = (long long)all.size();
-    bs.nitems = (long long)idx.size();
-    bs.lds = lds[st];
     bs.alg_bytes = bytes[st];
-    for (size_t j : idx) all.push_back(per_stage[st][j]);
+    for (int c = 2; c >= 0; --c) {
+      std::vector<size_t> idx;
+      for (size_t j = 0; j < per_stage[st].size(); ++j) {
+        const size_t l = item_lds[st][j];
+        const int cls = l <= kClasses[0] ? 0 : (l <= kClasses[1] ? 1 : 2);
+        if (cls == c) idx.push_back(j);
+      }
+      if (idx.empty()) continue;
+      std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return cost[st][a] > cost[st][b]; });
+      BatchLaunch bl;
+      bl.item0 = (long long)all.size();
+      bl.nitems = (long long)idx.size();
+      for (size_t j : idx) {
+        all.push_back(per_stage[st][j]);
+        bl.lds = std::max(bl.lds, item_lds[st][j]);
+      }
+      bs.nitems += bl.nitems;
+      bs.lds = std::max(bs.lds, bl.lds);
+      bs.launches.push_back(bl);
+    }
     plan->stages.push_back(bs);
   }
   if (!all.empty()) {
@@ -756,8 +780,9 @@ static void batch_stage(BatchPlanImpl* p, int st, const void* in, void* out, hip
   hip_check(hipGetDevice(&dev), "hipGetDevice");
   if (dev != p->dev) fail(LFX_EINVAL, "plan was created on another device");
   const BatchStage& bs = p->stages[st];
-  hip_check(launch_batch(p->storage, p->fwd, p->bc, p->d_items + bs.item0, bs.nitems, bs.lds, in, out, p->d_scr, s),
-            "batch kernel");
+  for (const BatchLaunch& bl : bs.launches)
+    hip_check(launch_batch(p->storage, p->fwd, p->bc, p->d_items + bl.item0, bl.nitems, bl.lds, in, out, p->d_scr, s),
+              "batch kernel");
 }
 
 }  // namespace lfx

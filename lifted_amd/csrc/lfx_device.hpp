@@ -183,12 +183,105 @@ __device__ __forceinline__ C apply_folds(const DStep& st, const C* __restrict__ 
   return acc;
 }
 
+// Interior elements [i0, i1) (local indices): every tap is inside the band and
+// inside the window, no boundary map, no folds. One specialised loop per step
+// shape; coefficients are uniform and stay in scalar registers.
+template <class C, int KIND, int N, bool SUB>
+__device__ __forceinline__ void interior_loop(C* __restrict__ y, const C* __restrict__ x, int i0, int i1, int xs,
+                                              const C* __restrict__ cf, int tid, int nthr) {
+  C c[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) c[k] = cf[KIND == kGeneric ? k : 0];
+#ifdef LFX_NO_UNROLL
+#pragma unroll 1
+#endif
+  for (int i = i0 + tid; i < i1; i += nthr) {
+    const C* xp = x + (i + xs);
+    C acc = y[i];
+    if (KIND == kGeneric) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) acc = fmac<C>(c[k], xp[k], acc);
+    } else if (KIND == kRepeat) {
+      C t = xp[0];
+#pragma unroll
+      for (int k = 1; k < N; ++k) t += xp[k];
+      acc = fmac<C>(c[0], t, acc);
+    } else {
+      acc = SUB ? acc - xp[0] : acc + xp[0];
+    }
+    y[i] = acc;
+  }
+}
+
+template <class C>
+__device__ __forceinline__ void interior_dispatch(const DStep& st, C* __restrict__ y, const C* __restrict__ x, int i0, int i1,
+                                  int xs, const C* __restrict__ cf, int tid, int nthr) {
+  if (i1 <= i0) return;
+  if (st.kind == kUnit) {
+    if (st.sub) interior_loop<C, kUnit, 1, true>(y, x, i0, i1, xs, cf, tid, nthr);
+    else interior_loop<C, kUnit, 1, false>(y, x, i0, i1, xs, cf, tid, nthr);
+    return;
+  }
+  if (st.kind == kRepeat) {
+    switch (st.N) {
+      case 1: interior_loop<C, kRepeat, 1, false>(y, x, i0, i1, xs, cf, tid, nthr); return;
+      case 2: interior_loop<C, kRepeat, 2, false>(y, x, i0, i1, xs, cf, tid, nthr); return;
+      case 3: interior_loop<C, kRepeat, 3, false>(y, x, i0, i1, xs, cf, tid, nthr); return;
+      default:
+        for (int i = i0 + tid; i < i1; i += nthr) y[i] = interior<C>(st, cf, x, i + xs, y[i]);
+        return;
+    }
+  }
+  switch (st.N) {
+    case 1: interior_loop<C, kGeneric, 1, false>(y, x, i0, i1, xs, cf, tid, nthr); return;
+    case 2: interior_loop<C, kGeneric, 2, false>(y, x, i0, i1, xs, cf, tid, nthr); return;
+    case 3: interior_loop<C, kGeneric, 3, false>(y, x, i0, i1, xs, cf, tid, nthr); return;
+    case 4: interior_loop<C, kGeneric, 4, false>(y, x, i0, i1, xs, cf, tid, nthr); return;
+    case 5: interior_loop<C, kGeneric, 5, false>(y, x, i0, i1, xs, cf, tid, nthr); return;
+    case 6: interior_loop<C, kGeneric, 6, false>(y, x, i0, i1, xs, cf, tid, nthr); return;
+    case 8: interior_loop<C, kGeneric, 8, false>(y, x, i0, i1, xs, cf, tid, nthr); return;
+    case 9: interior_loop<C, kGeneric, 9, false>(y, x, i0, i1, xs, cf, tid, nthr); return;
+    default:
+      for (int i = i0 + tid; i < i1; i += nthr) y[i] = interior<C>(st, cf, x, i + xs, y[i]);
+      return;
+  }
+}
+
+// One element with the full semantics: boundary map (normal) or zero-BC
+// stencil plus folds (adjoint), and the front/back double update when the
+// band is shorter than the stencil.
+#ifndef LFX_EDGE_INLINE
+#define LFX_EDGE_INLINE __forceinline__
+#endif
+template <class C>
+__device__ LFX_EDGE_INLINE C edge_element(const DStep& st, const C* __restrict__ cf, int bc, const Band<C>& x,
+                                          long long nx, long long ny, long long n1, long long n2, long long g,
+                                          long long f0, long long f1, long long flo, long long fhi, long long b0,
+                                          long long b1, long long blo, long long bhi, C acc) {
+  if (!st.adjoint) {
+    if (g >= n1 && g < n2) return bc_stencil<C>(st, cf, kZero, x, nx, g + st.off, acc);
+    if (g < n1) acc = bc_stencil<C>(st, cf, bc, x, nx, g + st.off, acc);
+    if (g >= n2) acc = bc_stencil<C>(st, cf, bc, x, nx, g + st.off, acc);
+    return acc;
+  }
+  if (g >= flo && g <= fhi) acc = apply_folds<C>(st, cf, bc, x, nx, ny, f0, f1, g, acc);
+  if (g >= n1 && g < n2) {
+    acc = bc_stencil<C>(st, cf, kZero, x, nx, g + st.off, acc);
+  } else {
+    if (g < n1) acc = bc_stencil<C>(st, cf, kZero, x, nx, g + st.off, acc);
+    if (g >= n2) acc = bc_stencil<C>(st, cf, kZero, x, nx, g + st.off, acc);
+  }
+  if (g >= blo && g <= bhi) acc = apply_folds<C>(st, cf, bc, x, nx, ny, b0, b1, g, acc);
+  return acc;
+}
+
 // Apply one lifting step (not Scale) to the s/d bands held in LDS. ns/nd are
 // the global band lengths of this level. Only elements whose global index lies
 // in [ylo, yhi) are updated (the whole window when called with the full band).
-// Caller synchronizes afterwards.
+// Elements in [n1, n2) away from any fold target take the specialised interior
+// loop; the few others take the full per-element path. Caller synchronizes.
 template <class C>
-__device__ void lift_step(const DStep& st, const C* __restrict__ cf, int bc, const Band<C>& s, const Band<C>& d,
+__device__ __forceinline__ void lift_step(const DStep& st, const C* __restrict__ cf, int bc, const Band<C>& s, const Band<C>& d,
                           long long ns, long long nd, long long ylo, long long yhi, int tid, int nthr) {
   const Band<C>& y = st.y_is_d ? d : s;
   const Band<C>& x = st.y_is_d ? s : d;
@@ -196,44 +289,42 @@ __device__ void lift_step(const DStep& st, const C* __restrict__ cf, int bc, con
   const long long nx = st.y_is_d ? ns : nd;
   const long long n1 = (long long)st.nf < ny ? (long long)st.nf : ny;
   const long long n2 = nd > (long long)st.nb ? nd - (long long)st.nb : 0;
-  const long long xshift = y.g0 - x.g0 + st.off;  // local x index of tap 0 = i + xshift
-  const long long i0 = ylo - y.g0, i1 = yhi - y.g0;
-
-  if (!st.adjoint) {
-    for (long long i = i0 + tid; i < i1; i += nthr) {
-      const long long g = y.g0 + i;
-      C acc = y.p[i];
-      if (g >= n1 && g < n2) {
-        acc = interior<C>(st, cf, x.p, (int)(i + xshift), acc);
-      } else {
-        if (g < n1) acc = bc_stencil<C>(st, cf, bc, x, nx, g + st.off, acc);
-        if (g >= n2) acc = bc_stencil<C>(st, cf, bc, x, nx, g + st.off, acc);
-      }
-      y.p[i] = acc;
-    }
-    return;
+  long long f0 = 0, f1 = 0, b0 = 0, b1 = 0, flo = 1, fhi = 0, blo = 1, bhi = 0;
+  if (st.adjoint) {
+    // fold sources: front m in [o, 0) (unit steps also stop at m = o + nx,
+    // steps:697), back m in [nd, nx - o_r)
+    f0 = st.fold_off;
+    f1 = 0;
+    if (st.kind == kUnit && f0 + nx < f1) f1 = f0 + nx;
+    b0 = nd;
+    b1 = nx - st.off;
+    fold_range(bc, f0, f1, ny, flo, fhi);
+    fold_range(bc, b0, b1, ny, blo, bhi);
   }
-  // adjoint: fold sources. Front: m in [o, 0) (unit steps also stop at m = o + nx,
-  // steps:697); back: m in [nd, nx - o_r).
-  const long long f0 = st.fold_off;
-  long long f1 = 0;
-  if (st.kind == kUnit && f0 + nx < f1) f1 = f0 + nx;
-  const long long b0 = nd, b1 = nx - st.off;
-  long long flo, fhi, blo, bhi;
-  fold_range(bc, f0, f1, ny, flo, fhi);
-  fold_range(bc, b0, b1, ny, blo, bhi);
-  for (long long i = i0 + tid; i < i1; i += nthr) {
-    const long long g = y.g0 + i;
-    C acc = y.p[i];
-    if (g >= flo && g <= fhi) acc = apply_folds<C>(st, cf, bc, x, nx, ny, f0, f1, g, acc);
-    if (g >= n1 && g < n2) {
-      acc = interior<C>(st, cf, x.p, (int)(i + xshift), acc);
-    } else {
-      if (g < n1) acc = bc_stencil<C>(st, cf, kZero, x, nx, g + st.off, acc);
-      if (g >= n2) acc = bc_stencil<C>(st, cf, kZero, x, nx, g + st.off, acc);
+  // interior range (global), excluding fold targets at the band ends
+  long long A = ylo > n1 ? ylo : n1;
+  long long B = yhi < n2 ? yhi : n2;
+  if (st.adjoint) {
+    if (fhi >= flo) {
+      if (fhi + 1 > A) A = fhi + 1;
+      if (flo < B && flo >= A) A = B;  // targets inside the interior (Periodic wrap): no fast path
     }
-    if (g >= blo && g <= bhi) acc = apply_folds<C>(st, cf, bc, x, nx, ny, b0, b1, g, acc);
-    y.p[i] = acc;
+    if (bhi >= blo) {
+      if (blo < B) B = blo;
+      if (bhi >= A && bhi < B) A = B;
+    }
+  }
+  if (A > yhi) A = yhi;
+  if (B > yhi) B = yhi;
+  if (B < A) B = A;
+  const int xs = (int)(y.g0 - x.g0 + st.off);
+  interior_dispatch<C>(st, y.p, x.p, (int)(A - y.g0), (int)(B - y.g0), xs, cf, tid, nthr);
+  // edges: [ylo, A) and [B, yhi)
+  const long long e0 = A - ylo, e1 = yhi - B;
+  for (long long e = tid; e < e0 + e1; e += nthr) {
+    const long long g = e < e0 ? ylo + e : B + (e - e0);
+    const long long i = g - y.g0;
+    y.p[i] = edge_element<C>(st, cf, bc, x, nx, ny, n1, n2, g, f0, f1, flo, fhi, b0, b1, blo, bhi, y.p[i]);
   }
 }
 
@@ -247,7 +338,7 @@ __device__ __forceinline__ void scale_step(const C* __restrict__ cf, const Band<
 
 // All steps of one level on full bands held in LDS.
 template <class C>
-__device__ void level_steps(const DProgram<C>* __restrict__ prog, int bc, const Band<C>& s, const Band<C>& d,
+__device__ __forceinline__ void level_steps(const DProgram<C>* __restrict__ prog, int bc, const Band<C>& s, const Band<C>& d,
                             long long ns, long long nd, int tid, int nthr) {
   const int nsteps = prog->nsteps;
   for (int k = 0; k < nsteps; ++k) {

@@ -283,7 +283,7 @@ struct ArrSel {
 
 // Dispatch the item's array kinds to the body with matching element types.
 template <class C, class S, bool FWD>
-__device__ void batch_item(const BatchItem& it, int bc, const ArrSel<C, S>& a, C* P) {
+__device__ __forceinline__ void batch_item(const BatchItem& it, int bc, const ArrSel<C, S>& a, C* P) {
   const DProgram<C>* prog = reinterpret_cast<const DProgram<C>*>(it.prog);
   const bool s0 = it.arr[0] == kArrScr, s1 = it.arr[1] == kArrScr, s2 = it.arr[2] == kArrScr;
   auto user_in = [&](int i) -> const S* { return (it.arr[i] == kArrIn ? a.in : a.out) + it.off[i]; };

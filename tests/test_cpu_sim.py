@@ -93,3 +93,30 @@ def test_tiled_mock_wavelets(sim, oracle, name):
         ref = oracle.lwt(x, 0, tf, bc, level=[3], steps=st)
         got, info = run_sim(sim, 0, tf, bc, x, 3, limit=100, tile=256, k=0, steps=spec)
         np.testing.assert_array_equal(got, ref, err_msg=f"{name} bc={bc} tf={tf} n={n}\n{info}")
+
+
+@pytest.mark.parametrize("name", list(MOCK_WAVELETS))
+def test_whole_line_all_bcs_mock(sim, oracle, name):
+    """Whole-line path (no tiling) with all five BCs, including Periodic folds
+    at non-power-of-two lengths whose targets land mid-band."""
+    st = oracle_steps(MOCK_WAVELETS[name])
+    spec = _spec(MOCK_WAVELETS[name])
+    g = np.random.default_rng(6)
+    for bc, tf, n in itertools.product(range(5), (0, 1, 2, 3), (153, 510, 1000, 37)):
+        x = g.standard_normal(n)
+        ref = oracle.lwt(x, 0, tf, bc, level=[2], steps=st)
+        got, info = run_sim(sim, 0, tf, bc, x, 2, limit=1 << 20, tile=0, k=0, steps=spec)
+        np.testing.assert_array_equal(got, ref, err_msg=f"{name} bc={bc} tf={tf} n={n}\n{info}")
+
+
+@pytest.mark.parametrize("w", [w for w in WAVELETS if w != 0])
+def test_whole_line_all_bcs(sim, oracle, w):
+    g = np.random.default_rng(w + 1)
+    for bc, tf, n in itertools.product(range(5), (0, 1, 2, 3), (1000, 333)):
+        L = min(4, max(1, oracle.max_level(w, n)))
+        if not ok_bands(bc, n, L):
+            continue
+        x = g.standard_normal(n).astype(np.float32)
+        ref = oracle.lwt(x, w, tf, bc, level=[L])
+        got, info = run_sim(sim, w, tf, bc, x, L, limit=1 << 20, tile=0, k=0)
+        np.testing.assert_array_equal(got, ref, err_msg=f"w={w} bc={bc} tf={tf} n={n}")
