@@ -115,6 +115,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a device copy of the workload's input size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); this build has no multi-GPU path: "
@@ -127,6 +129,14 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     sigs, total = make_workload()
+    if args.copy_calib:
+        a = torch.empty(total, device=dev, dtype=torch.float32)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": total * 4, "copy_bytes_written": total * 4, "launches": 3}))
+        return
     gen = torch.Generator(device=dev)
     gen.manual_seed(SEED)
     x = torch.randn(total, generator=gen, device=dev, dtype=torch.float32)

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -222,7 +223,13 @@ struct Tuning {
   long long tile = 0;  // 0: default tile length
   int k = 0;           // 0: default levels per pass
 };
-static thread_local Tuning g_tuning;
+static Tuning env_tuning() {
+  Tuning t;
+  if (const char* e = std::getenv("LFX_TILE")) t.tile = std::atoll(e);
+  if (const char* e = std::getenv("LFX_KPASS")) t.k = std::atoi(e);
+  return t;
+}
+static thread_local Tuning g_tuning = env_tuning();
 
 template <class C>
 static void transform_lines_t(const Plan& p, const WaveletDefL& w, const void* prog, bool fwd, const long long* shp,

@@ -17,7 +17,8 @@ enum Storage : int32_t { kF32 = 0, kF64 = 1, kBF16 = 2, kF16 = 3 };
 inline size_t storage_bytes(int s) { return s == kF64 ? 8 : (s == kF32 ? 4 : 2); }
 inline size_t compute_bytes(int s) { return s == kF64 ? 8 : 4; }
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 1024;  // launch bound; the block size is chosen at launch
+int block_threads();             // threads per workgroup used by the launchers
 constexpr int kMaxDims = 8;
 // LDS one workgroup may use (gfx950: 160 KiB per CU)
 constexpr size_t kMaxLdsBytes = 160 * 1024;

@@ -192,9 +192,7 @@ __device__ __forceinline__ void interior_loop(C* __restrict__ y, const C* __rest
   C c[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) c[k] = cf[KIND == kGeneric ? k : 0];
-#ifdef LFX_NO_UNROLL
-#pragma unroll 1
-#endif
+#pragma unroll 4
   for (int i = i0 + tid; i < i1; i += nthr) {
     const C* xp = x + (i + xs);
     C acc = y[i];

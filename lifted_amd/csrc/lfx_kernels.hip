@@ -3,6 +3,7 @@
 // LineMap3) and their launchers.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <mutex>
 #include <set>
 #include <type_traits>
@@ -107,6 +108,15 @@ __global__ void k_copy_nd(const S* src, S* dst, CopyMap cm, long long total) {
 // ---------------------------------------------------------------------------
 // launchers
 
+int block_threads() {
+  static const int t = [] {
+    const char* e = std::getenv("LFX_THREADS");
+    const int v = e ? std::atoi(e) : 256;
+    return (v == 64 || v == 128 || v == 256 || v == 512 || v == 1024) ? v : 256;
+  }();
+  return t;
+}
+
 template <class K>
 static hipError_t ensure_lds(K kern) {
   // raise the dynamic-LDS limit once per kernel (and device)
@@ -157,7 +167,7 @@ static hipError_t line_fwd_t(const LineLaunch& ll, hipStream_t st) {
   const long long cap0 = ll.n;
   const size_t lds = line_lds_bytes(ll.n, sizeof(C));
   return grid_loop(ll.nlines, [&](long long b0, unsigned cnt) {
-    hipLaunchKernelGGL(kf, dim3(cnt), dim3(kThreads), lds, st, reinterpret_cast<const DProgram<C>*>(ll.prog), ll.bc,
+    hipLaunchKernelGGL(kf, dim3(cnt), dim3(block_threads()), lds, st, reinterpret_cast<const DProgram<C>*>(ll.prog), ll.bc,
                        ll.n, ll.L, reinterpret_cast<const SI*>(ll.a0), reinterpret_cast<SO*>(ll.a1), ll.lm, cap0, b0);
   });
 }
@@ -174,7 +184,7 @@ static hipError_t line_rev_t(const LineLaunch& ll, hipStream_t st) {
   const long long cap0 = ll.n;
   const size_t lds = line_lds_bytes(ll.n, sizeof(C));
   return grid_loop(ll.nlines, [&](long long b0, unsigned cnt) {
-    hipLaunchKernelGGL(kf, dim3(cnt), dim3(kThreads), lds, st, reinterpret_cast<const DProgram<C>*>(ll.prog), ll.bc,
+    hipLaunchKernelGGL(kf, dim3(cnt), dim3(block_threads()), lds, st, reinterpret_cast<const DProgram<C>*>(ll.prog), ll.bc,
                        ll.n, ll.L, reinterpret_cast<const SI*>(ll.a0), reinterpret_cast<const SS*>(ll.a2),
                        reinterpret_cast<SO*>(ll.a1), ll.lm, cap0, b0);
   });
@@ -192,7 +202,7 @@ static hipError_t tile_fwd_t(const TileLaunch& tl, hipStream_t st) {
   const long long cap0 = tile_fwd_window(tl.ta);
   const size_t lds = tile_lds_bytes(tl.ta, true, sizeof(C));
   return grid_loop(tl.nlines * tl.ta.ntiles, [&](long long b0, unsigned cnt) {
-    hipLaunchKernelGGL(kf, dim3(cnt), dim3(kThreads), lds, st, reinterpret_cast<const DProgram<C>*>(tl.prog), tl.bc,
+    hipLaunchKernelGGL(kf, dim3(cnt), dim3(block_threads()), lds, st, reinterpret_cast<const DProgram<C>*>(tl.prog), tl.bc,
                        tl.ta, reinterpret_cast<const SI*>(tl.a0), reinterpret_cast<SO*>(tl.a1),
                        reinterpret_cast<SK*>(tl.a2), tl.lm, cap0, b0);
   });
@@ -210,7 +220,7 @@ static hipError_t tile_rev_t(const TileLaunch& tl, hipStream_t st) {
   const long long cap0 = tile_rev_window(tl.ta);
   const size_t lds = tile_lds_bytes(tl.ta, false, sizeof(C));
   return grid_loop(tl.nlines * tl.ta.ntiles, [&](long long b0, unsigned cnt) {
-    hipLaunchKernelGGL(kf, dim3(cnt), dim3(kThreads), lds, st, reinterpret_cast<const DProgram<C>*>(tl.prog), tl.bc,
+    hipLaunchKernelGGL(kf, dim3(cnt), dim3(block_threads()), lds, st, reinterpret_cast<const DProgram<C>*>(tl.prog), tl.bc,
                        tl.ta, reinterpret_cast<const SI*>(tl.a0), reinterpret_cast<const SK*>(tl.a2),
                        reinterpret_cast<SO*>(tl.a1), tl.lm, cap0, b0);
   });
@@ -341,7 +351,7 @@ static hipError_t batch_t(int bc, const BatchItem* items, long long nitems, size
   hipError_t e = ensure_lds(kf);
   if (e != hipSuccess) return e;
   return grid_loop(nitems, [&](long long b0, unsigned cnt) {
-    hipLaunchKernelGGL(kf, dim3(cnt), dim3(kThreads), lds, st, items, nitems, bc, reinterpret_cast<const S*>(in),
+    hipLaunchKernelGGL(kf, dim3(cnt), dim3(block_threads()), lds, st, items, nitems, bc, reinterpret_cast<const S*>(in),
                        reinterpret_cast<S*>(out), reinterpret_cast<C*>(scr), b0);
   });
 }
