@@ -2,6 +2,7 @@
 
     python -m lifted_amd.build        (or lifted_amd.build.build())
 """
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -22,6 +23,10 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 
 SOURCES = [
     ("lfx_kernels.hip", True),
+    ("lfx_reg_tf0.hip", True),
+    ("lfx_reg_tf1.hip", True),
+    ("lfx_reg_tf2.hip", True),
+    ("lfx_reg_tf3.hip", True),
     ("lfx_api.cpp", False),
     ("lfx_table.cpp", False),
 ]
@@ -37,7 +42,7 @@ def _hipcc():
 def build(verbose=False):
     hipcc = _hipcc()
     os.makedirs(OBJDIR, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src, device in SOURCES:
         obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
         cmd = [hipcc, "-c"] + COMMON
@@ -45,8 +50,15 @@ def build(verbose=False):
         cmd += [os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd))
-        subprocess.run(cmd, check=True)
+        cmds.append(cmd)
         objs.append(obj)
+    # compile the translation units in parallel
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
+        for r in ex.map(lambda c: subprocess.run(c, capture_output=True, text=True), cmds):
+            if r.returncode != 0:
+                sys.stderr.write(r.stdout + r.stderr)
+                raise subprocess.CalledProcessError(r.returncode, r.args)
     tmp = OUT + ".tmp"
     cmd = [hipcc, "-shared", "-o", tmp] + objs + [f"--offload-arch={ARCH}"]
     if verbose:

@@ -543,11 +543,16 @@ struct BatchLaunch {
   long long item0 = 0, nitems = 0;
   size_t lds = 0;
 };
+struct RegLaunch {
+  uint32_t wavelet = 0;
+  long long item0 = 0, nitems = 0;
+};
 struct BatchStage {
   long long nitems = 0;
   size_t lds = 0;
   double alg_bytes = 0;  // bytes the stage must read and write (owned ranges)
   std::vector<BatchLaunch> launches;
+  std::vector<RegLaunch> regs;  // register-tile items, one launch per wavelet
 };
 
 struct BatchPlanImpl {
@@ -570,11 +575,12 @@ struct Group {
 
 template <class C>
 static std::vector<Group> signal_groups(const WaveletDefL& w, int tf, int storage, int bc, long long n, long long L,
-                                        bool plain) {
+                                        bool plain, bool reg) {
   const DProgram<C> hp = resolve<C>(w, tf);
   const bool fwd = tf == kForward || tf == kInverseAdjoint;
   std::vector<Group> g;
-  const LineSchedule sc = schedule(hp, fwd, n, L, storage, bc, g_tuning.tile, plain ? 1 : g_tuning.k);
+  const int k_hint = plain ? 1 : (g_tuning.k > 0 ? g_tuning.k : (reg ? kRegLevels : 0));
+  const LineSchedule sc = schedule(hp, fwd, n, L, storage, bc, g_tuning.tile, k_hint);
   for (auto& p : sc.passes) g.push_back(Group{true, p.m0, p.k, p.T, p.H, p.ntiles});
   if (sc.L_line > 0) {
     if (plain) {
@@ -608,6 +614,7 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
   std::vector<size_t> lds;
   std::vector<std::vector<size_t>> item_lds;
   std::vector<double> bytes;
+  std::vector<std::vector<std::pair<uint32_t, BatchItem>>> reg_stage;
   long long scr_elems = 0;
   for (int i = 0; i < nsig; ++i) {
     const lfx_signal& sg = sigs[i];
@@ -625,15 +632,35 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
       continue;
     }
     const void* prog = device_program(sg.wavelet, tf, storage);
-    std::vector<Group> gs = cs == 8 ? signal_groups<double>(w, tf, storage, bc, sg.length, L, plain)
-                                    : signal_groups<float>(w, tf, storage, bc, sg.length, L, plain);
+    const bool reg = storage == kF32 && !plain && sg.wavelet < kCustomBase;
+    std::vector<Group> gs = cs == 8 ? signal_groups<double>(w, tf, storage, bc, sg.length, L, plain, reg)
+                                    : signal_groups<float>(w, tf, storage, bc, sg.length, L, plain, reg);
+    const DProgram<float> hpf = resolve<float>(w, tf);
+    const DProgram<double> hpd = resolve<double>(w, tf);
+    auto items_of = [&](const Group& gr) {
+      const PassPlan pp{gr.m0, gr.k, gr.T, gr.H, gr.ntiles};
+      return cs == 8 ? pass_items(hpd, plan->fwd, pp, false, storage) : pass_items(hpf, plan->fwd, pp, reg, storage);
+    };
     const int G = (int)gs.size();
     // where each group's output band goes: user output or a scratch region
     // forward: group g writes d's to out, s to (last ? out : next input)
     // reverse: group g (coarse -> fine order) writes its band to
     //          (finest or next is a whole-line group) ? out : scratch
     std::vector<long long> scr_off(G, -1);
-    auto add_item = [&](int stage, const BatchItem& it, double c, size_t l, double b) {
+    auto add_item = [&](int stage, const BatchItem& it, double c, size_t l, double b, uint32_t wid) {
+      if (it.kind == kItemReg) {
+        if ((int)reg_stage.size() <= stage) reg_stage.resize(stage + 1);
+        reg_stage[stage].push_back(std::make_pair(wid, it));
+        if ((int)bytes.size() <= stage) {
+          per_stage.resize(stage + 1);
+          cost.resize(stage + 1);
+          lds.resize(stage + 1, 0);
+          item_lds.resize(stage + 1);
+          bytes.resize(stage + 1, 0.0);
+        }
+        bytes[stage] += b;
+        return;
+      }
       if ((int)per_stage.size() <= stage) {
         per_stage.resize(stage + 1);
         cost.resize(stage + 1);
@@ -672,16 +699,17 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
           scr_elems += mk;
         }
         if (gr.tile) {
-          const TileArgs ta{gr.m0, gr.T, gr.H, gr.ntiles, gr.k, 0};
-          for (long long t = 0; t < gr.ntiles; ++t) {
+          for (const ItemSpec& sp : items_of(gr)) {
             BatchItem it{};
-            it.prog = prog; it.kind = kItemTile; it.m0 = gr.m0; it.T = gr.T; it.H = gr.H; it.tile = t; it.k = gr.k;
+            it.prog = prog; it.m0 = gr.m0; it.a = sp.a; it.b = sp.b; it.H = sp.H; it.k = gr.k;
+            it.kind = sp.kind == kSpecReg ? kItemReg : kItemTile;
             it.arr[0] = in_arr; it.off[0] = in_off;
             it.arr[1] = kArrOut; it.off[1] = sg.offset;
             it.arr[2] = s_arr; it.off[2] = s_off;
-            const long long own = std::min(gr.T, gr.m0 - t * gr.T);
-            add_item(g, it, (double)(gr.T + 2 * gr.H), tile_lds_bytes(ta, true, cs),
-                     own * (arr_bytes(in_arr) + (double)es));
+            const TileArgs ta{gr.m0, sp.b - sp.a, sp.H, 0, gr.k, 0};
+            add_item(g, it, (double)(sp.b - sp.a + 2 * sp.H),
+                     sp.kind == kSpecReg ? 0 : tile_lds_bytes(ta, true, cs),
+                     (sp.b - sp.a) * (arr_bytes(in_arr) + (double)es), sg.wavelet);
           }
         } else {
           BatchItem it{};
@@ -689,7 +717,8 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
           it.arr[0] = in_arr; it.off[0] = in_off;
           it.arr[1] = kArrOut; it.off[1] = sg.offset;
           it.arr[2] = in_arr; it.off[2] = in_off;
-          add_item(g, it, (double)gr.m0, line_lds_bytes(gr.m0, cs), gr.m0 * (arr_bytes(in_arr) + (double)es));
+          add_item(g, it, (double)gr.m0, line_lds_bytes(gr.m0, cs), gr.m0 * (arr_bytes(in_arr) + (double)es),
+                   sg.wavelet);
         }
         in_arr = s_arr;
         in_off = s_off;
@@ -713,16 +742,16 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
           scr_elems += gr.m0;
         }
         if (gr.tile) {
-          const TileArgs ta{gr.m0, gr.T, gr.H, gr.ntiles, gr.k, 0};
-          for (long long t = 0; t < gr.ntiles; ++t) {
+          for (const ItemSpec& sp : items_of(gr)) {
             BatchItem it{};
-            it.prog = prog; it.kind = kItemTile; it.m0 = gr.m0; it.T = gr.T; it.H = gr.H; it.tile = t; it.k = gr.k;
+            it.prog = prog; it.m0 = gr.m0; it.a = sp.a; it.b = sp.b; it.H = sp.H; it.k = gr.k;
+            it.kind = sp.kind == kSpecReg ? kItemReg : kItemTile;
             it.arr[0] = kArrIn; it.off[0] = sg.offset;
             it.arr[1] = o_arr; it.off[1] = o_off;
             it.arr[2] = s_arr; it.off[2] = s_off;
-            const long long own = std::min(gr.T, gr.m0 - t * gr.T);
-            add_item(gi, it, (double)tile_rev_window(ta), tile_lds_bytes(ta, false, cs),
-                     own * ((double)es + arr_bytes(o_arr)));
+            const TileArgs ta{gr.m0, sp.b - sp.a, sp.H, 0, gr.k, 0};
+            add_item(gi, it, (double)tile_rev_window(ta), sp.kind == kSpecReg ? 0 : tile_lds_bytes(ta, false, cs),
+                     (sp.b - sp.a) * ((double)es + arr_bytes(o_arr)), sg.wavelet);
           }
         } else {
           BatchItem it{};
@@ -730,7 +759,8 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
           it.arr[0] = kArrIn; it.off[0] = sg.offset;
           it.arr[1] = o_arr; it.off[1] = o_off;
           it.arr[2] = s_arr; it.off[2] = s_off;
-          add_item(gi, it, (double)gr.m0, line_lds_bytes(gr.m0, cs), gr.m0 * ((double)es + arr_bytes(o_arr)));
+          add_item(gi, it, (double)gr.m0, line_lds_bytes(gr.m0, cs), gr.m0 * ((double)es + arr_bytes(o_arr)),
+                   sg.wavelet);
         }
         s_arr = o_arr;
         s_off = o_off;
@@ -765,6 +795,19 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
       bs.lds = std::max(bs.lds, bl.lds);
       bs.launches.push_back(bl);
     }
+    if (st < reg_stage.size()) {
+      auto& rs = reg_stage[st];
+      std::stable_sort(rs.begin(), rs.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+      for (size_t j = 0; j < rs.size();) {
+        RegLaunch rl;
+        rl.wavelet = rs[j].first;
+        rl.item0 = (long long)all.size();
+        while (j < rs.size() && rs[j].first == rl.wavelet) all.push_back(rs[j++].second);
+        rl.nitems = (long long)all.size() - rl.item0;
+        bs.nitems += rl.nitems;
+        bs.regs.push_back(rl);
+      }
+    }
     plan->stages.push_back(bs);
   }
   if (!all.empty()) {
@@ -787,6 +830,8 @@ static void batch_stage(BatchPlanImpl* p, int st, const void* in, void* out, hip
   hip_check(hipGetDevice(&dev), "hipGetDevice");
   if (dev != p->dev) fail(LFX_EINVAL, "plan was created on another device");
   const BatchStage& bs = p->stages[st];
+  for (const RegLaunch& rl : bs.regs)
+    hip_check(launch_reg(p->tf, rl.wavelet, p->d_items + rl.item0, rl.nitems, in, out, p->d_scr, s), "register kernel");
   for (const BatchLaunch& bl : bs.launches)
     hip_check(launch_batch(p->storage, p->fwd, p->bc, p->d_items + bl.item0, bl.nitems, bl.lds, in, out, p->d_scr, s),
               "batch kernel");

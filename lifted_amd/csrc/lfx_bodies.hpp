@@ -135,14 +135,14 @@ __device__ __forceinline__ void tile_level_steps(const DProgram<C>* __restrict__
   }
 }
 
-// Forward tile: input band `in` (length m0) -> d_r to out[(ns_r + i) * o_ax]
-// for r = 1..k, s_k to sout[i * so_ax].
+// Forward tile owning [a, b) of the input band `in` (length m0; a and, unless
+// b == m0, b multiples of 2^k) -> d_r to out[(ns_r + i) * o_ax] for r = 1..k,
+// s_k to sout[i * so_ax].
 template <class C, class SI, class SO, class SK>
-__device__ __forceinline__ void tile_fwd_body(const DProgram<C>* prog, int bc, long long m0, int k, long long T, long long H,
-                              long long tile, const SI* in, long long i_ax, SO* out, long long o_ax, SK* sout,
-                              long long so_ax, C* P, C* Q) {
+__device__ __forceinline__ void tile_fwd_body(const DProgram<C>* prog, int bc, long long m0, int k, long long a,
+                                              long long b, long long H, const SI* in, long long i_ax, SO* out,
+                                              long long o_ax, SK* sout, long long so_ax, C* P, C* Q) {
   const int tid = LFX_TID, nthr = LFX_NTHR;
-  const long long a = tile * T, b = lmin(a + T, m0);
   const long long wa = lmax(0, a - H), wb = lmin(m0, b + H);
   long long g0 = wa / 2;
   int sw = (int)((wb + 1) / 2 - g0), dw = (int)(wb / 2 - g0);
@@ -186,13 +186,12 @@ __device__ __forceinline__ void tile_fwd_body(const DProgram<C>* prog, int bc, l
 }
 
 // Reverse tile: s_k from sin[i * si_ax], d_r from in[(ns_r + i) * i_ax]
-// -> tile [a, b) of the output band (length m0) to out[q * o_ax].
+// -> owned range [a, b) of the output band (length m0) to out[q * o_ax].
 template <class C, class SI, class SK, class SO>
-__device__ __forceinline__ void tile_rev_body(const DProgram<C>* prog, int bc, long long m0, int k, long long T, long long Hk,
-                              long long tile, const SI* in, long long i_ax, const SK* sin, long long si_ax,
-                              SO* out, long long o_ax, C* P, C* Q) {
+__device__ __forceinline__ void tile_rev_body(const DProgram<C>* prog, int bc, long long m0, int k, long long a,
+                                              long long b, long long Hk, const SI* in, long long i_ax, const SK* sin,
+                                              long long si_ax, SO* out, long long o_ax, C* P, C* Q) {
   const int tid = LFX_TID, nthr = LFX_NTHR;
-  const long long a = tile * T, b = lmin(a + T, m0);
   // level r lives in buf[r & 1]: level 1 (largest window) in P
   C* buf[2] = {Q, P};
   long long m = band_len(m0, k - 1);

@@ -98,12 +98,12 @@ struct TileLaunch {
 // three logical arrays (as in LineLaunch / TileLaunch) lives in one of the
 // user input, the user output or the plan's scratch, at an element offset.
 enum BatchArr : int32_t { kArrIn = 0, kArrOut = 1, kArrScr = 2 };
-enum BatchKind : int32_t { kItemLine = 0, kItemTile = 1 };
+enum BatchKind : int32_t { kItemLine = 0, kItemTile = 1, kItemReg = 2 };
 struct BatchItem {
   const void* prog;  // device program
   long long m0;      // band length
-  long long T, H;    // tile geometry (tile items)
-  long long tile;    // tile index (tile items)
+  long long a, b;    // owned range of the band (tile items)
+  long long H;       // halo (tile items)
   long long off[3];  // element offsets of the three logical arrays
   int32_t kind;      // BatchKind
   int32_t k;         // levels of this item

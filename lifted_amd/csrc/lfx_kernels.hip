@@ -70,8 +70,9 @@ __global__ void __launch_bounds__(kThreads) k_tile_fwd(const DProgram<C>* prog, 
   long long off[3];
   line_bases(lm, line, off);
   C* P = lds_base<C>();
-  tile_fwd_body<C, SI, SO, SK>(prog, bc, ta.m0, ta.k, ta.T, ta.H, tile, src + off[0], lm.ax[0], dst + off[1],
-                               lm.ax[1], sdst + off[2], lm.ax[2], P, P + cap0);
+  const long long a = tile * ta.T, b = lmin(a + ta.T, ta.m0);
+  tile_fwd_body<C, SI, SO, SK>(prog, bc, ta.m0, ta.k, a, b, ta.H, src + off[0], lm.ax[0], dst + off[1], lm.ax[1],
+                               sdst + off[2], lm.ax[2], P, P + cap0);
 }
 
 // reverse tiles: array 0 = input (d regions), 1 = output band, 2 = s_k in
@@ -84,8 +85,9 @@ __global__ void __launch_bounds__(kThreads) k_tile_rev(const DProgram<C>* prog, 
   long long off[3];
   line_bases(lm, line, off);
   C* P = lds_base<C>();
-  tile_rev_body<C, SI, SK, SO>(prog, bc, ta.m0, ta.k, ta.T, ta.H, tile, src + off[0], lm.ax[0], ssrc + off[2],
-                               lm.ax[2], dst + off[1], lm.ax[1], P, P + cap0);
+  const long long a = tile * ta.T, b = lmin(a + ta.T, ta.m0);
+  tile_rev_body<C, SI, SK, SO>(prog, bc, ta.m0, ta.k, a, b, ta.H, src + off[0], lm.ax[0], ssrc + off[2], lm.ax[2],
+                               dst + off[1], lm.ax[1], P, P + cap0);
 }
 
 // Strided elementwise copy over an ND box (used by the level-by-level inverse,
@@ -310,24 +312,24 @@ __device__ __forceinline__ void batch_item(const BatchItem& it, int bc, const Ar
     }
     return;
   }
-  const TileArgs ta{it.m0, it.T, it.H, 0, it.k, 0};
+  const TileArgs ta{it.m0, it.b - it.a, it.H, 0, it.k, 0};
   if (FWD) {
     C* Q = P + tile_fwd_window(ta);
     if (s0) {
-      if (s2) tile_fwd_body<C, C, S, C>(prog, bc, it.m0, it.k, it.T, it.H, it.tile, scr(0), 1, user_out(1), 1, scr(2), 1, P, Q);
-      else tile_fwd_body<C, C, S, S>(prog, bc, it.m0, it.k, it.T, it.H, it.tile, scr(0), 1, user_out(1), 1, user_out(2), 1, P, Q);
+      if (s2) tile_fwd_body<C, C, S, C>(prog, bc, it.m0, it.k, it.a, it.b, it.H, scr(0), 1, user_out(1), 1, scr(2), 1, P, Q);
+      else tile_fwd_body<C, C, S, S>(prog, bc, it.m0, it.k, it.a, it.b, it.H, scr(0), 1, user_out(1), 1, user_out(2), 1, P, Q);
     } else {
-      if (s2) tile_fwd_body<C, S, S, C>(prog, bc, it.m0, it.k, it.T, it.H, it.tile, user_in(0), 1, user_out(1), 1, scr(2), 1, P, Q);
-      else tile_fwd_body<C, S, S, S>(prog, bc, it.m0, it.k, it.T, it.H, it.tile, user_in(0), 1, user_out(1), 1, user_out(2), 1, P, Q);
+      if (s2) tile_fwd_body<C, S, S, C>(prog, bc, it.m0, it.k, it.a, it.b, it.H, user_in(0), 1, user_out(1), 1, scr(2), 1, P, Q);
+      else tile_fwd_body<C, S, S, S>(prog, bc, it.m0, it.k, it.a, it.b, it.H, user_in(0), 1, user_out(1), 1, user_out(2), 1, P, Q);
     }
   } else {
     C* Q = P + tile_rev_window(ta);
     if (s2) {
-      if (s1) tile_rev_body<C, S, C, C>(prog, bc, it.m0, it.k, it.T, it.H, it.tile, user_in(0), 1, scr(2), 1, scr(1), 1, P, Q);
-      else tile_rev_body<C, S, C, S>(prog, bc, it.m0, it.k, it.T, it.H, it.tile, user_in(0), 1, scr(2), 1, user_out(1), 1, P, Q);
+      if (s1) tile_rev_body<C, S, C, C>(prog, bc, it.m0, it.k, it.a, it.b, it.H, user_in(0), 1, scr(2), 1, scr(1), 1, P, Q);
+      else tile_rev_body<C, S, C, S>(prog, bc, it.m0, it.k, it.a, it.b, it.H, user_in(0), 1, scr(2), 1, user_out(1), 1, P, Q);
     } else {
-      if (s1) tile_rev_body<C, S, S, C>(prog, bc, it.m0, it.k, it.T, it.H, it.tile, user_in(0), 1, user_in(2), 1, scr(1), 1, P, Q);
-      else tile_rev_body<C, S, S, S>(prog, bc, it.m0, it.k, it.T, it.H, it.tile, user_in(0), 1, user_in(2), 1, user_out(1), 1, P, Q);
+      if (s1) tile_rev_body<C, S, S, C>(prog, bc, it.m0, it.k, it.a, it.b, it.H, user_in(0), 1, user_in(2), 1, scr(1), 1, P, Q);
+      else tile_rev_body<C, S, S, S>(prog, bc, it.m0, it.k, it.a, it.b, it.H, user_in(0), 1, user_in(2), 1, user_out(1), 1, P, Q);
     }
   }
 }

@@ -1,0 +1,283 @@
+// Register-tiled lifting: one wavefront owns one tile and runs four levels in
+// registers, with no LDS traffic and no barriers.
+//
+// Each of the 64 lanes holds 16 consecutive samples of the tile's window
+// (1024 samples). Level 1 splits them into 8 s and 8 d values per lane
+// (s[e] = x[2e], d[e] = x[2e+1]); level l + 1 splits s_l the same way, so after
+// four levels each lane holds one s_4 and one d_4. A lifting step
+//     y[e] += sum_k c[k] * x[e + off + k]
+// reads x values past the lane's own chunk from neighbouring lanes (shuffles
+// by whole chunks). The step shapes (kind, band, offset, taps) are compile-time
+// (lfx_wavelet_types.hpp); the coefficients come from the runtime program.
+// The window's outer lanes compute values that depend on samples outside the
+// window; the halo is sized on the host (lfx_tiles.hpp rules) so that only
+// values owned by the tile -- all exact -- are stored. Tiles never touch a
+// band end, so no boundary map or adjoint fold is involved; the parts of a
+// band next to its ends run through the LDS tile bodies.
+//
+// The lane policy P abstracts the wavefront: on the GPU a value is one float
+// per lane and cross-lane reads are shuffles; in the CPU simulation a value is
+// an array of 64 floats.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <utility>
+
+#include "lfx_device.hpp"
+#include "lfx_wavelet_types.hpp"
+
+namespace lfx {
+namespace reg {
+
+constexpr int kLanes = 64;
+constexpr int kLevels = 4;              // levels per register tile
+constexpr int kPerLane = 1 << kLevels;  // samples per lane at the finest level
+constexpr int kWindow = kLanes * kPerLane;
+
+#ifdef LFX_CPU_SIM
+// CPU simulation: a value is the 64 lanes of the wavefront.
+struct LaneCpu {
+  struct V {
+    float v[kLanes];
+  };
+  // value of x held by lane (lane + q); lanes outside the wavefront keep their
+  // own value, like the GPU shuffles
+  static V from(const V& x, int q) {
+    V r;
+    for (int l = 0; l < kLanes; ++l) {
+      const int src = l + q;
+      r.v[l] = (src >= 0 && src < kLanes) ? x.v[src] : x.v[l];
+    }
+    return r;
+  }
+  static V fma(float c, V x, V acc) {
+    for (int l = 0; l < kLanes; ++l) acc.v[l] = std::fma(c, x.v[l], acc.v[l]);
+    return acc;
+  }
+  static V add(V a, const V& b) {
+    for (int l = 0; l < kLanes; ++l) a.v[l] = a.v[l] + b.v[l];
+    return a;
+  }
+  static V sub(V a, const V& b) {
+    for (int l = 0; l < kLanes; ++l) a.v[l] = a.v[l] - b.v[l];
+    return a;
+  }
+  static V mul(V a, float c) {
+    for (int l = 0; l < kLanes; ++l) a.v[l] = a.v[l] * c;
+    return a;
+  }
+  template <class S, class F>
+  static V ld(const S* p, F idx) {
+    V r;
+    for (int l = 0; l < kLanes; ++l) r.v[l] = dev::load_as<S, float>(p, idx(l));
+    return r;
+  }
+  template <class S, class F, class G>
+  static void st(S* p, const V& v, F idx, G pred) {
+    for (int l = 0; l < kLanes; ++l)
+      if (pred(l)) dev::store_as<S, float>(p, idx(l), v.v[l]);
+  }
+};
+#else
+struct LaneGpu {
+  using V = float;
+  static __device__ __forceinline__ int lane() { return (int)(threadIdx.x & (kLanes - 1)); }
+  // value of v held by lane (lane + q)
+  static __device__ __forceinline__ V from(V v, int q) {
+    return q > 0 ? __shfl_down(v, (unsigned)q, kLanes) : __shfl_up(v, (unsigned)(-q), kLanes);
+  }
+  static __device__ __forceinline__ V fma(float c, V x, V acc) { return __builtin_fmaf(c, x, acc); }
+  static __device__ __forceinline__ V add(V a, V b) { return a + b; }
+  static __device__ __forceinline__ V sub(V a, V b) { return a - b; }
+  static __device__ __forceinline__ V mul(V a, float c) { return a * c; }
+  template <class S, class F>
+  static __device__ __forceinline__ V ld(const S* p, F idx) {
+    return dev::load_as<S, float>(p, idx(lane()));
+  }
+  template <class S, class F, class G>
+  static __device__ __forceinline__ void st(S* p, V v, F idx, G pred) {
+    const int l = lane();
+    if (pred(l)) dev::store_as<S, float>(p, idx(l), v);
+  }
+};
+#endif
+
+template <int A, int B>
+constexpr int fdiv() {
+  return (A >= 0) ? A / B : -((-A + B - 1) / B);
+}
+
+template <int... I, class F>
+__device__ __forceinline__ void sfor_impl(std::integer_sequence<int, I...>, F&& f) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+// f(integral_constant<int, i>) for i = 0 .. N-1, unrolled at compile time
+template <int N, class F>
+__device__ __forceinline__ void sfor(F&& f) {
+  sfor_impl(std::make_integer_sequence<int, N>{}, f);
+}
+
+// x value at chunk-relative index J (may lie in another lane's chunk)
+template <class P, int E, int J>
+__device__ __forceinline__ typename P::V xat(const typename P::V (&x)[E]) {
+  constexpr int q = fdiv<J, E>();
+  constexpr int r = J - q * E;
+  if constexpr (q == 0) return x[r];
+  else return P::from(x[r], q);
+}
+
+// One lifting step on E elements per lane: y[e] (+)= stencil of x.
+template <class P, int E, int KIND, int OFF, int N, bool SUB>
+__device__ __forceinline__ void rstep(typename P::V (&y)[E], const typename P::V (&x)[E], const float* cf) {
+  using V = typename P::V;
+  constexpr int NH = E + N - 1;  // x indices OFF .. OFF + E + N - 2
+  V hx[NH];
+  sfor<NH>([&](auto j) { hx[j] = xat<P, E, OFF + decltype(j)::value>(x); });
+  if constexpr (KIND == kGeneric) {
+    float c[N];
+    sfor<N>([&](auto k) { c[k] = cf[k]; });
+    sfor<E>([&](auto e) {
+      V acc = y[e];
+      sfor<N>([&](auto k) { acc = P::fma(c[k], hx[decltype(e)::value + decltype(k)::value], acc); });
+      y[e] = acc;
+    });
+  } else if constexpr (KIND == kRepeat) {
+    const float c0 = cf[0];
+    sfor<E>([&](auto e) {
+      V t = hx[e];
+      sfor<N - 1>([&](auto k) { t = P::add(t, hx[decltype(e)::value + decltype(k)::value + 1]); });
+      y[e] = P::fma(c0, t, y[e]);
+    });
+  } else {
+    sfor<E>([&](auto e) { y[e] = SUB ? P::sub(y[e], hx[e]) : P::add(y[e], hx[e]); });
+  }
+}
+
+// All steps of one level (wavelet W, transform TF) on E elements per lane.
+template <class P, class W, int TF, int E>
+__device__ __forceinline__ void level(typename P::V (&s)[E], typename P::V (&d)[E],
+                                      const DProgram<float>* __restrict__ prog) {
+  using L = typename W::list;
+  sfor<L::size>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    using ST = wt::Exec<L, TF, i>;
+    const float* cf = prog->coef + prog->steps[i].coef;
+    if constexpr (ST::kind == kScale) {
+      const float sm = cf[0], dm = cf[1];
+      sfor<E>([&](auto e) {
+        s[e] = P::mul(s[e], sm);
+        d[e] = P::mul(d[e], dm);
+      });
+    } else if constexpr (ST::y_is_d) {
+      rstep<P, E, ST::kind, ST::off, ST::n, ST::sub>(d, s, cf);
+    } else {
+      rstep<P, E, ST::kind, ST::off, ST::n, ST::sub>(s, d, cf);
+    }
+  });
+}
+
+template <class P, int E>
+__device__ __forceinline__ void split(const typename P::V (&x)[2 * E], typename P::V (&s)[E],
+                                      typename P::V (&d)[E]) {
+  sfor<E>([&](auto e) {
+    s[e] = x[2 * decltype(e)::value];
+    d[e] = x[2 * decltype(e)::value + 1];
+  });
+}
+template <class P, int E>
+__device__ __forceinline__ void merge(const typename P::V (&s)[E], const typename P::V (&d)[E],
+                                      typename P::V (&x)[2 * E]) {
+  sfor<E>([&](auto e) {
+    x[2 * decltype(e)::value] = s[e];
+    x[2 * decltype(e)::value + 1] = d[e];
+  });
+}
+
+// store the owned part [lo, hi) of a band held E per lane from global index g0
+template <class P, int E, class S>
+__device__ __forceinline__ void store_band(S* p, long long pos0, const typename P::V (&v)[E], long long g0,
+                                           long long lo, long long hi) {
+  sfor<E>([&](auto e) {
+    constexpr int ee = decltype(e)::value;
+    P::st(p, v[ee], [&](int l) { return pos0 + g0 + (long long)l * E + ee; },
+          [&](int l) {
+            const long long g = g0 + (long long)l * E + ee;
+            return g >= lo && g < hi;
+          });
+  });
+}
+
+// Forward tile (Forward / InverseAdjoint): owned [a, b) of the input band of
+// length m0, window [a - H, a - H + 1024) inside the band. Writes d_l of the
+// owned range to out[ns_l + i] (l = 1..4) and s_4 to sout[i].
+template <class P, class W, int TF, class SI, class SO, class SK>
+__device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long m0, long long a, long long b,
+                                         long long H, const SI* in, SO* out, SK* sout) {
+  using V = typename P::V;
+  const long long w0 = a - H;
+  V x[16];
+  sfor<16>([&](auto e) {
+    x[e] = P::ld(in, [&](int l) { return w0 + (long long)l * 16 + decltype(e)::value; });
+  });
+  V s1[8], d1[8], s2[4], d2[4], s3[2], d3[2], s4[1], d4[1];
+  split<P, 8>(x, s1, d1);
+  level<P, W, TF, 8>(s1, d1, prog);
+  long long m = m0;
+  long long ns = m - m / 2;
+  store_band<P, 8>(out, ns, d1, w0 / 2, a >> 1, b >> 1);
+  split<P, 4>(s1, s2, d2);
+  level<P, W, TF, 4>(s2, d2, prog);
+  m = ns;
+  ns = m - m / 2;
+  store_band<P, 4>(out, ns, d2, w0 / 4, a >> 2, b >> 2);
+  split<P, 2>(s2, s3, d3);
+  level<P, W, TF, 2>(s3, d3, prog);
+  m = ns;
+  ns = m - m / 2;
+  store_band<P, 2>(out, ns, d3, w0 / 8, a >> 3, b >> 3);
+  split<P, 1>(s3, s4, d4);
+  level<P, W, TF, 1>(s4, d4, prog);
+  m = ns;
+  ns = m - m / 2;
+  store_band<P, 1>(out, ns, d4, w0 / 16, a >> 4, b >> 4);
+  store_band<P, 1>(sout, 0, s4, w0 / 16, a >> 4, b >> 4);
+}
+
+// Reverse tile (Inverse / ForwardAdjoint): owned [a, b) of the output band of
+// length m0; level-4 windows of s_4 and d_4 are [g0, g0 + 64) with
+// g0 = (a >> 4) - Hk, inside both bands. Reads s_4 from sin[i] and d_l from
+// in[ns_l + i]; writes the owned band values to out[q].
+template <class P, class W, int TF, class SI, class SK, class SO>
+__device__ __forceinline__ void rev_tile(const DProgram<float>* prog, long long m0, long long a, long long b,
+                                         long long Hk, const SI* in, const SK* sin, SO* out) {
+  using V = typename P::V;
+  const long long g0 = (a >> 4) - Hk;
+  const long long ns1 = m0 - m0 / 2;
+  const long long ns2 = ns1 - ns1 / 2;
+  const long long ns3 = ns2 - ns2 / 2;
+  const long long ns4 = ns3 - ns3 / 2;
+  V s4[1], d4[1], s3[2], d3[2], s2[4], d2[4], s1[8], d1[8], x[16];
+  s4[0] = P::ld(sin, [&](int l) { return g0 + l; });
+  d4[0] = P::ld(in, [&](int l) { return ns4 + g0 + l; });
+  level<P, W, TF, 1>(s4, d4, prog);
+  merge<P, 1>(s4, d4, s3);
+  sfor<2>([&](auto e) {
+    d3[e] = P::ld(in, [&](int l) { return ns3 + 2 * g0 + (long long)l * 2 + decltype(e)::value; });
+  });
+  level<P, W, TF, 2>(s3, d3, prog);
+  merge<P, 2>(s3, d3, s2);
+  sfor<4>([&](auto e) {
+    d2[e] = P::ld(in, [&](int l) { return ns2 + 4 * g0 + (long long)l * 4 + decltype(e)::value; });
+  });
+  level<P, W, TF, 4>(s2, d2, prog);
+  merge<P, 4>(s2, d2, s1);
+  sfor<8>([&](auto e) {
+    d1[e] = P::ld(in, [&](int l) { return ns1 + 8 * g0 + (long long)l * 8 + decltype(e)::value; });
+  });
+  level<P, W, TF, 8>(s1, d1, prog);
+  merge<P, 8>(s1, d1, x);
+  store_band<P, 16>(out, 0, x, 16 * g0, a, b);
+}
+
+}  // namespace reg
+}  // namespace lfx

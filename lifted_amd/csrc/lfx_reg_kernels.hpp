@@ -1,0 +1,56 @@
+// Register-tile kernels of one transform type (included once per transform by
+// lfx_reg_tf{0,1,2,3}.hip, so the 34 wavelet instantiations of each transform
+// compile in parallel). One wavefront runs one BatchItem of kind register tile.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "lfx_kernels.hpp"
+#include "lfx_reg.hpp"
+
+namespace lfx {
+
+template <class W, int TF>
+__global__ void __launch_bounds__(256) k_reg(const BatchItem* items, long long nitems, const float* uin, float* uout,
+                                             float* scr) {
+  const long long w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  if (w >= nitems) return;
+  const BatchItem it = items[w];
+  const DProgram<float>* prog = reinterpret_cast<const DProgram<float>*>(it.prog);
+  float* base[3] = {const_cast<float*>(uin), uout, scr};
+  float* a0 = base[it.arr[0]] + it.off[0];
+  float* a1 = base[it.arr[1]] + it.off[1];
+  float* a2 = base[it.arr[2]] + it.off[2];
+  if constexpr (TF == kForward || TF == kInverseAdjoint)
+    reg::fwd_tile<reg::LaneGpu, W, TF>(prog, it.m0, it.a, it.b, it.H, (const float*)a0, a1, a2);
+  else
+    reg::rev_tile<reg::LaneGpu, W, TF>(prog, it.m0, it.a, it.b, it.H, (const float*)a0, (const float*)a2, a1);
+}
+
+template <int TF>
+hipError_t launch_reg_tf(uint32_t wavelet, const BatchItem* items, long long nitems, const void* in, void* out,
+                         void* scr, hipStream_t st) {
+  if (nitems <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)((nitems + 3) / 4);
+  switch (wavelet) {
+#define LFX_WAVELET(nm, idv, w)                                                                            \
+  case idv:                                                                                                \
+    hipLaunchKernelGGL((k_reg<wt::WL_##nm, TF>), dim3(blocks), dim3(256), 0, st, items, nitems,            \
+                       (const float*)in, (float*)out, (float*)scr);                                        \
+    return hipGetLastError();
+#define LFX_GENERIC(...)
+#define LFX_UNIT(...)
+#define LFX_REPEAT(...)
+#define LFX_SCALE(...)
+#define LFX_END(nm)
+#include "wavelet_table.inc"
+#undef LFX_WAVELET
+#undef LFX_GENERIC
+#undef LFX_UNIT
+#undef LFX_REPEAT
+#undef LFX_SCALE
+#undef LFX_END
+  }
+  return hipErrorNotSupported;
+}
+
+}  // namespace lfx

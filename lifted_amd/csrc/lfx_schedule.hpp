@@ -96,4 +96,134 @@ LineSchedule schedule(const DProgram<C>& hp, bool fwd, long long n, long long L,
   return s;
 }
 
+// ---------------------------------------------------------------------------
+// Work items of one group of levels of one signal.
+//
+// A tiled group of exactly four levels of a float32 table wavelet runs its
+// interior as register tiles (one wavefront per 1024-sample window,
+// lfx_reg.hpp) and the parts next to the band ends as LDS tiles; other groups
+// are LDS tiles of the schedule's tile length, or one whole-line item.
+
+enum ItemKind { kSpecLine = 0, kSpecTile = 1, kSpecReg = 2 };
+struct ItemSpec {
+  int kind;
+  long long m0;  // band length entering the group
+  int k;         // levels
+  long long a, b, H;
+};
+
+constexpr int kRegLevels = 4;
+constexpr long long kRegWindow = 1024;  // samples per register tile window
+
+// Halo of an interior register tile: forward H (finest-level samples, multiple
+// of 16, owned 1024 - 2H); reverse Hk (level-4 units, owned 16 (64 - 2 Hk)).
+// -1 when the stencil is too wide for the register window to pay off.
+template <class C>
+long long reg_halo(const DProgram<C>& hp, bool fwd) {
+  const long long m0 = 1ll << 40, a0 = 1ll << 36;
+  if (fwd) {
+    for (long long H = 0; 2 * H <= kRegWindow - 256; H += 16)
+      if (fwd_tile_ok(hp, m0, kRegLevels, a0, a0 + kRegWindow - 2 * H, H)) return H;
+  } else {
+    for (long long Hk = 0; 16 * (64 - 2 * Hk) >= 256; ++Hk)
+      if (rev_tile_ok(hp, m0, kRegLevels, a0, a0 + 16 * (64 - 2 * Hk), Hk)) return Hk;
+  }
+  return -1;
+}
+
+// Smallest LDS-tile halo for the owned range [a, b) (-1: none below the cap).
+template <class C>
+long long lds_halo(const DProgram<C>& hp, bool fwd, long long m0, int k, long long a, long long b, long long cap) {
+  const long long step = fwd ? (1ll << k) : 1;
+  for (long long H = 0; H <= cap; H += step) {
+    const bool ok = fwd ? fwd_tile_ok(hp, m0, k, a, b, H) : rev_tile_ok(hp, m0, k, a, b, H);
+    if (ok) return H;
+  }
+  return -1;
+}
+
+// Items of a tiled pass (band m0, k levels). `reg` allows register tiles.
+template <class C>
+std::vector<ItemSpec> pass_items(const DProgram<C>& hp, bool fwd, const PassPlan& pp, bool reg, int storage) {
+  std::vector<ItemSpec> out;
+  const long long m0 = pp.m0;
+  long long r0 = 0, r1 = 0, Tr = 0, H = -1;
+  if (reg && pp.k == kRegLevels) H = reg_halo(hp, fwd);
+  if (H >= 0) {
+    // Register windows stay strictly inside the band at every level (they
+    // apply no boundary map), with a margin of 64 samples at the finest level.
+    if (fwd) {
+      Tr = kRegWindow - 2 * H;
+      r0 = H + 64;
+      const long long last = m0 - kRegWindow + H - 64;  // window [a - H, a - H + 1024)
+      if (last >= r0) r1 = r0 + ((last - r0) / Tr + 1) * Tr;
+    } else {
+      Tr = 16 * (64 - 2 * H);
+      r0 = 16 * (H + 4);
+      // every level's window [2^(4-l) g0, 2^(4-l) (g0 + 64)) inside its d band, and s_4's inside ns_4
+      long long gmax = -1;
+      long long m = m0;
+      bool ok = true;
+      for (int l = 1; l <= kRegLevels; ++l) {
+        const long long nd = m / 2, ns = m - nd;
+        const long long span = 1ll << (kRegLevels - l);
+        const long long g = nd / span - 68;  // g0 + 64 < nd / span, with a margin
+        gmax = (gmax < 0 || g < gmax) ? g : gmax;
+        if (l == kRegLevels) gmax = std::min(gmax, ns - 68);
+        m = ns;
+        if (g < 0) ok = false;
+      }
+      if (ok && gmax >= 0) {
+        const long long last = 16 * (gmax + H);  // g0 = a / 16 - H <= gmax
+        if (last >= r0) r1 = r0 + ((last - r0) / Tr + 1) * Tr;
+      }
+    }
+  }
+  if (r1 > r0) {
+    // The halo was sized on a tile far from the band ends; tiles near an end
+    // also lose the elements that boundary folds and maps reach, so check the
+    // first and last tiles with the exact rule and move them inward until they
+    // hold (interior tiles are translates of a checked one).
+    auto ok = [&](long long a) {
+      return fwd ? fwd_tile_ok(hp, m0, pp.k, a, a + Tr, H) : rev_tile_ok(hp, m0, pp.k, a, a + Tr, H);
+    };
+    const long long ntl = (r1 - r0) / Tr;
+    long long first = 0, last = ntl - 1;
+    while (first <= last && !(ok(r0 + first * Tr) && ok(r0 + std::min(first + 1, last) * Tr) &&
+                              ok(r0 + std::min(first + 2, last) * Tr)))
+      ++first;
+    while (last >= first && !(ok(r0 + last * Tr) && ok(r0 + std::max(last - 1, first) * Tr) &&
+                              ok(r0 + std::max(last - 2, first) * Tr)))
+      --last;
+    if (last >= first) {
+      r1 = r0 + (last + 1) * Tr;
+      r0 = r0 + first * Tr;
+    } else {
+      r1 = r0;
+    }
+  }
+  if (r1 <= r0) {
+    for (long long t = 0; t < pp.ntiles; ++t) {
+      const long long a = t * pp.T, b = std::min(a + pp.T, m0);
+      out.push_back(ItemSpec{kSpecTile, m0, pp.k, a, b, pp.H});
+    }
+    return out;
+  }
+  const long long cap = std::max<long long>(4 * pp.T, 4096);
+  auto lds_range = [&](long long a, long long b) {
+    // cut [a, b) into pieces of at most pp.T, each with its own halo
+    for (long long x = a; x < b; x += pp.T) {
+      const long long y = std::min(b, x + pp.T);
+      const long long h = lds_halo(hp, fwd, m0, pp.k, x, y, cap);
+      if (h < 0) fail(LFX_EUNSUPPORTED, "no halo found for a band-end tile");
+      out.push_back(ItemSpec{kSpecTile, m0, pp.k, x, y, h});
+    }
+  };
+  if (r0 > 0) lds_range(0, r0);
+  for (long long a = r0; a < r1; a += Tr) out.push_back(ItemSpec{kSpecReg, m0, pp.k, a, a + Tr, H});
+  if (r1 < m0) lds_range(r1, m0);
+  (void)storage;
+  return out;
+}
+
 }  // namespace lfx

@@ -17,10 +17,54 @@
 #include <vector>
 
 #include "../../lifted_amd/csrc/lfx_bodies.hpp"
+#include "../../lifted_amd/csrc/lfx_reg.hpp"
 #include "../../lifted_amd/csrc/lfx_schedule.hpp"
 #include "../../lifted_amd/csrc/lfx_table.hpp"
 
 using namespace lfx;
+
+// register tile of wavelet `wid` (compile-time type) on the 64-lane CPU model
+template <int TF>
+static bool reg_item(uint32_t wid, const DProgram<float>& prog, const ItemSpec& it, const float* in, float* out,
+                     float* sk) {
+  constexpr bool fwd = TF == kForward || TF == kInverseAdjoint;
+  switch (wid) {
+#define LFX_WAVELET(nm, idv, w)                                                                          \
+  case idv:                                                                                              \
+    if constexpr (fwd) reg::fwd_tile<reg::LaneCpu, wt::WL_##nm, TF>(&prog, it.m0, it.a, it.b, it.H, in, out, sk); \
+    else reg::rev_tile<reg::LaneCpu, wt::WL_##nm, TF>(&prog, it.m0, it.a, it.b, it.H, in, sk, out);   \
+    return true;
+#define LFX_GENERIC(...)
+#define LFX_UNIT(...)
+#define LFX_REPEAT(...)
+#define LFX_SCALE(...)
+#define LFX_END(nm)
+#include "../../lifted_amd/csrc/wavelet_table.inc"
+#undef LFX_WAVELET
+#undef LFX_GENERIC
+#undef LFX_UNIT
+#undef LFX_REPEAT
+#undef LFX_SCALE
+#undef LFX_END
+  }
+  return false;
+}
+
+template <class C>
+static void reg_dispatch(int tf, uint32_t wid, const DProgram<C>& prog, const ItemSpec& it, const C* in, C* out, C* sk) {
+  if constexpr (sizeof(C) == 4) {
+    bool ok = false;
+    switch (tf) {
+      case 0: ok = reg_item<0>(wid, prog, it, in, out, sk); break;
+      case 1: ok = reg_item<1>(wid, prog, it, in, out, sk); break;
+      case 2: ok = reg_item<2>(wid, prog, it, in, out, sk); break;
+      case 3: ok = reg_item<3>(wid, prog, it, in, out, sk); break;
+    }
+    if (!ok) fail(LFX_EINVAL, "no register kernel for this wavelet");
+  } else {
+    fail(LFX_EINVAL, "register tiles are float32 only");
+  }
+}
 
 static WaveletDefL parse_custom(const std::string& spec) {
   WaveletDefL w{};
@@ -46,8 +90,8 @@ static WaveletDefL parse_custom(const std::string& spec) {
 }
 
 template <class C>
-static void run(const WaveletDefL& w, int tf, int bc, long long n, long long L, long long limit, long long tile, int k,
-                const std::vector<C>& in, std::vector<C>& out) {
+static void run(const WaveletDefL& w, bool use_reg, int tf, int bc, long long n, long long L, long long limit,
+                long long tile, int k, const std::vector<C>& in, std::vector<C>& out) {
   const DProgram<C> prog = resolve<C>(w, tf);
   const bool fwd = tf == kForward || tf == kInverseAdjoint;
   const int storage = sizeof(C) == 8 ? kF64 : kF32;
@@ -70,11 +114,17 @@ static void run(const WaveletDefL& w, int tf, int bc, long long n, long long L, 
       const PassPlan& pp = sc.passes[i];
       const bool last = (i + 1 == sc.passes.size()) && sc.L_line == 0;
       std::vector<C> sk(band_len(pp.m0, pp.k));
-      const TileArgs ta{pp.m0, pp.T, pp.H, pp.ntiles, pp.k, 0};
-      for (long long t = 0; t < pp.ntiles; ++t) {
+      for (const ItemSpec& it : pass_items(prog, true, pp, use_reg, storage)) {
+        C* skp = last ? out.data() : sk.data();
+        if (it.kind == kSpecReg && !std::getenv("LFX_SIM_REG_AS_LDS")) {
+          std::fprintf(stderr, "  reg [%lld, %lld) H=%lld\n", it.a, it.b, it.H);
+          reg_dispatch<C>(tf, w.id, prog, it, cur.data(), out.data(), skp);
+          continue;
+        }
+        const TileArgs ta{pp.m0, it.b - it.a, it.H, 0, pp.k, 0};
         std::vector<C> lds(tile_lds_bytes(ta, true, sizeof(C)) / sizeof(C));
-        tile_fwd_body<C, C, C, C>(&prog, bc, pp.m0, pp.k, pp.T, pp.H, t, cur.data(), 1, out.data(), 1,
-                                  last ? out.data() : sk.data(), 1, lds.data(), lds.data() + tile_fwd_window(ta));
+        tile_fwd_body<C, C, C, C>(&prog, bc, pp.m0, pp.k, it.a, it.b, it.H, cur.data(), 1, out.data(), 1, skp, 1,
+                                  lds.data(), lds.data() + tile_fwd_window(ta));
       }
       cur = sk;
     }
@@ -90,13 +140,19 @@ static void run(const WaveletDefL& w, int tf, int bc, long long n, long long L, 
   }
   for (size_t j = sc.passes.size(); j-- > 0;) {
     const PassPlan& pp = sc.passes[j];
-    const TileArgs ta{pp.m0, pp.T, pp.H, pp.ntiles, pp.k, 0};
     std::vector<C> band(j == 0 ? 0 : pp.m0);
     C* dst = j == 0 ? out.data() : band.data();
-    for (long long t = 0; t < pp.ntiles; ++t) {
+    for (const ItemSpec& it : pass_items(prog, false, pp, use_reg, storage)) {
+      const C* sp = s_user ? in.data() : s.data();
+      if (it.kind == kSpecReg && !std::getenv("LFX_SIM_REG_AS_LDS")) {
+        std::fprintf(stderr, "  reg [%lld, %lld) Hk=%lld\n", it.a, it.b, it.H);
+        reg_dispatch<C>(tf, w.id, prog, it, in.data(), dst, const_cast<C*>(sp));
+        continue;
+      }
+      const TileArgs ta{pp.m0, it.b - it.a, it.H, 0, pp.k, 0};
       std::vector<C> lds(tile_lds_bytes(ta, false, sizeof(C)) / sizeof(C));
-      tile_rev_body<C, C, C, C>(&prog, bc, pp.m0, pp.k, pp.T, pp.H, t, in.data(), 1, s_user ? in.data() : s.data(),
-                                1, dst, 1, lds.data(), lds.data() + tile_rev_window(ta));
+      tile_rev_body<C, C, C, C>(&prog, bc, pp.m0, pp.k, it.a, it.b, it.H, in.data(), 1, sp, 1, dst, 1, lds.data(),
+                                lds.data() + tile_rev_window(ta));
     }
     if (j > 0) {
       s = band;
@@ -116,17 +172,19 @@ int main(int argc, char** argv) {
     const long long n = std::atoll(argv[5]), L = std::atoll(argv[6]), limit = std::atoll(argv[7]);
     const long long tile = std::atoll(argv[8]);
     const int k = std::atoi(argv[9]);
+    // register tiles for table wavelets unless LFX_SIM_NOREG is set
+    const bool use_reg = wname != "custom" && !std::getenv("LFX_SIM_NOREG");
     const WaveletDefL w = wname == "custom" ? parse_custom(argc > 10 ? argv[10] : "")
                                             : find_wavelet((uint32_t)std::strtoul(wname.c_str(), nullptr, 10));
     if (dt == 1) {
       std::vector<double> in(n), out(n, 0.0);
       if (std::fread(in.data(), 8, n, stdin) != (size_t)n) return 3;
-      run<double>(w, tf, bc, n, L, limit, tile, k, in, out);
+      run<double>(w, false, tf, bc, n, L, limit, tile, k, in, out);
       std::fwrite(out.data(), 8, n, stdout);
     } else {
       std::vector<float> in(n), out(n, 0.0f);
       if (std::fread(in.data(), 4, n, stdin) != (size_t)n) return 3;
-      run<float>(w, tf, bc, n, L, limit, tile, k, in, out);
+      run<float>(w, use_reg, tf, bc, n, L, limit, tile, k, in, out);
       std::fwrite(out.data(), 4, n, stdout);
     }
   } catch (const Error& e) {

@@ -120,3 +120,16 @@ def test_whole_line_all_bcs(sim, oracle, w):
         ref = oracle.lwt(x, w, tf, bc, level=[L])
         got, info = run_sim(sim, w, tf, bc, x, L, limit=1 << 20, tile=0, k=0)
         np.testing.assert_array_equal(got, ref, err_msg=f"w={w} bc={bc} tf={tf} n={n}")
+
+
+@pytest.mark.parametrize("w", [w for w in WAVELETS if w != 0])
+def test_register_tiles_all_wavelets(sim, oracle, w):
+    """Passes of four levels whose interior runs as register tiles (64-lane
+    model of a wavefront) and whose band ends run as LDS tiles."""
+    g = np.random.default_rng(w + 7)
+    for bc, tf, n in itertools.product((0, 3), (0, 1, 2, 3), (30011, 24576)):
+        L = min(6, oracle.max_level(w, n))
+        x = g.standard_normal(n).astype(np.float32)
+        ref = oracle.lwt(x, w, tf, bc, level=[L])
+        got, info = run_sim(sim, w, tf, bc, x, L, limit=2000, tile=8192, k=4)
+        np.testing.assert_array_equal(got, ref, err_msg=f"w={w} bc={bc} tf={tf} n={n}\n{info}")
