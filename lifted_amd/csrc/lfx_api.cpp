@@ -222,11 +222,13 @@ static void copy_box(int storage, int ndim, const long long* shp, const void* sr
 struct Tuning {
   long long tile = 0;  // 0: default tile length
   int k = 0;           // 0: default levels per pass
+  long long line = 0;  // 0: default whole-line threshold of register-tiled plans
 };
 static Tuning env_tuning() {
   Tuning t;
   if (const char* e = std::getenv("LFX_TILE")) t.tile = std::atoll(e);
   if (const char* e = std::getenv("LFX_KPASS")) t.k = std::atoi(e);
+  if (const char* e = std::getenv("LFX_LINE")) t.line = std::atoll(e);
   return t;
 }
 static thread_local Tuning g_tuning = env_tuning();
@@ -545,6 +547,7 @@ struct BatchLaunch {
 };
 struct RegLaunch {
   uint32_t wavelet = 0;
+  const void* prog = nullptr;
   long long item0 = 0, nitems = 0;
 };
 struct BatchStage {
@@ -580,7 +583,10 @@ static std::vector<Group> signal_groups(const WaveletDefL& w, int tf, int storag
   const bool fwd = tf == kForward || tf == kInverseAdjoint;
   std::vector<Group> g;
   const int k_hint = plain ? 1 : (g_tuning.k > 0 ? g_tuning.k : (reg ? kRegLevels : 0));
-  const LineSchedule sc = schedule(hp, fwd, n, L, storage, bc, g_tuning.tile, k_hint);
+  // with register tiles, bands longer than the whole-line threshold are tiled so
+  // that whole-line items stay small (several workgroups per CU)
+  const long long limit = reg ? (g_tuning.line > 0 ? g_tuning.line : 4096) : 0;
+  const LineSchedule sc = schedule(hp, fwd, n, L, storage, bc, g_tuning.tile, k_hint, bc == kPeriodic ? 0 : limit);
   for (auto& p : sc.passes) g.push_back(Group{true, p.m0, p.k, p.T, p.H, p.ntiles});
   if (sc.L_line > 0) {
     if (plain) {
@@ -801,6 +807,7 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
       for (size_t j = 0; j < rs.size();) {
         RegLaunch rl;
         rl.wavelet = rs[j].first;
+        rl.prog = rs[j].second.prog;
         rl.item0 = (long long)all.size();
         while (j < rs.size() && rs[j].first == rl.wavelet) all.push_back(rs[j++].second);
         rl.nitems = (long long)all.size() - rl.item0;
@@ -831,7 +838,8 @@ static void batch_stage(BatchPlanImpl* p, int st, const void* in, void* out, hip
   if (dev != p->dev) fail(LFX_EINVAL, "plan was created on another device");
   const BatchStage& bs = p->stages[st];
   for (const RegLaunch& rl : bs.regs)
-    hip_check(launch_reg(p->tf, rl.wavelet, p->d_items + rl.item0, rl.nitems, in, out, p->d_scr, s), "register kernel");
+    hip_check(launch_reg(p->tf, rl.wavelet, rl.prog, p->d_items + rl.item0, rl.nitems, in, out, p->d_scr, s),
+              "register kernel");
   for (const BatchLaunch& bl : bs.launches)
     hip_check(launch_batch(p->storage, p->fwd, p->bc, p->d_items + bl.item0, bl.nitems, bl.lds, in, out, p->d_scr, s),
               "batch kernel");

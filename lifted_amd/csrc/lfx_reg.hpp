@@ -77,6 +77,19 @@ struct LaneCpu {
     for (int l = 0; l < kLanes; ++l)
       if (pred(l)) dev::store_as<S, float>(p, idx(l), v.v[l]);
   }
+  template <int E, class S>
+  static void ld_chunks(const S* p, long long base, V (&x)[E], float*) {
+    for (int e = 0; e < E; ++e)
+      for (int l = 0; l < kLanes; ++l) x[e].v[l] = dev::load_as<S, float>(p, base + (long long)E * l + e);
+  }
+  template <int E, class S>
+  static void st_chunks(S* p, long long pos0, const V (&v)[E], long long g0, long long lo, long long hi, float*) {
+    for (int e = 0; e < E; ++e)
+      for (int l = 0; l < kLanes; ++l) {
+        const long long g = g0 + (long long)E * l + e;
+        if (g >= lo && g < hi) dev::store_as<S, float>(p, pos0 + g, v[e].v[l]);
+      }
+  }
 };
 #else
 struct LaneGpu {
@@ -98,6 +111,52 @@ struct LaneGpu {
   static __device__ __forceinline__ void st(S* p, V v, F idx, G pred) {
     const int l = lane();
     if (pred(l)) dev::store_as<S, float>(p, idx(l), v);
+  }
+  static __device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  // x[e] = p[base + E * lane + e] for e < E: coalesced global loads (lane l
+  // reads base + 64 j + l), transposed through the wave's LDS buffer (row
+  // stride E + 1 keeps the chunk reads conflict-free).
+  template <int E, class S>
+  static __device__ __forceinline__ void ld_chunks(const S* p, long long base, V (&x)[E], float* lds) {
+    const int l = lane();
+    const S* q = p + base;
+    float t[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j) t[j] = dev::load_as<S, float>(q, j * kLanes + l);  // all loads in flight
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const int u = j * kLanes + l;
+      lds[(u / E) * (E + 1) + (u % E)] = t[j];
+    }
+    wave_sync();
+#pragma unroll
+    for (int e = 0; e < E; ++e) x[e] = lds[l * (E + 1) + e];
+    wave_sync();
+  }
+  // p[pos0 + g] = v for the owned globals g = g0 + E * lane + e in [lo, hi),
+  // transposed through LDS so that the global stores are coalesced.
+  template <int E, class S>
+  static __device__ __forceinline__ void st_chunks(S* p, long long pos0, const V (&v)[E], long long g0, long long lo,
+                                                   long long hi, float* lds) {
+    const int l = lane();
+#pragma unroll
+    for (int e = 0; e < E; ++e) lds[l * (E + 1) + e] = v[e];
+    wave_sync();
+    // owned window-relative range [t0, t1) of the E * 64 values
+    const int t0 = (int)(lo - g0 < 0 ? 0 : lo - g0);
+    const int t1 = (int)(hi - g0 > E * kLanes ? E * kLanes : hi - g0);
+    S* q = p + pos0 + g0;
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const int u = j * kLanes + l;
+      const float val = lds[(u / E) * (E + 1) + (u % E)];
+      if (u >= t0 && u < t1) dev::store_as<S, float>(q, u, val);
+    }
+    wave_sync();
   }
 };
 #endif
@@ -212,29 +271,27 @@ __device__ __forceinline__ void store_band(S* p, long long pos0, const typename 
 // owned range to out[ns_l + i] (l = 1..4) and s_4 to sout[i].
 template <class P, class W, int TF, class SI, class SO, class SK>
 __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long m0, long long a, long long b,
-                                         long long H, const SI* in, SO* out, SK* sout) {
+                                         long long H, const SI* in, SO* out, SK* sout, float* lds) {
   using V = typename P::V;
   const long long w0 = a - H;
   V x[16];
-  sfor<16>([&](auto e) {
-    x[e] = P::ld(in, [&](int l) { return w0 + (long long)l * 16 + decltype(e)::value; });
-  });
+  P::template ld_chunks<16>(in, w0, x, lds);
   V s1[8], d1[8], s2[4], d2[4], s3[2], d3[2], s4[1], d4[1];
   split<P, 8>(x, s1, d1);
   level<P, W, TF, 8>(s1, d1, prog);
   long long m = m0;
   long long ns = m - m / 2;
-  store_band<P, 8>(out, ns, d1, w0 / 2, a >> 1, b >> 1);
+  P::template st_chunks<8>(out, ns, d1, w0 / 2, a >> 1, b >> 1, lds);
   split<P, 4>(s1, s2, d2);
   level<P, W, TF, 4>(s2, d2, prog);
   m = ns;
   ns = m - m / 2;
-  store_band<P, 4>(out, ns, d2, w0 / 4, a >> 2, b >> 2);
+  P::template st_chunks<4>(out, ns, d2, w0 / 4, a >> 2, b >> 2, lds);
   split<P, 2>(s2, s3, d3);
   level<P, W, TF, 2>(s3, d3, prog);
   m = ns;
   ns = m - m / 2;
-  store_band<P, 2>(out, ns, d3, w0 / 8, a >> 3, b >> 3);
+  P::template st_chunks<2>(out, ns, d3, w0 / 8, a >> 3, b >> 3, lds);
   split<P, 1>(s3, s4, d4);
   level<P, W, TF, 1>(s4, d4, prog);
   m = ns;
@@ -249,7 +306,7 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long 
 // in[ns_l + i]; writes the owned band values to out[q].
 template <class P, class W, int TF, class SI, class SK, class SO>
 __device__ __forceinline__ void rev_tile(const DProgram<float>* prog, long long m0, long long a, long long b,
-                                         long long Hk, const SI* in, const SK* sin, SO* out) {
+                                         long long Hk, const SI* in, const SK* sin, SO* out, float* lds) {
   using V = typename P::V;
   const long long g0 = (a >> 4) - Hk;
   const long long ns1 = m0 - m0 / 2;
@@ -261,22 +318,16 @@ __device__ __forceinline__ void rev_tile(const DProgram<float>* prog, long long 
   d4[0] = P::ld(in, [&](int l) { return ns4 + g0 + l; });
   level<P, W, TF, 1>(s4, d4, prog);
   merge<P, 1>(s4, d4, s3);
-  sfor<2>([&](auto e) {
-    d3[e] = P::ld(in, [&](int l) { return ns3 + 2 * g0 + (long long)l * 2 + decltype(e)::value; });
-  });
+  P::template ld_chunks<2>(in, ns3 + 2 * g0, d3, lds);
   level<P, W, TF, 2>(s3, d3, prog);
   merge<P, 2>(s3, d3, s2);
-  sfor<4>([&](auto e) {
-    d2[e] = P::ld(in, [&](int l) { return ns2 + 4 * g0 + (long long)l * 4 + decltype(e)::value; });
-  });
+  P::template ld_chunks<4>(in, ns2 + 4 * g0, d2, lds);
   level<P, W, TF, 4>(s2, d2, prog);
   merge<P, 4>(s2, d2, s1);
-  sfor<8>([&](auto e) {
-    d1[e] = P::ld(in, [&](int l) { return ns1 + 8 * g0 + (long long)l * 8 + decltype(e)::value; });
-  });
+  P::template ld_chunks<8>(in, ns1 + 8 * g0, d1, lds);
   level<P, W, TF, 8>(s1, d1, prog);
   merge<P, 8>(s1, d1, x);
-  store_band<P, 16>(out, 0, x, 16 * g0, a, b);
+  P::template st_chunks<16>(out, 0, x, 16 * g0, a, b, lds);
 }
 
 }  // namespace reg

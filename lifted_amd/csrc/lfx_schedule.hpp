@@ -68,7 +68,7 @@ template <class C>
 LineSchedule schedule(const DProgram<C>& hp, bool fwd, long long n, long long L, int storage, int bc,
                       long long tile_hint, int k_hint, long long limit_override = 0) {
   LineSchedule s{{}, n, L};
-  const long long limit = limit_override > 0 ? limit_override : line_limit(storage);
+  const long long limit = limit_override > 0 ? std::min(limit_override, line_limit(storage)) : line_limit(storage);
   if (n <= limit || L == 0) return s;
   if (bc == kPeriodic)
     fail(LFX_EUNSUPPORTED, "Periodic boundary on lines longer than " + std::to_string(limit) +
