@@ -128,7 +128,10 @@ typedef struct lfx_batch_plan_s* lfx_batch_plan;
 
 int lfx_batch_plan_create(const lfx_signal* signals, int32_t nsignals, uint32_t transform, uint32_t bc,
                           int32_t storage, uint32_t flags, lfx_batch_plan* plan);
-/* All stages, stream-ordered; in and out must not alias. */
+/* All stages, stream-ordered; in and out must not alias. The launches of a
+ * stage are spread over side streams the plan owns, forked from and joined
+ * back to `stream` with events (graph capture of `stream` captures them too);
+ * one plan must not be executed from two host threads at once. */
 int lfx_batch_execute(lfx_batch_plan plan, const void* in_dev, void* out_dev, void* stream);
 int lfx_batch_num_stages(lfx_batch_plan plan);
 int lfx_batch_execute_stage(lfx_batch_plan plan, int32_t stage, const void* in_dev, void* out_dev, void* stream);

@@ -220,15 +220,19 @@ static void copy_box(int storage, int ndim, const long long* shp, const void* sr
 }
 
 struct Tuning {
-  long long tile = 0;  // 0: default tile length
-  int k = 0;           // 0: default levels per pass
-  long long line = 0;  // 0: default whole-line threshold of register-tiled plans
+  long long tile = 0;    // 0: default tile length
+  int k = 0;             // 0: default levels per pass
+  long long line = 0;    // 0: default whole-line threshold of register-tiled plans
+  int streams = -1;      // side streams of a batched stage (-1: default)
+  long long small = -1;  // LDS bytes up to which batch items run as one-wave workgroups (-1: default)
 };
 static Tuning env_tuning() {
   Tuning t;
   if (const char* e = std::getenv("LFX_TILE")) t.tile = std::atoll(e);
   if (const char* e = std::getenv("LFX_KPASS")) t.k = std::atoi(e);
   if (const char* e = std::getenv("LFX_LINE")) t.line = std::atoll(e);
+  if (const char* e = std::getenv("LFX_STREAMS")) t.streams = std::atoi(e);
+  if (const char* e = std::getenv("LFX_SMALL")) t.small = std::atoll(e);
   return t;
 }
 static thread_local Tuning g_tuning = env_tuning();
@@ -544,6 +548,7 @@ static int lwt_overload(uint32_t wavelet, uint32_t transform, uint32_t bc, size_
 struct BatchLaunch {
   long long item0 = 0, nitems = 0;
   size_t lds = 0;
+  int threads = 0;  // 0: default workgroup size
 };
 struct RegLaunch {
   uint32_t wavelet = 0;
@@ -567,6 +572,12 @@ struct BatchPlanImpl {
   BatchItem* d_items = nullptr;
   void* d_scr = nullptr;
   size_t scr_bytes = 0;
+  // side streams: the launches of one stage are independent (disjoint owned
+  // outputs), so they are spread over the caller's stream and these, joined
+  // by events before the next stage
+  std::vector<hipStream_t> side;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
 };
 
 struct Group {
@@ -601,6 +612,9 @@ static std::vector<Group> signal_groups(const WaveletDefL& w, int tf, int storag
   }
   return g;  // fine -> coarse
 }
+
+static void free_plan(BatchPlanImpl* p);
+static void plan_streams(BatchPlanImpl* p);
 
 static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int bc, int storage, uint32_t flags) {
   if (tf < 0 || tf > 3) fail(LFX_EINVAL, "unknown transform");
@@ -775,22 +789,27 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
   }
   // each stage: items grouped into LDS classes (one launch per class, so small
   // tiles are not launched with the LDS reservation of the longest line), each
-  // class ordered by decreasing cost; upload
+  // class ordered by decreasing cost; upload. Items of the smallest class
+  // (band-end pieces, short lines) are latency-bound: they run as one-wave
+  // workgroups, so that four times as many are in flight and their barriers
+  // are wavefront-local.
   std::vector<BatchItem> all;
-  const size_t kClasses[] = {40 * 1024, 80 * 1024, kMaxLdsBytes};
+  const size_t small = g_tuning.small >= 0 ? (size_t)g_tuning.small : 8 * 1024;
+  const size_t kClasses[] = {small, 40 * 1024, 80 * 1024, kMaxLdsBytes};
   for (size_t st = 0; st < per_stage.size(); ++st) {
     BatchStage bs;
     bs.alg_bytes = bytes[st];
-    for (int c = 2; c >= 0; --c) {
+    for (int c = 3; c >= 0; --c) {
       std::vector<size_t> idx;
       for (size_t j = 0; j < per_stage[st].size(); ++j) {
         const size_t l = item_lds[st][j];
-        const int cls = l <= kClasses[0] ? 0 : (l <= kClasses[1] ? 1 : 2);
+        const int cls = l <= kClasses[0] ? 0 : (l <= kClasses[1] ? 1 : (l <= kClasses[2] ? 2 : 3));
         if (cls == c) idx.push_back(j);
       }
       if (idx.empty()) continue;
       std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return cost[st][a] > cost[st][b]; });
       BatchLaunch bl;
+      bl.threads = c == 0 ? 64 : 0;
       bl.item0 = (long long)all.size();
       bl.nitems = (long long)idx.size();
       for (size_t j : idx) {
@@ -825,10 +844,43 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
   plan->scr_bytes = (size_t)std::max<long long>(scr_elems, 1) * cs;
   if (hipMalloc(&plan->d_scr, plan->scr_bytes) != hipSuccess) {
     (void)hipGetLastError();
-    if (plan->d_items) (void)hipFree(plan->d_items);
+    plan->d_scr = nullptr;
+    free_plan(plan.release());
     fail(LFX_ENOMEM, "batch scratch allocation failed");
   }
+  try {
+    plan_streams(plan.get());
+  } catch (...) {
+    free_plan(plan.release());
+    throw;
+  }
   return plan.release();
+}
+
+static void free_plan(BatchPlanImpl* p) {
+  for (hipStream_t q : p->side) (void)hipStreamSynchronize(q);
+  if (p->d_items) (void)hipFree(p->d_items);
+  if (p->d_scr) (void)hipFree(p->d_scr);
+  for (hipStream_t q : p->side) (void)hipStreamDestroy(q);
+  for (hipEvent_t e : p->join) (void)hipEventDestroy(e);
+  if (p->fork) (void)hipEventDestroy(p->fork);
+  delete p;
+}
+
+// created with the plan, not at first execution: stream and event creation is
+// not allowed while the caller's stream is being captured into a graph
+static void plan_streams(BatchPlanImpl* p) {
+  if (p->fork) return;
+  const int n = g_tuning.streams >= 0 ? std::min(g_tuning.streams, 8) : 3;
+  hip_check(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming), "hipEventCreate");
+  for (int i = 0; i < n; ++i) {
+    hipStream_t q = nullptr;
+    hipEvent_t e = nullptr;
+    hip_check(hipStreamCreateWithFlags(&q, hipStreamNonBlocking), "hipStreamCreate");
+    p->side.push_back(q);
+    hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    p->join.push_back(e);
+  }
 }
 
 static void batch_stage(BatchPlanImpl* p, int st, const void* in, void* out, hipStream_t s) {
@@ -837,12 +889,35 @@ static void batch_stage(BatchPlanImpl* p, int st, const void* in, void* out, hip
   hip_check(hipGetDevice(&dev), "hipGetDevice");
   if (dev != p->dev) fail(LFX_EINVAL, "plan was created on another device");
   const BatchStage& bs = p->stages[st];
-  for (const RegLaunch& rl : bs.regs)
-    hip_check(launch_reg(p->tf, rl.wavelet, rl.prog, p->d_items + rl.item0, rl.nitems, in, out, p->d_scr, s),
-              "register kernel");
+  const int nl = (int)(bs.launches.size() + bs.regs.size());
+  const int nside = std::min<int>((int)p->side.size(), nl - 1);
+  std::vector<hipStream_t> lanes{s};
+  if (nside > 0) {
+    hip_check(hipEventRecord(p->fork, s), "hipEventRecord");
+    for (int i = 0; i < nside; ++i) {
+      hip_check(hipStreamWaitEvent(p->side[i], p->fork, 0), "hipStreamWaitEvent");
+      lanes.push_back(p->side[i]);
+    }
+  }
+  // latency-bound LDS launches first, on the side streams; register launches
+  // round-robin over all lanes
+  int li = nside > 0 ? 1 : 0;
+  auto next = [&] {
+    hipStream_t q = lanes[li];
+    li = (li + 1) % (int)lanes.size();
+    return q;
+  };
   for (const BatchLaunch& bl : bs.launches)
-    hip_check(launch_batch(p->storage, p->fwd, p->bc, p->d_items + bl.item0, bl.nitems, bl.lds, in, out, p->d_scr, s),
+    hip_check(launch_batch(p->storage, p->fwd, p->bc, p->d_items + bl.item0, bl.nitems, bl.lds, bl.threads, in, out,
+                           p->d_scr, next()),
               "batch kernel");
+  for (const RegLaunch& rl : bs.regs)
+    hip_check(launch_reg(p->tf, rl.wavelet, rl.prog, p->d_items + rl.item0, rl.nitems, in, out, p->d_scr, next()),
+              "register kernel");
+  for (int i = 0; i < nside; ++i) {
+    hip_check(hipEventRecord(p->join[i], p->side[i]), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(s, p->join[i], 0), "hipStreamWaitEvent");
+  }
 }
 
 }  // namespace lfx
@@ -997,9 +1072,7 @@ int lfx_batch_stage_info(lfx_batch_plan plan, int32_t stage, int64_t* nitems, do
 void lfx_batch_plan_destroy(lfx_batch_plan plan) {
   BatchPlanImpl* p = reinterpret_cast<BatchPlanImpl*>(plan);
   if (!p) return;
-  if (p->d_items) (void)hipFree(p->d_items);
-  if (p->d_scr) (void)hipFree(p->d_scr);
-  delete p;
+  free_plan(p);
 }
 
 int lfx_set_tuning(int64_t tile, int32_t levels_per_pass) {

@@ -345,24 +345,24 @@ __global__ void __launch_bounds__(kThreads) k_batch(const BatchItem* items, long
 }
 
 template <int SI, bool FWD>
-static hipError_t batch_t(int bc, const BatchItem* items, long long nitems, size_t lds, const void* in, void* out,
-                          void* scr, hipStream_t st) {
+static hipError_t batch_t(int bc, const BatchItem* items, long long nitems, size_t lds, int threads, const void* in,
+                          void* out, void* scr, hipStream_t st) {
   using C = typename StoreT<SI>::C;
   using S = typename StoreT<SI>::T;
   auto kf = k_batch<C, S, FWD>;
   hipError_t e = ensure_lds(kf);
   if (e != hipSuccess) return e;
   return grid_loop(nitems, [&](long long b0, unsigned cnt) {
-    hipLaunchKernelGGL(kf, dim3(cnt), dim3(block_threads()), lds, st, items, nitems, bc, reinterpret_cast<const S*>(in),
-                       reinterpret_cast<S*>(out), reinterpret_cast<C*>(scr), b0);
+    hipLaunchKernelGGL(kf, dim3(cnt), dim3(threads > 0 ? threads : block_threads()), lds, st, items, nitems, bc,
+                       reinterpret_cast<const S*>(in), reinterpret_cast<S*>(out), reinterpret_cast<C*>(scr), b0);
   });
 }
 
 hipError_t launch_batch(int storage, bool fwd, int bc, const BatchItem* items, long long nitems, size_t lds_bytes,
-                        const void* in, void* out, void* scr, hipStream_t st) {
+                        int threads, const void* in, void* out, void* scr, hipStream_t st) {
   if (nitems <= 0) return hipSuccess;
-#define CALL_B(S) fwd ? batch_t<S, true>(bc, items, nitems, lds_bytes, in, out, scr, st) \
-                      : batch_t<S, false>(bc, items, nitems, lds_bytes, in, out, scr, st)
+#define CALL_B(S) fwd ? batch_t<S, true>(bc, items, nitems, lds_bytes, threads, in, out, scr, st) \
+                      : batch_t<S, false>(bc, items, nitems, lds_bytes, threads, in, out, scr, st)
   LFX_DISPATCH_STORAGE(storage, CALL_B)
 #undef CALL_B
 }

@@ -10,8 +10,9 @@ hipError_t launch_line(const LineLaunch& ll, hipStream_t st);
 hipError_t launch_tile(const TileLaunch& tl, hipStream_t st);
 // One batched stage: `nitems` items (sorted by decreasing cost), arrays
 // `in` / `out` in storage type, `scr` in compute type.
+// threads: workgroup size (0: the default of block_threads())
 hipError_t launch_batch(int storage, bool fwd, int bc, const BatchItem* items, long long nitems, size_t lds_bytes,
-                        const void* in, void* out, void* scr, hipStream_t st);
+                        int threads, const void* in, void* out, void* scr, hipStream_t st);
 // Register-tile items (float32) of one wavelet and transform type.
 hipError_t launch_reg_0(uint32_t wavelet, const void* prog, const BatchItem* items, long long nitems,
                         const void* in, void* out, void* scr, hipStream_t st);

@@ -1,5 +1,6 @@
-// Register-tiled lifting: one wavefront owns one tile and runs four levels in
-// registers, with no LDS traffic and no barriers.
+// Register-tiled lifting: one wavefront owns one tile and runs up to four
+// levels in registers; LDS serves only to transpose global loads and stores
+// into coalesced ones (no workgroup barriers).
 //
 // Each of the 64 lanes holds 16 consecutive samples of the tile's window
 // (1024 samples). Level 1 splits them into 8 s and 8 d values per lane
@@ -266,11 +267,12 @@ __device__ __forceinline__ void store_band(S* p, long long pos0, const typename 
   });
 }
 
-// Forward tile (Forward / InverseAdjoint): owned [a, b) of the input band of
-// length m0, window [a - H, a - H + 1024) inside the band. Writes d_l of the
-// owned range to out[ns_l + i] (l = 1..4) and s_4 to sout[i].
+// Forward tile (Forward / InverseAdjoint) of K (1..4) levels: owned [a, b)
+// of the input band of length m0, window [a - H, a - H + 1024) inside the
+// band. Writes d_l of the owned range to out[ns_l + i] (l = 1..K) and s_K to
+// sout[i]. K is uniform over the wavefront.
 template <class P, class W, int TF, class SI, class SO, class SK>
-__device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long m0, long long a, long long b,
+__device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long m0, int K, long long a, long long b,
                                          long long H, const SI* in, SO* out, SK* sout, float* lds) {
   using V = typename P::V;
   const long long w0 = a - H;
@@ -282,16 +284,28 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long 
   long long m = m0;
   long long ns = m - m / 2;
   P::template st_chunks<8>(out, ns, d1, w0 / 2, a >> 1, b >> 1, lds);
+  if (K == 1) {
+    P::template st_chunks<8>(sout, 0, s1, w0 / 2, a >> 1, b >> 1, lds);
+    return;
+  }
   split<P, 4>(s1, s2, d2);
   level<P, W, TF, 4>(s2, d2, prog);
   m = ns;
   ns = m - m / 2;
   P::template st_chunks<4>(out, ns, d2, w0 / 4, a >> 2, b >> 2, lds);
+  if (K == 2) {
+    P::template st_chunks<4>(sout, 0, s2, w0 / 4, a >> 2, b >> 2, lds);
+    return;
+  }
   split<P, 2>(s2, s3, d3);
   level<P, W, TF, 2>(s3, d3, prog);
   m = ns;
   ns = m - m / 2;
   P::template st_chunks<2>(out, ns, d3, w0 / 8, a >> 3, b >> 3, lds);
+  if (K == 3) {
+    P::template st_chunks<2>(sout, 0, s3, w0 / 8, a >> 3, b >> 3, lds);
+    return;
+  }
   split<P, 1>(s3, s4, d4);
   level<P, W, TF, 1>(s4, d4, prog);
   m = ns;
@@ -300,12 +314,13 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long 
   store_band<P, 1>(sout, 0, s4, w0 / 16, a >> 4, b >> 4);
 }
 
-// Reverse tile (Inverse / ForwardAdjoint): owned [a, b) of the output band of
-// length m0; level-4 windows of s_4 and d_4 are [g0, g0 + 64) with
-// g0 = (a >> 4) - Hk, inside both bands. Reads s_4 from sin[i] and d_l from
-// in[ns_l + i]; writes the owned band values to out[q].
+// Reverse tile (Inverse / ForwardAdjoint) of K (1..4) levels: owned [a, b) of
+// the output band of length m0. Windows are counted in level-4 units whatever
+// K is: with g0 = (a >> 4) - Hk, the level-l windows are
+// [2^(4-l) g0, 2^(4-l) (g0 + 64)), inside their bands. Reads s_K from sin[i]
+// and d_l from in[ns_l + i]; writes the owned band values to out[q].
 template <class P, class W, int TF, class SI, class SK, class SO>
-__device__ __forceinline__ void rev_tile(const DProgram<float>* prog, long long m0, long long a, long long b,
+__device__ __forceinline__ void rev_tile(const DProgram<float>* prog, long long m0, int K, long long a, long long b,
                                          long long Hk, const SI* in, const SK* sin, SO* out, float* lds) {
   using V = typename P::V;
   const long long g0 = (a >> 4) - Hk;
@@ -314,16 +329,28 @@ __device__ __forceinline__ void rev_tile(const DProgram<float>* prog, long long 
   const long long ns3 = ns2 - ns2 / 2;
   const long long ns4 = ns3 - ns3 / 2;
   V s4[1], d4[1], s3[2], d3[2], s2[4], d2[4], s1[8], d1[8], x[16];
-  s4[0] = P::ld(sin, [&](int l) { return g0 + l; });
-  d4[0] = P::ld(in, [&](int l) { return ns4 + g0 + l; });
-  level<P, W, TF, 1>(s4, d4, prog);
-  merge<P, 1>(s4, d4, s3);
-  P::template ld_chunks<2>(in, ns3 + 2 * g0, d3, lds);
-  level<P, W, TF, 2>(s3, d3, prog);
-  merge<P, 2>(s3, d3, s2);
-  P::template ld_chunks<4>(in, ns2 + 4 * g0, d2, lds);
-  level<P, W, TF, 4>(s2, d2, prog);
-  merge<P, 4>(s2, d2, s1);
+  if (K == 4) {
+    s4[0] = P::ld(sin, [&](int l) { return g0 + l; });
+    d4[0] = P::ld(in, [&](int l) { return ns4 + g0 + l; });
+    level<P, W, TF, 1>(s4, d4, prog);
+    merge<P, 1>(s4, d4, s3);
+  } else if (K == 3) {
+    P::template ld_chunks<2>(sin, 2 * g0, s3, lds);
+  }
+  if (K >= 3) {
+    P::template ld_chunks<2>(in, ns3 + 2 * g0, d3, lds);
+    level<P, W, TF, 2>(s3, d3, prog);
+    merge<P, 2>(s3, d3, s2);
+  } else if (K == 2) {
+    P::template ld_chunks<4>(sin, 4 * g0, s2, lds);
+  }
+  if (K >= 2) {
+    P::template ld_chunks<4>(in, ns2 + 4 * g0, d2, lds);
+    level<P, W, TF, 4>(s2, d2, prog);
+    merge<P, 4>(s2, d2, s1);
+  } else {
+    P::template ld_chunks<8>(sin, 8 * g0, s1, lds);
+  }
   P::template ld_chunks<8>(in, ns1 + 8 * g0, d1, lds);
   level<P, W, TF, 8>(s1, d1, prog);
   merge<P, 8>(s1, d1, x);

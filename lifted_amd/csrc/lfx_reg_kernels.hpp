@@ -27,9 +27,9 @@ __global__ void __launch_bounds__(256) k_reg(const BatchItem* items, long long n
   float* a1 = pick(it.arr[1]) + it.off[1];
   float* a2 = pick(it.arr[2]) + it.off[2];
   if constexpr (TF == kForward || TF == kInverseAdjoint)
-    reg::fwd_tile<reg::LaneGpu, W, TF>(prog, it.m0, it.a, it.b, it.H, (const float*)a0, a1, a2, lds);
+    reg::fwd_tile<reg::LaneGpu, W, TF>(prog, it.m0, it.k, it.a, it.b, it.H, (const float*)a0, a1, a2, lds);
   else
-    reg::rev_tile<reg::LaneGpu, W, TF>(prog, it.m0, it.a, it.b, it.H, (const float*)a0, (const float*)a2, a1, lds);
+    reg::rev_tile<reg::LaneGpu, W, TF>(prog, it.m0, it.k, it.a, it.b, it.H, (const float*)a0, (const float*)a2, a1, lds);
 }
 
 template <int TF>

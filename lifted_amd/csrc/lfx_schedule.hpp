@@ -99,7 +99,7 @@ LineSchedule schedule(const DProgram<C>& hp, bool fwd, long long n, long long L,
 // ---------------------------------------------------------------------------
 // Work items of one group of levels of one signal.
 //
-// A tiled group of exactly four levels of a float32 table wavelet runs its
+// A tiled group of one to four levels of a float32 table wavelet runs its
 // interior as register tiles (one wavefront per 1024-sample window,
 // lfx_reg.hpp) and the parts next to the band ends as LDS tiles; other groups
 // are LDS tiles of the schedule's tile length, or one whole-line item.
@@ -115,18 +115,19 @@ struct ItemSpec {
 constexpr int kRegLevels = 4;
 constexpr long long kRegWindow = 1024;  // samples per register tile window
 
-// Halo of an interior register tile: forward H (finest-level samples, multiple
-// of 16, owned 1024 - 2H); reverse Hk (level-4 units, owned 16 (64 - 2 Hk)).
-// -1 when the stencil is too wide for the register window to pay off.
+// Halo of an interior register tile of K levels: forward H (finest-level
+// samples, multiple of 16, owned 1024 - 2H); reverse Hk (level-4 units whatever
+// K is, owned 16 (64 - 2 Hk)). -1 when the stencil is too wide for the register
+// window to pay off.
 template <class C>
-long long reg_halo(const DProgram<C>& hp, bool fwd) {
+long long reg_halo(const DProgram<C>& hp, bool fwd, int K) {
   const long long m0 = 1ll << 40, a0 = 1ll << 36;
   if (fwd) {
     for (long long H = 0; 2 * H <= kRegWindow - 256; H += 16)
-      if (fwd_tile_ok(hp, m0, kRegLevels, a0, a0 + kRegWindow - 2 * H, H)) return H;
+      if (fwd_tile_ok(hp, m0, K, a0, a0 + kRegWindow - 2 * H, H)) return H;
   } else {
     for (long long Hk = 0; 16 * (64 - 2 * Hk) >= 256; ++Hk)
-      if (rev_tile_ok(hp, m0, kRegLevels, a0, a0 + 16 * (64 - 2 * Hk), Hk)) return Hk;
+      if (rev_tile_ok(hp, m0, K, a0, a0 + 16 * (64 - 2 * Hk), Hk << (kRegLevels - K))) return Hk;
   }
   return -1;
 }
@@ -148,7 +149,7 @@ std::vector<ItemSpec> pass_items(const DProgram<C>& hp, bool fwd, const PassPlan
   std::vector<ItemSpec> out;
   const long long m0 = pp.m0;
   long long r0 = 0, r1 = 0, Tr = 0, H = -1;
-  if (reg && pp.k == kRegLevels) H = reg_halo(hp, fwd);
+  if (reg && pp.k >= 1 && pp.k <= kRegLevels) H = reg_halo(hp, fwd, pp.k);
   if (H >= 0) {
     // Register windows stay strictly inside the band at every level (they
     // apply no boundary map), with a margin of 64 samples at the finest level.
@@ -160,16 +161,16 @@ std::vector<ItemSpec> pass_items(const DProgram<C>& hp, bool fwd, const PassPlan
     } else {
       Tr = 16 * (64 - 2 * H);
       r0 = 16 * (H + 4);
-      // every level's window [2^(4-l) g0, 2^(4-l) (g0 + 64)) inside its d band, and s_4's inside ns_4
+      // every level's window [2^(4-l) g0, 2^(4-l) (g0 + 64)) inside its d band, and s_k's inside ns_k
       long long gmax = -1;
       long long m = m0;
       bool ok = true;
-      for (int l = 1; l <= kRegLevels; ++l) {
+      for (int l = 1; l <= pp.k; ++l) {
         const long long nd = m / 2, ns = m - nd;
         const long long span = 1ll << (kRegLevels - l);
         const long long g = nd / span - 68;  // g0 + 64 < nd / span, with a margin
         gmax = (gmax < 0 || g < gmax) ? g : gmax;
-        if (l == kRegLevels) gmax = std::min(gmax, ns - 68);
+        if (l == pp.k) gmax = std::min(gmax, ns / span - 68);
         m = ns;
         if (g < 0) ok = false;
       }
@@ -185,7 +186,8 @@ std::vector<ItemSpec> pass_items(const DProgram<C>& hp, bool fwd, const PassPlan
     // first and last tiles with the exact rule and move them inward until they
     // hold (interior tiles are translates of a checked one).
     auto ok = [&](long long a) {
-      return fwd ? fwd_tile_ok(hp, m0, pp.k, a, a + Tr, H) : rev_tile_ok(hp, m0, pp.k, a, a + Tr, H);
+      return fwd ? fwd_tile_ok(hp, m0, pp.k, a, a + Tr, H)
+                 : rev_tile_ok(hp, m0, pp.k, a, a + Tr, H << (kRegLevels - pp.k));
     };
     const long long ntl = (r1 - r0) / Tr;
     long long first = 0, last = ntl - 1;

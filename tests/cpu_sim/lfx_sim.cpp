@@ -31,8 +31,8 @@ static bool reg_item(uint32_t wid, const DProgram<float>& prog, const ItemSpec& 
   switch (wid) {
 #define LFX_WAVELET(nm, idv, w)                                                                          \
   case idv:                                                                                              \
-    if constexpr (fwd) reg::fwd_tile<reg::LaneCpu, wt::WL_##nm, TF>(&prog, it.m0, it.a, it.b, it.H, in, out, sk, nullptr); \
-    else reg::rev_tile<reg::LaneCpu, wt::WL_##nm, TF>(&prog, it.m0, it.a, it.b, it.H, in, sk, out, nullptr); \
+    if constexpr (fwd) reg::fwd_tile<reg::LaneCpu, wt::WL_##nm, TF>(&prog, it.m0, it.k, it.a, it.b, it.H, in, out, sk, nullptr); \
+    else reg::rev_tile<reg::LaneCpu, wt::WL_##nm, TF>(&prog, it.m0, it.k, it.a, it.b, it.H, in, sk, out, nullptr); \
     return true;
 #define LFX_GENERIC(...)
 #define LFX_UNIT(...)

@@ -133,3 +133,19 @@ def test_register_tiles_all_wavelets(sim, oracle, w):
         ref = oracle.lwt(x, w, tf, bc, level=[L])
         got, info = run_sim(sim, w, tf, bc, x, L, limit=2000, tile=8192, k=4)
         np.testing.assert_array_equal(got, ref, err_msg=f"w={w} bc={bc} tf={tf} n={n}\n{info}")
+
+
+@pytest.mark.parametrize("w", [w for w in WAVELETS if w != 0])
+def test_register_tiles_fewer_levels(sim, oracle, w):
+    """Register passes of one to three levels (level counts that are not a
+    multiple of four; the last pass of a 5-level transform is one level)."""
+    g = np.random.default_rng(w + 11)
+    n = 20011
+    for L, tf in itertools.product((1, 2, 3, 5), (0, 1, 2, 3)):
+        if L > oracle.max_level(w, n):
+            continue
+        x = g.standard_normal(n).astype(np.float32)
+        ref = oracle.lwt(x, w, tf, 3, level=[L])
+        got, info = run_sim(sim, w, tf, 3, x, L, limit=600, tile=8192, k=4)
+        assert "reg [" in info, info
+        np.testing.assert_array_equal(got, ref, err_msg=f"w={w} L={L} tf={tf}\n{info}")
