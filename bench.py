@@ -3,21 +3,29 @@
 Workload (the headline configuration): 4096 1-D float32 signals, lengths
 round(2^U(10, 20)), each with a wavelet drawn uniformly from the 33 non-Lazy
 predefined wavelets and a level count drawn uniformly from [3, max_level];
-values iid N(0, 1); Zero boundary. One step = forward transform of every
-signal, then the forward-adjoint of the result (two batched plans, four stage
-launches). Inputs are resident in HBM before the timed region.
+values iid N(0, 1); Zero boundary; total capped at 2^30 samples. One step =
+forward transform of every signal, then the forward-adjoint of the result
+(two batched plans, three stages each). Inputs are resident in HBM before the
+timed region.
 
-Reports Gsamples/s (2 x samples per step / time), the HBM roofline of the
-dominant kernel (measured with HIP events on the launch stream), the plain
-per-level kernels (one launch per level) for comparison, and the CPU
-restatement of the reference timed on this host.
+Reports Gsamples/s (2 x samples per step / time); the HBM roofline of the
+dominant kernel, timed with HIP events recorded around each of its launches on
+the stream it runs on, inside the timed region; its HBM traffic from two
+rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) scaled by the same counters
+read on a device copy of known size; the plain per-level kernels (one launch
+per level) for comparison; and the CPU restatement of the reference timed on
+this host.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-plain] [--no-traffic]
 """
 import argparse
+import csv
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -30,6 +38,12 @@ HBM_PEAK = 8.0e12
 SEED = 20261201 + 3  # fourth workload of the benchmark set
 NSIG = 4096
 CAP = 1 << 30
+KIND_NAMES = {  # (plan transform, launch kind) -> kernel symbol prefix
+    (0, "register"): "void lfx::k_reg<0>(",
+    (2, "register"): "void lfx::k_reg<2>(",
+    (0, "lds"): "void lfx::k_batch<float, float, true>(",
+    (2, "lds"): "void lfx::k_batch<float, float, false>(",
+}
 
 
 def make_workload(seed=SEED, nsig=NSIG):
@@ -49,26 +63,40 @@ def make_workload(seed=SEED, nsig=NSIG):
     return sigs, off
 
 
-def time_plans(plans, x, bufs, steps, warmup):
-    """Run the plans in sequence (x -> bufs[0] -> bufs[1] ...), timing each
-    stage with HIP events on the launch stream. Returns (ms per step, per-stage
-    average ms list of (plan index, stage, ms))."""
+def time_plans(plans, x, bufs, steps, warmup, per_launch=False):
+    """Run the plans in sequence (x -> bufs[0] -> bufs[1] ...). Returns
+    (ms per step, per-stage average ms [(plan, stage, ms)], per-launch records).
+    Stage times come from events on the caller's stream; with per_launch, each
+    kernel launch is also bracketed by events on its own stream."""
     import torch
     stream = torch.cuda.current_stream()
+    infos = [[p.launch_info(s) for s in range(p.num_stages)] for p in plans]
 
-    def one_step(events=None):
+    def new_ev():
+        e = torch.cuda.Event(enable_timing=True)
+        e.record(stream)  # creates the underlying event
+        return e
+
+    def one_step(stage_ev=None, launch_ev=None):
         src = x
         for pi, p in enumerate(plans):
             for s in range(p.num_stages):
-                p.execute_stage(s, src, bufs[pi])
-                if events is not None:
-                    events[pi][s].record(stream)
+                if launch_ev is not None:
+                    p.execute_stage_timed(s, src, bufs[pi], launch_ev[pi][s])
+                else:
+                    p.execute_stage(s, src, bufs[pi])
+                if stage_ev is not None:
+                    stage_ev[pi][s].record(stream)
             src = bufs[pi]
 
     for _ in range(warmup):
         one_step()
     torch.cuda.synchronize()
-    ev = [[[torch.cuda.Event(enable_timing=True) for _ in range(p.num_stages)] for p in plans] for _ in range(steps)]
+    sev = [[[torch.cuda.Event(enable_timing=True) for _ in range(p.num_stages)] for p in plans] for _ in range(steps)]
+    lev = None
+    if per_launch:
+        lev = [[[[(new_ev(), new_ev()) for _ in infos[pi][s]] for s in range(p.num_stages)]
+                for pi, p in enumerate(plans)] for _ in range(steps)]
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
@@ -76,7 +104,7 @@ def time_plans(plans, x, bufs, steps, warmup):
     t0.record(stream)
     for k in range(steps):
         starts[k].record(stream)
-        one_step(ev[k])
+        one_step(sev[k], lev[k] if lev else None)
     t1.record(stream)
     torch.cuda.synchronize()
     total_ms = t0.elapsed_time(t1)
@@ -86,25 +114,138 @@ def time_plans(plans, x, bufs, steps, warmup):
             acc = 0.0
             for k in range(steps):
                 prev = starts[k] if (pi == 0 and s == 0) else (
-                    ev[k][pi][s - 1] if s > 0 else ev[k][pi - 1][plans[pi - 1].num_stages - 1])
-                acc += prev.elapsed_time(ev[k][pi][s])
+                    sev[k][pi][s - 1] if s > 0 else sev[k][pi - 1][plans[pi - 1].num_stages - 1])
+                acc += prev.elapsed_time(sev[k][pi][s])
             stage_ms.append((pi, s, acc / steps))
-    return total_ms / steps, stage_ms
+    launches = []
+    if per_launch:
+        for pi, p in enumerate(plans):
+            for s in range(p.num_stages):
+                for li, info in enumerate(infos[pi][s]):
+                    ms = sum(lev[k][pi][s][li][0].elapsed_time(lev[k][pi][s][li][1]) for k in range(steps)) / steps
+                    launches.append(dict(info, plan=pi, stage=s, ms=ms))
+    return total_ms / steps, stage_ms, launches
 
 
-def cpu_baseline(sigs, total, x_host):
+def dominant_kernel(plans, launches):
+    """Kernel (symbol) with the largest time per step: its launches per step,
+    algorithmic bytes per launch and average launch duration."""
+    by = {}
+    for r in launches:
+        key = (int(plans[r["plan"]].transform), r["kind"])
+        d = by.setdefault(key, {"launches": 0, "bytes": 0.0, "ms": 0.0, "items": 0})
+        d["launches"] += 1
+        d["bytes"] += r["alg_bytes"]
+        d["ms"] += r["ms"]
+        d["items"] += r["items"]
+    key = max(by, key=lambda k: by[k]["ms"])
+    d = by[key]
+    return {"symbol": KIND_NAMES.get(key, str(key)), "launches_per_step": d["launches"],
+            "alg_bytes_per_launch": d["bytes"] / d["launches"], "avg_launch_ms": d["ms"] / d["launches"],
+            "ms_per_step": d["ms"]}
+
+
+def _pmc_pass(counter, child_args, tmp):
+    """One rocprofv3 counter pass over a child run of this script; returns
+    {kernel symbol: (dispatches, summed counter value)}."""
+    out = os.path.join(tmp, counter + "_" + child_args[-1].lstrip("-"))
+    cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "run", "--",
+           sys.executable, "-u", os.path.abspath(__file__)] + child_args
+    env = dict(os.environ, TMPDIR="/tmp")
+    r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=150)
+    if r.returncode != 0:
+        raise RuntimeError(f"rocprofv3 {counter} exited {r.returncode}: {r.stderr[-400:]}")
+    path = None
+    for root, _, files in os.walk(out):
+        for f in files:
+            if f.endswith("counter_collection.csv"):
+                path = os.path.join(root, f)
+    if path is None:
+        raise RuntimeError(f"rocprofv3 {counter}: no counter file")
+    agg = {}
+    with open(path) as fh:
+        for row in csv.DictReader(fh):
+            k = row["Kernel_Name"]
+            n, v = agg.get(k, (0, 0.0))
+            agg[k] = (n + 1, v + float(row["Counter_Value"]))
+    return agg
+
+
+def measure_traffic(total, symbol):
+    """HBM bytes per launch of `symbol` for one step of the workload, from
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by the ratio of
+    known bytes to the counter on a device copy of the input size."""
+    if shutil.which("rocprofv3") is None:
+        return None, "rocprofv3 not found"
+    tmp = tempfile.mkdtemp(dir="/tmp", prefix="lfx_pmc_")
+    try:
+        res = {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            copy = _pmc_pass(counter, ["--copy-calib"], tmp)
+            ck = [k for k in copy if "copyBuffer" in k or "copy" in k.lower()]
+            n, v = copy[ck[0]]
+            factor = (total * 4 / 1024.0) / (v / n)
+            work = _pmc_pass(counter, ["--pmc-child"], tmp)
+            wk = [k for k in work if k.startswith(symbol)]
+            if not wk:
+                raise RuntimeError(f"{symbol} not in the {counter} pass")
+            n, v = work[wk[0]]
+            res[counter] = (v / n * 1024.0 * factor, factor)
+        traffic = res["FETCH_SIZE"][0] + res["WRITE_SIZE"][0]
+        return traffic, {"fetch_bytes": round(res["FETCH_SIZE"][0]), "write_bytes": round(res["WRITE_SIZE"][0]),
+                         "fetch_factor": round(res["FETCH_SIZE"][1], 4), "write_factor": round(res["WRITE_SIZE"][1], 4)}
+    except Exception as e:  # the counters are a report, not the measurement
+        return None, f"traffic not measured: {e}"
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def cpu_baseline(sigs, total, x_host, reps=3):
     import oracle as O
     threads = min(os.cpu_count() or 1, 16)
     y = np.empty_like(x_host)
     z = np.empty_like(x_host)
+    times = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        O.batch(x_host, sigs, O.FORWARD, O.ZERO, out=y, n_threads=threads)
+        O.batch(y, sigs, O.FORWARD_ADJOINT, O.ZERO, out=z, n_threads=threads)
+        times.append(time.perf_counter() - t)
+    dt = float(np.median(times))
+    # one core on the first eighth of the signals
+    sub = sigs[: len(sigs) // 8]
+    sub_n = sum(s[1] for s in sub)
     t = time.perf_counter()
-    O.batch(x_host, sigs, O.FORWARD, O.ZERO, out=y, n_threads=threads)
-    O.batch(y, sigs, O.FORWARD_ADJOINT, O.ZERO, out=z, n_threads=threads)
-    dt = time.perf_counter() - t
+    O.batch(x_host, sub, O.FORWARD, O.ZERO, out=y, n_threads=1)
+    O.batch(y, sub, O.FORWARD_ADJOINT, O.ZERO, out=z, n_threads=1)
+    dt1 = time.perf_counter() - t
     return {"value": round(2 * total / dt / 1e9, 4), "unit": "Gsamples/s", "cores": threads, "kind": "port",
             "sample": f"the full workload ({len(sigs)} signals, {total} samples), forward then forward-adjoint, "
-                      f"once, {dt:.2f} s, CPU restatement of the reference (its Highway build needs network "
-                      f"access); dynamic parallel-for over signals"}
+                      f"median of {reps} runs ({dt:.2f} s each), CPU restatement of the reference algorithm "
+                      f"(its Highway build is unavailable offline), dynamic parallel-for over signals",
+            "single_core": {"value": round(2 * sub_n / dt1 / 1e9, 4), "cores": 1,
+                            "sample": f"first {len(sub)} signals ({sub_n} samples), once, {dt1:.2f} s"}}
+
+
+def pmc_child():
+    """One step of the workload, untimed (run under rocprofv3 counter passes)."""
+    import torch
+    import lifted_amd as la
+    from lifted_amd.batch import BatchPlan
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    sigs, total = make_workload()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(SEED)
+    x = torch.randn(total, generator=gen, device=dev, dtype=torch.float32)
+    y = torch.empty_like(x)
+    z = torch.empty_like(x)
+    fwd = BatchPlan(sigs, la.Transform.Forward, dtype=torch.float32)
+    adj = BatchPlan(sigs, la.Transform.ForwardAdjoint, dtype=torch.float32)
+    fwd.execute(x, y)
+    adj.execute(y, z)
+    torch.cuda.synchronize()
+    print("pmc child done", flush=True)
 
 
 def main():
@@ -115,14 +256,19 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a device copy of the workload's input size (HBM counter calibration)")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); this build has no multi-GPU path: "
               "every signal is independent, so the workload would shard by signal with no exchange.",
               file=sys.stderr)
         sys.exit(2)
+    if args.pmc_child:
+        pmc_child()
+        return
     import torch
     import lifted_amd as la
     from lifted_amd.batch import BatchPlan
@@ -146,14 +292,26 @@ def main():
     fwd = BatchPlan(sigs, la.Transform.Forward, dtype=torch.float32)
     adj = BatchPlan(sigs, la.Transform.ForwardAdjoint, dtype=torch.float32)
     t_plan = time.perf_counter() - t_plan
+    plans = [fwd, adj]
     print(f"[bench] {len(sigs)} signals, {total} samples, plans in {t_plan:.1f}s, stages "
           f"{fwd.num_stages}+{adj.num_stages}", file=sys.stderr, flush=True)
-    ms, stage_ms = time_plans([fwd, adj], x, [y, z], args.steps, args.warmup)
-    plans = [fwd, adj]
-    dom = max(stage_ms, key=lambda t: t[2])
-    info = plans[dom[0]].stage_info(dom[1])
-    achieved = info["alg_bytes"] / (dom[2] * 1e-3)
+    ms, stage_ms, launches = time_plans(plans, x, [y, z], args.steps, args.warmup, per_launch=True)
+    dom = dominant_kernel(plans, launches)
+    achieved = dom["alg_bytes_per_launch"] / (dom["avg_launch_ms"] * 1e-3)
     value = 2 * total / (ms * 1e-3) / 1e9
+    print(f"[bench] fused: {ms:.3f} ms/step, {value:.2f} Gsamples/s; dominant {dom['symbol']} "
+          f"{dom['avg_launch_ms']:.3f} ms/launch, {achieved / 1e9:.0f} GB/s", file=sys.stderr, flush=True)
+    # copy of the input size: the achievable streaming rate on this device
+    a = torch.empty_like(x)
+    a.copy_(x)
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    c0.record()
+    for _ in range(5):
+        a.copy_(x)
+    c1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2 * total * 4 * 5 / (c0.elapsed_time(c1) * 1e-3) / 1e9
+    del a
     res = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gsamples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
@@ -162,18 +320,22 @@ def main():
                                "random non-Lazy wavelet, levels U[3,max], Zero BC",
                    "signals": len(sigs), "samples": total, "seed": SEED},
         "hbm_frac_whole_step": round(4 * total * 4 / (ms * 1e-3) / HBM_PEAK, 4),
+        "copy_GBps": round(copy_gbs, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
-                     "kernel": f"k_batch stage {dom[1]} of the {'forward' if dom[0] == 0 else 'adjoint'} plan",
-                     "kernel_ms": round(dom[2], 4), "alg_bytes": info["alg_bytes"]},
+                     "kernel": dom["symbol"].rstrip("("), "launches_per_step": dom["launches_per_step"],
+                     "avg_launch_ms": round(dom["avg_launch_ms"], 4),
+                     "alg_bytes_per_launch": round(dom["alg_bytes_per_launch"])},
         "stages": [{"plan": "fwd" if p == 0 else "adj", "stage": s, "ms": round(t, 4),
                     **plans[p].stage_info(s)} for p, s, t in stage_ms],
+        "launches": [{"plan": "fwd" if r["plan"] == 0 else "adj", "stage": r["stage"], "kind": r["kind"],
+                      "threads": r["threads"], "items": r["items"], "alg_bytes": round(r["alg_bytes"]),
+                      "ms": round(r["ms"], 4)} for r in launches],
     }
-    print(f"[bench] fused: {ms:.3f} ms/step, {value:.2f} Gsamples/s", file=sys.stderr, flush=True)
     if not args.no_plain:
         pf = BatchPlan(sigs, la.Transform.Forward, dtype=torch.float32, plain=True)
         pa = BatchPlan(sigs, la.Transform.ForwardAdjoint, dtype=torch.float32, plain=True)
-        pms, _ = time_plans([pf, pa], x, [y, z], args.plain_steps, 1)
+        pms, _, _ = time_plans([pf, pa], x, [y, z], args.plain_steps, 1)
         res["plain_ms_per_step"] = round(pms, 4)
         res["vs_plain"] = round(pms / ms, 3)
         res["plain_stages"] = pf.num_stages + pa.num_stages
@@ -181,9 +343,18 @@ def main():
               flush=True)
         pf.close()
         pa.close()
+    fwd.close()
+    adj.close()
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(sigs, total, x.cpu().numpy())
         print(f"[bench] cpu: {res['cpu_baseline']['value']} Gsamples/s", file=sys.stderr, flush=True)
+    if not args.no_traffic:
+        del x, y, z
+        torch.cuda.empty_cache()
+        traffic, note = measure_traffic(total, dom["symbol"])
+        res["roofline"]["traffic"] = None if traffic is None else round(traffic)
+        res["traffic_detail"] = note
+        print(f"[bench] traffic: {traffic} ({note})", file=sys.stderr, flush=True)
     print(json.dumps(res))
 
 

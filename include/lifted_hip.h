@@ -139,6 +139,25 @@ int lfx_batch_execute_stage(lfx_batch_plan plan, int32_t stage, const void* in_d
 int lfx_batch_stage_info(lfx_batch_plan plan, int32_t stage, int64_t* nitems, double* alg_bytes, int64_t* lds_bytes);
 void lfx_batch_plan_destroy(lfx_batch_plan plan);
 
+/* One kernel launch of a batched stage. */
+typedef struct {
+  int32_t kind;      /* 0: LDS items (tiles, whole lines), 1: register tiles */
+  int32_t threads;   /* workgroup size */
+  int64_t nitems;    /* work items (workgroups for kind 0, wavefronts for kind 1) */
+  double alg_bytes;  /* algorithmic bytes: owned samples read + written */
+  int64_t lds_bytes; /* dynamic LDS per workgroup (kind 0) */
+} lfx_launch_info;
+
+/* Launches of a stage (LDS launches first, then the register launch): fills
+ * info[0..min(n, max_launches)) and returns n, or an error code. */
+int lfx_batch_launch_info(lfx_batch_plan plan, int32_t stage, int32_t max_launches, lfx_launch_info* info);
+
+/* lfx_batch_execute_stage that also records caller-created hipEvent_t pairs
+ * around each launch, on the stream the launch goes to (launch order of
+ * lfx_batch_launch_info). Nothing is synchronized: read the events later. */
+int lfx_batch_execute_stage_timed(lfx_batch_plan plan, int32_t stage, const void* in_dev, void* out_dev, void* stream,
+                                  int32_t nevents, void* const* ev_start, void* const* ev_end);
+
 /* Tile length (samples at a pass's finest level, 0 = default) and levels per
  * tiled pass (0 = default) used by plans and transforms created afterwards on
  * this thread; set by the autotuner. */

@@ -26,8 +26,17 @@ def _bind():
         L.lfx_batch_plan_destroy.argtypes = [vp]
         L.lfx_batch_plan_destroy.restype = None
         L.lfx_set_tuning.argtypes = [ctypes.c_int64, ctypes.c_int32]
+        L.lfx_batch_launch_info.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(LaunchInfo)]
+        L.lfx_batch_execute_stage_timed.argtypes = [vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int32,
+                                                    ctypes.POINTER(vp), ctypes.POINTER(vp)]
         L._batch_bound = True
     return L
+
+
+class LaunchInfo(ctypes.Structure):
+    """lfx_launch_info: one kernel launch of a batched stage."""
+    _fields_ = [("kind", ctypes.c_int32), ("threads", ctypes.c_int32), ("nitems", ctypes.c_int64),
+                ("alg_bytes", ctypes.c_double), ("lds_bytes", ctypes.c_int64)]
 
 
 def set_tuning(tile=0, levels_per_pass=0):
@@ -55,6 +64,7 @@ class BatchPlan:
         _lib.check(L.lfx_batch_plan_create(arr, len(sigs), int(transform), int(mode), storage,
                                            LFX_FLAG_PLAIN if plain else 0, ctypes.byref(h)))
         self._h = h
+        self.transform = int(transform)
         self.signals = sigs
         self.total = max((o + n for o, n, _, _ in sigs), default=0)
         self.num_stages = L.lfx_batch_num_stages(h)
@@ -92,6 +102,30 @@ class BatchPlan:
         _lib.check(_bind().lfx_batch_stage_info(self._h, int(stage), ctypes.byref(n), ctypes.byref(b),
                                                 ctypes.byref(l)))
         return {"items": n.value, "alg_bytes": b.value, "lds_bytes": l.value}
+
+    def launch_info(self, stage):
+        """Kernel launches of a stage (LDS launches first, then the register
+        launch): dicts with kind ("lds" / "register"), threads, items,
+        alg_bytes, lds_bytes."""
+        arr = (LaunchInfo * 16)()
+        n = _lib.check(_bind().lfx_batch_launch_info(self._h, int(stage), 16, arr))
+        return [{"kind": "register" if arr[i].kind == 1 else "lds", "threads": arr[i].threads,
+                 "items": arr[i].nitems, "alg_bytes": arr[i].alg_bytes, "lds_bytes": arr[i].lds_bytes}
+                for i in range(n)]
+
+    def execute_stage_timed(self, stage, x, out, events):
+        """execute_stage, recording events[i] = (start, end) (torch.cuda.Event
+        with timing, already recorded once so that they exist) around launch i
+        on the stream it goes to. Nothing is synchronized."""
+        import torch
+        self._check_args(x, out)
+        st = torch.cuda.current_stream(x.device).cuda_stream
+        n = len(events)
+        starts = (ctypes.c_void_p * max(n, 1))(*[e[0].cuda_event for e in events])
+        ends = (ctypes.c_void_p * max(n, 1))(*[e[1].cuda_event for e in events])
+        _lib.check(_bind().lfx_batch_execute_stage_timed(self._h, int(stage), x.data_ptr(), out.data_ptr(), st, n,
+                                                         starts, ends))
+        return out
 
     def close(self):
         if getattr(self, "_h", None):

@@ -548,19 +548,16 @@ static int lwt_overload(uint32_t wavelet, uint32_t transform, uint32_t bc, size_
 struct BatchLaunch {
   long long item0 = 0, nitems = 0;
   size_t lds = 0;
-  int threads = 0;  // 0: default workgroup size
-};
-struct RegLaunch {
-  uint32_t wavelet = 0;
-  const void* prog = nullptr;
-  long long item0 = 0, nitems = 0;
+  int threads = 0;   // 0: default workgroup size
+  double bytes = 0;  // algorithmic bytes of its items
 };
 struct BatchStage {
   long long nitems = 0;
   size_t lds = 0;
   double alg_bytes = 0;  // bytes the stage must read and write (owned ranges)
   std::vector<BatchLaunch> launches;
-  std::vector<RegLaunch> regs;  // register-tile items, one launch per wavelet
+  long long reg0 = 0, nreg = 0;  // register-tile items (one launch, all wavelets)
+  double reg_bytes = 0;
 };
 
 struct BatchPlanImpl {
@@ -572,6 +569,7 @@ struct BatchPlanImpl {
   BatchItem* d_items = nullptr;
   void* d_scr = nullptr;
   size_t scr_bytes = 0;
+  void* d_progs = nullptr;  // DProgram<float> table of the register items
   // side streams: the launches of one stage are independent (disjoint owned
   // outputs), so they are spread over the caller's stream and these, joined
   // by events before the next stage
@@ -634,7 +632,11 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
   std::vector<size_t> lds;
   std::vector<std::vector<size_t>> item_lds;
   std::vector<double> bytes;
-  std::vector<std::vector<std::pair<uint32_t, BatchItem>>> reg_stage;
+  std::vector<std::vector<BatchItem>> reg_stage;
+  std::vector<std::vector<double>> item_bytes;
+  std::vector<double> reg_bytes;
+  std::vector<DProgram<float>> reg_progs;  // program table of register items
+  std::map<uint32_t, int> reg_pidx;
   long long scr_elems = 0;
   for (int i = 0; i < nsig; ++i) {
     const lfx_signal& sg = sigs[i];
@@ -670,15 +672,26 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
     auto add_item = [&](int stage, const BatchItem& it, double c, size_t l, double b, uint32_t wid) {
       if (it.kind == kItemReg) {
         if ((int)reg_stage.size() <= stage) reg_stage.resize(stage + 1);
-        reg_stage[stage].push_back(std::make_pair(wid, it));
+        auto f = reg_pidx.find(wid);
+        if (f == reg_pidx.end()) {
+          f = reg_pidx.emplace(wid, (int)reg_progs.size()).first;
+          reg_progs.push_back(resolve<float>(find_wavelet(wid), tf));
+        }
+        BatchItem ri = it;
+        ri.wavelet = wid;
+        ri.pidx = f->second;
+        reg_stage[stage].push_back(ri);
         if ((int)bytes.size() <= stage) {
           per_stage.resize(stage + 1);
           cost.resize(stage + 1);
           lds.resize(stage + 1, 0);
           item_lds.resize(stage + 1);
+          item_bytes.resize(stage + 1);
           bytes.resize(stage + 1, 0.0);
         }
+        if ((int)reg_bytes.size() <= stage) reg_bytes.resize(stage + 1, 0.0);
         bytes[stage] += b;
+        reg_bytes[stage] += b;
         return;
       }
       if ((int)per_stage.size() <= stage) {
@@ -686,9 +699,11 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
         cost.resize(stage + 1);
         lds.resize(stage + 1, 0);
         item_lds.resize(stage + 1);
+        item_bytes.resize(stage + 1);
         bytes.resize(stage + 1, 0.0);
       }
       per_stage[stage].push_back(it);
+      item_bytes[stage].push_back(b);
       item_lds[stage].push_back(l);
       cost[stage].push_back(c);
       lds[stage] = std::max(lds[stage], l);
@@ -815,24 +830,21 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
       for (size_t j : idx) {
         all.push_back(per_stage[st][j]);
         bl.lds = std::max(bl.lds, item_lds[st][j]);
+        bl.bytes += item_bytes[st][j];
       }
       bs.nitems += bl.nitems;
       bs.lds = std::max(bs.lds, bl.lds);
       bs.launches.push_back(bl);
     }
-    if (st < reg_stage.size()) {
+    if (st < reg_stage.size() && !reg_stage[st].empty()) {
+      // grouped by wavelet, so that neighbouring waves run the same code
       auto& rs = reg_stage[st];
-      std::stable_sort(rs.begin(), rs.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
-      for (size_t j = 0; j < rs.size();) {
-        RegLaunch rl;
-        rl.wavelet = rs[j].first;
-        rl.prog = rs[j].second.prog;
-        rl.item0 = (long long)all.size();
-        while (j < rs.size() && rs[j].first == rl.wavelet) all.push_back(rs[j++].second);
-        rl.nitems = (long long)all.size() - rl.item0;
-        bs.nitems += rl.nitems;
-        bs.regs.push_back(rl);
-      }
+      std::stable_sort(rs.begin(), rs.end(), [](const BatchItem& x, const BatchItem& y) { return x.pidx < y.pidx; });
+      bs.reg0 = (long long)all.size();
+      bs.nreg = (long long)rs.size();
+      bs.reg_bytes = reg_bytes[st];
+      all.insert(all.end(), rs.begin(), rs.end());
+      bs.nitems += bs.nreg;
     }
     plan->stages.push_back(bs);
   }
@@ -840,6 +852,11 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
     hip_check(hipMalloc(&plan->d_items, all.size() * sizeof(BatchItem)), "hipMalloc(items)");
     hip_check(hipMemcpy(plan->d_items, all.data(), all.size() * sizeof(BatchItem), hipMemcpyHostToDevice),
               "hipMemcpy(items)");
+  }
+  if (!reg_progs.empty()) {
+    const size_t pb = reg_progs.size() * sizeof(DProgram<float>);
+    hip_check(hipMalloc(&plan->d_progs, pb), "hipMalloc(programs)");
+    hip_check(hipMemcpy(plan->d_progs, reg_progs.data(), pb, hipMemcpyHostToDevice), "hipMemcpy(programs)");
   }
   plan->scr_bytes = (size_t)std::max<long long>(scr_elems, 1) * cs;
   if (hipMalloc(&plan->d_scr, plan->scr_bytes) != hipSuccess) {
@@ -861,6 +878,7 @@ static void free_plan(BatchPlanImpl* p) {
   for (hipStream_t q : p->side) (void)hipStreamSynchronize(q);
   if (p->d_items) (void)hipFree(p->d_items);
   if (p->d_scr) (void)hipFree(p->d_scr);
+  if (p->d_progs) (void)hipFree(p->d_progs);
   for (hipStream_t q : p->side) (void)hipStreamDestroy(q);
   for (hipEvent_t e : p->join) (void)hipEventDestroy(e);
   if (p->fork) (void)hipEventDestroy(p->fork);
@@ -883,40 +901,53 @@ static void plan_streams(BatchPlanImpl* p) {
   }
 }
 
-static void batch_stage(BatchPlanImpl* p, int st, const void* in, void* out, hipStream_t s) {
+// ev (optional): one start/end event pair per launch, recorded on the
+// launch's stream (LDS launches in order, then the register launch)
+static void batch_stage(BatchPlanImpl* p, int st, const void* in, void* out, hipStream_t s,
+                        const std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev = nullptr) {
   if (st < 0 || st >= (int)p->stages.size()) fail(LFX_EINVAL, "stage out of range");
   int dev = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
   if (dev != p->dev) fail(LFX_EINVAL, "plan was created on another device");
   const BatchStage& bs = p->stages[st];
-  const int nl = (int)(bs.launches.size() + bs.regs.size());
-  const int nside = std::min<int>((int)p->side.size(), nl - 1);
-  std::vector<hipStream_t> lanes{s};
-  if (nside > 0) {
+  // the LDS launches (latency-bound: few, long-lived workgroups) run on the
+  // side streams next to the register launch on the caller's stream
+  const int nside = std::min<int>((int)p->side.size(), (int)bs.launches.size());
+  std::vector<hipStream_t> lanes;
+  if (nside > 0 && bs.launches.size() + (bs.nreg > 0 ? 1 : 0) >= 2) {
     hip_check(hipEventRecord(p->fork, s), "hipEventRecord");
     for (int i = 0; i < nside; ++i) {
       hip_check(hipStreamWaitEvent(p->side[i], p->fork, 0), "hipStreamWaitEvent");
       lanes.push_back(p->side[i]);
     }
+  } else {
+    lanes.push_back(s);
   }
-  // latency-bound LDS launches first, on the side streams; register launches
-  // round-robin over all lanes
-  int li = nside > 0 ? 1 : 0;
-  auto next = [&] {
-    hipStream_t q = lanes[li];
-    li = (li + 1) % (int)lanes.size();
-    return q;
+  size_t ei = 0;
+  auto mark = [&](bool start, hipStream_t q) {
+    if (!ev) return;
+    hip_check(hipEventRecord(start ? (*ev)[ei].first : (*ev)[ei].second, q), "hipEventRecord");
+    if (!start) ++ei;
   };
-  for (const BatchLaunch& bl : bs.launches)
+  for (size_t j = 0; j < bs.launches.size(); ++j) {
+    const BatchLaunch& bl = bs.launches[j];
+    hipStream_t q = lanes[j % lanes.size()];
+    mark(true, q);
     hip_check(launch_batch(p->storage, p->fwd, p->bc, p->d_items + bl.item0, bl.nitems, bl.lds, bl.threads, in, out,
-                           p->d_scr, next()),
+                           p->d_scr, q),
               "batch kernel");
-  for (const RegLaunch& rl : bs.regs)
-    hip_check(launch_reg(p->tf, rl.wavelet, rl.prog, p->d_items + rl.item0, rl.nitems, in, out, p->d_scr, next()),
-              "register kernel");
-  for (int i = 0; i < nside; ++i) {
-    hip_check(hipEventRecord(p->join[i], p->side[i]), "hipEventRecord");
-    hip_check(hipStreamWaitEvent(s, p->join[i], 0), "hipStreamWaitEvent");
+    mark(false, q);
+  }
+  if (bs.nreg > 0) {
+    mark(true, s);
+    hip_check(launch_reg(p->tf, p->d_progs, p->d_items + bs.reg0, bs.nreg, in, out, p->d_scr, s), "register kernel");
+    mark(false, s);
+  }
+  if (lanes[0] != s) {
+    for (int i = 0; i < nside; ++i) {
+      hip_check(hipEventRecord(p->join[i], p->side[i]), "hipEventRecord");
+      hip_check(hipStreamWaitEvent(s, p->join[i], 0), "hipStreamWaitEvent");
+    }
   }
 }
 
@@ -1066,6 +1097,46 @@ int lfx_batch_stage_info(lfx_batch_plan plan, int32_t stage, int64_t* nitems, do
     if (nitems) *nitems = p->stages[stage].nitems;
     if (alg_bytes) *alg_bytes = p->stages[stage].alg_bytes;
     if (lds_bytes) *lds_bytes = (int64_t)p->stages[stage].lds;
+  });
+}
+
+int lfx_batch_launch_info(lfx_batch_plan plan, int32_t stage, int32_t max_launches, lfx_launch_info* info) {
+  int count = 0;
+  const int rc = guarded([&] {
+    BatchPlanImpl* p = reinterpret_cast<BatchPlanImpl*>(plan);
+    if (!p) fail(LFX_EINVAL, "null plan");
+    if (stage < 0 || stage >= (int)p->stages.size()) fail(LFX_EINVAL, "stage out of range");
+    const BatchStage& bs = p->stages[stage];
+    const int n = (int)bs.launches.size() + (bs.nreg > 0 ? 1 : 0);
+    for (int i = 0; i < n && i < max_launches && info; ++i) {
+      lfx_launch_info& r = info[i];
+      const bool reg = i == (int)bs.launches.size();
+      r.kind = reg ? 1 : 0;
+      r.threads = reg ? 256 : (bs.launches[i].threads > 0 ? bs.launches[i].threads : block_threads());
+      r.nitems = reg ? bs.nreg : bs.launches[i].nitems;
+      r.alg_bytes = reg ? bs.reg_bytes : bs.launches[i].bytes;
+      r.lds_bytes = reg ? 0 : (int64_t)bs.launches[i].lds;
+    }
+    count = n;
+  });
+  return rc == LFX_OK ? count : rc;
+}
+
+int lfx_batch_execute_stage_timed(lfx_batch_plan plan, int32_t stage, const void* in_dev, void* out_dev, void* stream,
+                                  int32_t nevents, void* const* ev_start, void* const* ev_end) {
+  return guarded([&] {
+    BatchPlanImpl* p = reinterpret_cast<BatchPlanImpl*>(plan);
+    if (!p) fail(LFX_EINVAL, "null plan");
+    if (stage < 0 || stage >= (int)p->stages.size()) fail(LFX_EINVAL, "stage out of range");
+    const BatchStage& bs = p->stages[stage];
+    const int n = (int)bs.launches.size() + (bs.nreg > 0 ? 1 : 0);
+    if (nevents < n || !ev_start || !ev_end) fail(LFX_EINVAL, "one start and one end event per launch are needed");
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev(n);
+    for (int i = 0; i < n; ++i) {
+      if (!ev_start[i] || !ev_end[i]) fail(LFX_EINVAL, "null event");
+      ev[i] = {(hipEvent_t)ev_start[i], (hipEvent_t)ev_end[i]};
+    }
+    batch_stage(p, stage, in_dev, out_dev, (hipStream_t)stream, &ev);
   });
 }
 

@@ -108,6 +108,8 @@ struct BatchItem {
   int32_t kind;      // BatchKind
   int32_t k;         // levels of this item
   int32_t arr[3];    // BatchArr of the three logical arrays
+  int32_t pidx;      // register items: index into the plan's program table
+  uint32_t wavelet;  // register items: wavelet id (selects the compiled step shapes)
   int32_t pad;
 };
 

@@ -13,22 +13,23 @@ hipError_t launch_tile(const TileLaunch& tl, hipStream_t st);
 // threads: workgroup size (0: the default of block_threads())
 hipError_t launch_batch(int storage, bool fwd, int bc, const BatchItem* items, long long nitems, size_t lds_bytes,
                         int threads, const void* in, void* out, void* scr, hipStream_t st);
-// Register-tile items (float32) of one wavelet and transform type.
-hipError_t launch_reg_0(uint32_t wavelet, const void* prog, const BatchItem* items, long long nitems,
-                        const void* in, void* out, void* scr, hipStream_t st);
-hipError_t launch_reg_1(uint32_t wavelet, const void* prog, const BatchItem* items, long long nitems,
-                        const void* in, void* out, void* scr, hipStream_t st);
-hipError_t launch_reg_2(uint32_t wavelet, const void* prog, const BatchItem* items, long long nitems,
-                        const void* in, void* out, void* scr, hipStream_t st);
-hipError_t launch_reg_3(uint32_t wavelet, const void* prog, const BatchItem* items, long long nitems,
-                        const void* in, void* out, void* scr, hipStream_t st);
-inline hipError_t launch_reg(int tf, uint32_t wavelet, const void* prog, const BatchItem* items, long long nitems,
-                             const void* in, void* out, void* scr, hipStream_t st) {
+// Register-tile items (float32) of one transform type, any table wavelet;
+// progs: the plan's device table of DProgram<float>, indexed by BatchItem::pidx.
+hipError_t launch_reg_0(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
+                        void* scr, hipStream_t st);
+hipError_t launch_reg_1(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
+                        void* scr, hipStream_t st);
+hipError_t launch_reg_2(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
+                        void* scr, hipStream_t st);
+hipError_t launch_reg_3(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
+                        void* scr, hipStream_t st);
+inline hipError_t launch_reg(int tf, const void* progs, const BatchItem* items, long long nitems, const void* in,
+                             void* out, void* scr, hipStream_t st) {
   switch (tf) {
-    case 0: return launch_reg_0(wavelet, prog, items, nitems, in, out, scr, st);
-    case 1: return launch_reg_1(wavelet, prog, items, nitems, in, out, scr, st);
-    case 2: return launch_reg_2(wavelet, prog, items, nitems, in, out, scr, st);
-    case 3: return launch_reg_3(wavelet, prog, items, nitems, in, out, scr, st);
+    case 0: return launch_reg_0(progs, items, nitems, in, out, scr, st);
+    case 1: return launch_reg_1(progs, items, nitems, in, out, scr, st);
+    case 2: return launch_reg_2(progs, items, nitems, in, out, scr, st);
+    case 3: return launch_reg_3(progs, items, nitems, in, out, scr, st);
   }
   return hipErrorInvalidValue;
 }

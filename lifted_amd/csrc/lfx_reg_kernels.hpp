@@ -12,7 +12,19 @@ namespace lfx {
 constexpr int kStageFloats = 64 * 17;
 
 template <class W, int TF>
-__global__ void __launch_bounds__(256) k_reg(const BatchItem* items, long long nitems, const DProgram<float>* prog,
+__device__ __forceinline__ void reg_item(const BatchItem& it, const DProgram<float>* prog, float* a0, float* a1,
+                                         float* a2, float* lds) {
+  if constexpr (TF == kForward || TF == kInverseAdjoint)
+    reg::fwd_tile<reg::LaneGpu, W, TF>(prog, it.m0, it.k, it.a, it.b, it.H, (const float*)a0, a1, a2, lds);
+  else
+    reg::rev_tile<reg::LaneGpu, W, TF>(prog, it.m0, it.k, it.a, it.b, it.H, (const float*)a0, (const float*)a2, a1, lds);
+}
+
+// All register-tile items of a stage, any table wavelet: the item's wavelet id
+// selects the compiled step shapes (a wave-uniform branch), its program index
+// the coefficients in the plan's program table.
+template <int TF>
+__global__ void __launch_bounds__(256) k_reg(const BatchItem* items, long long nitems, const DProgram<float>* progs,
                                              const float* uin, float* uout, float* scr) {
   // per-wave transposition buffer: 16 x 64 samples, rows padded to 17
   __shared__ float stage[4][kStageFloats];
@@ -20,29 +32,18 @@ __global__ void __launch_bounds__(256) k_reg(const BatchItem* items, long long n
   if (w >= nitems) return;
   float* lds = stage[threadIdx.x >> 6];
   const BatchItem it = items[w];
+  const DProgram<float>* prog = progs + it.pidx;
   // select among the kernel's pointer arguments (keeps them in the global
   // address space for the compiler, unlike indexing an array of pointers)
   auto pick = [&](int a) { return a == kArrIn ? const_cast<float*>(uin) : (a == kArrOut ? uout : scr); };
   float* a0 = pick(it.arr[0]) + it.off[0];
   float* a1 = pick(it.arr[1]) + it.off[1];
   float* a2 = pick(it.arr[2]) + it.off[2];
-  if constexpr (TF == kForward || TF == kInverseAdjoint)
-    reg::fwd_tile<reg::LaneGpu, W, TF>(prog, it.m0, it.k, it.a, it.b, it.H, (const float*)a0, a1, a2, lds);
-  else
-    reg::rev_tile<reg::LaneGpu, W, TF>(prog, it.m0, it.k, it.a, it.b, it.H, (const float*)a0, (const float*)a2, a1, lds);
-}
-
-template <int TF>
-hipError_t launch_reg_tf(uint32_t wavelet, const void* prog, const BatchItem* items, long long nitems, const void* in,
-                         void* out, void* scr, hipStream_t st) {
-  if (nitems <= 0) return hipSuccess;
-  const unsigned blocks = (unsigned)((nitems + 3) / 4);
-  switch (wavelet) {
-#define LFX_WAVELET(nm, idv, w)                                                                            \
-  case idv:                                                                                                \
-    hipLaunchKernelGGL((k_reg<wt::WL_##nm, TF>), dim3(blocks), dim3(256), 0, st, items, nitems,            \
-                       (const DProgram<float>*)prog, (const float*)in, (float*)out, (float*)scr);          \
-    return hipGetLastError();
+  switch (it.wavelet) {
+#define LFX_WAVELET(nm, idv, w)                                     \
+  case idv:                                                         \
+    reg_item<wt::WL_##nm, TF>(it, prog, a0, a1, a2, lds);           \
+    return;
 #define LFX_GENERIC(...)
 #define LFX_UNIT(...)
 #define LFX_REPEAT(...)
@@ -55,8 +56,19 @@ hipError_t launch_reg_tf(uint32_t wavelet, const void* prog, const BatchItem* it
 #undef LFX_REPEAT
 #undef LFX_SCALE
 #undef LFX_END
+    default:
+      return;
   }
-  return hipErrorNotSupported;
+}
+
+template <int TF>
+hipError_t launch_reg_tf(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
+                         void* scr, hipStream_t st) {
+  if (nitems <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)((nitems + 3) / 4);
+  hipLaunchKernelGGL((k_reg<TF>), dim3(blocks), dim3(256), 0, st, items, nitems, (const DProgram<float>*)progs,
+                     (const float*)in, (float*)out, (float*)scr);
+  return hipGetLastError();
 }
 
 }  // namespace lfx

@@ -1,9 +1,9 @@
-// Register-tile kernels for transform type 2 (see lfx_reg_kernels.hpp).
+// Register-tile kernel for transform type 2 (see lfx_reg_kernels.hpp).
 #include "lfx_reg_kernels.hpp"
 
 namespace lfx {
-hipError_t launch_reg_2(uint32_t wavelet, const void* prog, const BatchItem* items, long long nitems,
-                        const void* in, void* out, void* scr, hipStream_t st) {
-  return launch_reg_tf<2>(wavelet, prog, items, nitems, in, out, scr, st);
+hipError_t launch_reg_2(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
+                        void* scr, hipStream_t st) {
+  return launch_reg_tf<2>(progs, items, nitems, in, out, scr, st);
 }
 }  // namespace lfx
