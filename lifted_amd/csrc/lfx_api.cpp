@@ -661,7 +661,8 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
     const DProgram<double> hpd = resolve<double>(w, tf);
     auto items_of = [&](const Group& gr) {
       const PassPlan pp{gr.m0, gr.k, gr.T, gr.H, gr.ntiles};
-      return cs == 8 ? pass_items(hpd, plan->fwd, pp, false, storage) : pass_items(hpf, plan->fwd, pp, reg, storage);
+      return cs == 8 ? pass_items(hpd, plan->fwd, pp, false, storage, bc)
+                     : pass_items(hpf, plan->fwd, pp, reg, storage, bc);
     };
     const int G = (int)gs.size();
     // where each group's output band goes: user output or a scratch region

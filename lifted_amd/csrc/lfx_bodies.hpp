@@ -87,8 +87,10 @@ template <class C, class SI, class SS, class SO>
 __device__ __forceinline__ void line_rev_body(const DProgram<C>* prog, int bc, long long n, int L, const SI* in, long long i_ax,
                               const SS* sin, long long s_ax, SO* out, long long o_ax, C* P, C* Q) {
   const int tid = LFX_TID, nthr = LFX_NTHR;
-  C* buf[2] = {P, Q};
-  C* cur = buf[(L - 1) & 1];
+  // selects, not an array of pointers: the compiler then keeps the LDS
+  // address space (an indexed pointer array turns every access into a flat one)
+  auto buf = [&](int r) { return (r & 1) ? Q : P; };
+  C* cur = buf(L - 1);
   {
     const int mL = (int)band_len(n, L - 1);
     const int nsL = mL - mL / 2;
@@ -102,7 +104,7 @@ __device__ __forceinline__ void line_rev_body(const DProgram<C>* prog, int bc, l
     const Band<C> s{cur, 0, ns}, d{cur + ns, 0, nd};
     dev::level_steps<C>(prog, bc, s, d, ns, nd, tid, nthr);
     if (l > 0) {
-      C* nxt = buf[(l - 1) & 1];
+      C* nxt = buf(l - 1);
       for (int j = tid; j < m; j += nthr) nxt[j] = (j & 1) ? cur[ns + (j >> 1)] : cur[j >> 1];
       LFX_SYNC();
       cur = nxt;
@@ -127,7 +129,7 @@ __device__ __forceinline__ void tile_level_steps(const DProgram<C>* __restrict__
     if (st.kind == kScale) {
       dev::scale_step<C>(cf, s, d, tid, nthr);
     } else {
-      step_shrink(st, ns, nd, r);
+      step_shrink(st, ns, nd, r, bc);
       const long long ylo = st.y_is_d ? r.dlo : r.slo, yhi = st.y_is_d ? r.dhi : r.shi;
       dev::lift_step<C>(st, cf, bc, s, d, ns, nd, ylo, yhi, tid, nthr);
     }
@@ -192,14 +194,14 @@ __device__ __forceinline__ void tile_rev_body(const DProgram<C>* prog, int bc, l
                                               long long b, long long Hk, const SI* in, long long i_ax, const SK* sin,
                                               long long si_ax, SO* out, long long o_ax, C* P, C* Q) {
   const int tid = LFX_TID, nthr = LFX_NTHR;
-  // level r lives in buf[r & 1]: level 1 (largest window) in P
-  C* buf[2] = {Q, P};
+  // level r lives in buf(r): level 1 (largest window) in P
+  auto buf = [&](int r) { return (r & 1) ? P : Q; };
   long long m = band_len(m0, k - 1);
   long long ns = m - m / 2, nd = m / 2;
   long long gg = lmax(0, (a >> k) - Hk);
   const long long ghi = ((b + (1ll << k) - 1) >> k) + Hk;
   int sw = (int)lmax(0, lmin(ghi, ns) - gg), dw = (int)lmax(0, lmin(ghi, nd) - gg);
-  C* cur = buf[k & 1];
+  C* cur = buf(k);
   gload<SK, C>(sin + gg * si_ax, si_ax, 0, sw, tid, nthr, [&](int i, C v) { cur[i] = v; });
   {
     C* dd = cur + sw;
@@ -223,7 +225,7 @@ __device__ __forceinline__ void tile_rev_body(const DProgram<C>* prog, int bc, l
       });
       break;
     }
-    C* nxt = buf[(lv - 1) & 1];
+    C* nxt = buf(lv - 1);
     for (int q = tid; q < (int)wlen; q += nthr) nxt[q] = (q & 1) ? cur[sw + (q >> 1)] : cur[q >> 1];
     m = band_len(m0, lv - 2);
     ns = m - m / 2;

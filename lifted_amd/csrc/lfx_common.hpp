@@ -20,8 +20,11 @@ inline size_t compute_bytes(int s) { return s == kF64 ? 8 : 4; }
 constexpr int kThreads = 1024;  // launch bound; the block size is chosen at launch
 int block_threads();             // threads per workgroup used by the launchers
 constexpr int kMaxDims = 8;
-// LDS one workgroup may use (gfx950: 160 KiB per CU)
-constexpr size_t kMaxLdsBytes = 160 * 1024;
+// LDS of one workgroup (gfx950: 160 KiB per CU): the LDS kernels keep a copy
+// of the step program (kProgLdsBytes, static) next to their dynamic LDS of at
+// most kMaxLdsBytes
+constexpr size_t kProgLdsBytes = 3072;
+constexpr size_t kMaxLdsBytes = 160 * 1024 - kProgLdsBytes;
 
 // Base offsets of a line in three arrays: the line index is decomposed over
 // the non-transformed dims (last fastest); str[a][k] is array a's stride of

@@ -38,6 +38,21 @@ __device__ __forceinline__ C* lds_base() {
   return reinterpret_cast<C*>(smem_raw);
 }
 
+static_assert(sizeof(DProgram<double>) <= kProgLdsBytes, "step program does not fit its LDS copy");
+
+// Copy the step program into LDS once per workgroup: every lifting step reads
+// its descriptor and coefficients, and from global memory each of those reads
+// would be a dependent round trip (long while other kernels stream HBM).
+template <class C>
+__device__ __forceinline__ const DProgram<C>* stage_program(const DProgram<C>* g) {
+  __shared__ __align__(16) uint32_t prog_lds[kProgLdsBytes / 4];
+  constexpr int W = (int)(sizeof(DProgram<C>) / 4);
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(g);
+  for (int i = (int)threadIdx.x; i < W; i += (int)blockDim.x) prog_lds[i] = src[i];
+  __syncthreads();
+  return reinterpret_cast<const DProgram<C>*>(prog_lds);
+}
+
 // array 0: input, array 1: output, array 2: separate s_L source (reverse only)
 template <class C, class SI, class SO>
 __global__ void __launch_bounds__(kThreads) k_line_fwd(const DProgram<C>* prog, int bc, long long n, int L,
@@ -46,6 +61,7 @@ __global__ void __launch_bounds__(kThreads) k_line_fwd(const DProgram<C>* prog, 
   long long off[3];
   line_bases(lm, line0 + blockIdx.x, off);
   C* P = lds_base<C>();
+  prog = stage_program<C>(prog);
   line_fwd_body<C, SI, SO>(prog, bc, n, L, src + off[0], lm.ax[0], dst + off[1], lm.ax[1], P, P + cap0);
 }
 
@@ -56,6 +72,7 @@ __global__ void __launch_bounds__(kThreads) k_line_rev(const DProgram<C>* prog, 
   long long off[3];
   line_bases(lm, line0 + blockIdx.x, off);
   C* P = lds_base<C>();
+  prog = stage_program<C>(prog);
   line_rev_body<C, SI, SS, SO>(prog, bc, n, L, src + off[0], lm.ax[0], ssrc + off[2], lm.ax[2], dst + off[1],
                                lm.ax[1], P, P + cap0);
 }
@@ -71,6 +88,7 @@ __global__ void __launch_bounds__(kThreads) k_tile_fwd(const DProgram<C>* prog, 
   line_bases(lm, line, off);
   C* P = lds_base<C>();
   const long long a = tile * ta.T, b = lmin(a + ta.T, ta.m0);
+  prog = stage_program<C>(prog);
   tile_fwd_body<C, SI, SO, SK>(prog, bc, ta.m0, ta.k, a, b, ta.H, src + off[0], lm.ax[0], dst + off[1], lm.ax[1],
                                sdst + off[2], lm.ax[2], P, P + cap0);
 }
@@ -86,6 +104,7 @@ __global__ void __launch_bounds__(kThreads) k_tile_rev(const DProgram<C>* prog, 
   line_bases(lm, line, off);
   C* P = lds_base<C>();
   const long long a = tile * ta.T, b = lmin(a + ta.T, ta.m0);
+  prog = stage_program<C>(prog);
   tile_rev_body<C, SI, SK, SO>(prog, bc, ta.m0, ta.k, a, b, ta.H, src + off[0], lm.ax[0], ssrc + off[2], lm.ax[2],
                                dst + off[1], lm.ax[1], P, P + cap0);
 }
@@ -296,7 +315,7 @@ struct ArrSel {
 // Dispatch the item's array kinds to the body with matching element types.
 template <class C, class S, bool FWD>
 __device__ __forceinline__ void batch_item(const BatchItem& it, int bc, const ArrSel<C, S>& a, C* P) {
-  const DProgram<C>* prog = reinterpret_cast<const DProgram<C>*>(it.prog);
+  const DProgram<C>* prog = stage_program<C>(reinterpret_cast<const DProgram<C>*>(it.prog));
   const bool s0 = it.arr[0] == kArrScr, s1 = it.arr[1] == kArrScr, s2 = it.arr[2] == kArrScr;
   auto user_in = [&](int i) -> const S* { return (it.arr[i] == kArrIn ? a.in : a.out) + it.off[i]; };
   auto user_out = [&](int i) -> S* { return a.out + it.off[i]; };

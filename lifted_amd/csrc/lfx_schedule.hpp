@@ -47,7 +47,7 @@ inline std::vector<long long> probe_tiles(long long ntiles) {
 }
 
 template <class C>
-long long find_halo(const DProgram<C>& hp, bool fwd, long long m0, int k, long long T) {
+long long find_halo(const DProgram<C>& hp, bool fwd, long long m0, int k, long long T, int bc) {
   const long long ntiles = (m0 + T - 1) / T;
   const auto tiles = probe_tiles(ntiles);
   const long long hmax = fwd ? (T >> k) : T;
@@ -56,7 +56,7 @@ long long find_halo(const DProgram<C>& hp, bool fwd, long long m0, int k, long l
     bool ok = true;
     for (long long t : tiles) {
       const long long a = t * T, b = std::min(a + T, m0);
-      ok = fwd ? fwd_tile_ok(hp, m0, k, a, b, H) : rev_tile_ok(hp, m0, k, a, b, H);
+      ok = fwd ? fwd_tile_ok(hp, m0, k, a, b, H, bc) : rev_tile_ok(hp, m0, k, a, b, H, bc);
       if (!ok) break;
     }
     if (ok) return H;
@@ -76,15 +76,23 @@ LineSchedule schedule(const DProgram<C>& hp, bool fwd, long long n, long long L,
   long long m = n, lv = 0;
   const long long T0 = tile_hint > 0 ? tile_hint : (compute_bytes(storage) == 8 ? 4096 : 8192);
   while (lv < L && m > limit) {
-    int k = (int)std::min<long long>(L - lv, k_hint > 0 ? k_hint : 6);
-    long long T = T0, H = -1;
+    const int k0 = (int)std::min<long long>(L - lv, k_hint > 0 ? k_hint : 6);
+    int k = k0;
+    long long T = T0, Tk = T0, H = -1;
     for (;;) {
-      T = ((T >> k) << k);
-      H = find_halo(hp, fwd, m, k, T);
+      T = ((Tk >> k) << k);
+      H = find_halo(hp, fwd, m, k, T, bc);
       const long long halo_fine = fwd ? H : (H << k);
       const TileArgs ta{m, T, H, (m + T - 1) / T, k, 0};
-      if (H >= 0 && halo_fine * 4 <= T && tile_lds_bytes(ta, fwd, compute_bytes(storage)) <= kMaxLdsBytes) break;
+      const size_t lds = tile_lds_bytes(ta, fwd, compute_bytes(storage));
+      if (H >= 0 && halo_fine * 4 <= T && lds <= kMaxLdsBytes) break;
       if (k > 1) { --k; continue; }
+      // a tile too short for the halo of even one level: longer tiles
+      if (lds <= kMaxLdsBytes / 2 && Tk < m) {
+        Tk *= 2;
+        k = k0;
+        continue;
+      }
       fail(LFX_EUNSUPPORTED, "no tiling found for this wavelet");
     }
     s.passes.push_back(PassPlan{m, k, T, H, (m + T - 1) / T});
@@ -120,24 +128,25 @@ constexpr long long kRegWindow = 1024;  // samples per register tile window
 // K is, owned 16 (64 - 2 Hk)). -1 when the stencil is too wide for the register
 // window to pay off.
 template <class C>
-long long reg_halo(const DProgram<C>& hp, bool fwd, int K) {
+long long reg_halo(const DProgram<C>& hp, bool fwd, int K, int bc) {
   const long long m0 = 1ll << 40, a0 = 1ll << 36;
   if (fwd) {
     for (long long H = 0; 2 * H <= kRegWindow - 256; H += 16)
-      if (fwd_tile_ok(hp, m0, K, a0, a0 + kRegWindow - 2 * H, H)) return H;
+      if (fwd_tile_ok(hp, m0, K, a0, a0 + kRegWindow - 2 * H, H, bc)) return H;
   } else {
     for (long long Hk = 0; 16 * (64 - 2 * Hk) >= 256; ++Hk)
-      if (rev_tile_ok(hp, m0, K, a0, a0 + 16 * (64 - 2 * Hk), Hk << (kRegLevels - K))) return Hk;
+      if (rev_tile_ok(hp, m0, K, a0, a0 + 16 * (64 - 2 * Hk), Hk << (kRegLevels - K), bc)) return Hk;
   }
   return -1;
 }
 
 // Smallest LDS-tile halo for the owned range [a, b) (-1: none below the cap).
 template <class C>
-long long lds_halo(const DProgram<C>& hp, bool fwd, long long m0, int k, long long a, long long b, long long cap) {
+long long lds_halo(const DProgram<C>& hp, bool fwd, long long m0, int k, long long a, long long b, long long cap,
+                   int bc) {
   const long long step = fwd ? (1ll << k) : 1;
   for (long long H = 0; H <= cap; H += step) {
-    const bool ok = fwd ? fwd_tile_ok(hp, m0, k, a, b, H) : rev_tile_ok(hp, m0, k, a, b, H);
+    const bool ok = fwd ? fwd_tile_ok(hp, m0, k, a, b, H, bc) : rev_tile_ok(hp, m0, k, a, b, H, bc);
     if (ok) return H;
   }
   return -1;
@@ -145,11 +154,12 @@ long long lds_halo(const DProgram<C>& hp, bool fwd, long long m0, int k, long lo
 
 // Items of a tiled pass (band m0, k levels). `reg` allows register tiles.
 template <class C>
-std::vector<ItemSpec> pass_items(const DProgram<C>& hp, bool fwd, const PassPlan& pp, bool reg, int storage) {
+std::vector<ItemSpec> pass_items(const DProgram<C>& hp, bool fwd, const PassPlan& pp, bool reg, int storage,
+                                 int bc) {
   std::vector<ItemSpec> out;
   const long long m0 = pp.m0;
   long long r0 = 0, r1 = 0, Tr = 0, H = -1;
-  if (reg && pp.k >= 1 && pp.k <= kRegLevels) H = reg_halo(hp, fwd, pp.k);
+  if (reg && pp.k >= 1 && pp.k <= kRegLevels) H = reg_halo(hp, fwd, pp.k, bc);
   if (H >= 0) {
     // Register windows stay strictly inside the band at every level (they
     // apply no boundary map), with a margin of 64 samples at the finest level.
@@ -186,8 +196,8 @@ std::vector<ItemSpec> pass_items(const DProgram<C>& hp, bool fwd, const PassPlan
     // first and last tiles with the exact rule and move them inward until they
     // hold (interior tiles are translates of a checked one).
     auto ok = [&](long long a) {
-      return fwd ? fwd_tile_ok(hp, m0, pp.k, a, a + Tr, H)
-                 : rev_tile_ok(hp, m0, pp.k, a, a + Tr, H << (kRegLevels - pp.k));
+      return fwd ? fwd_tile_ok(hp, m0, pp.k, a, a + Tr, H, bc)
+                 : rev_tile_ok(hp, m0, pp.k, a, a + Tr, H << (kRegLevels - pp.k), bc);
     };
     const long long ntl = (r1 - r0) / Tr;
     long long first = 0, last = ntl - 1;
@@ -216,7 +226,7 @@ std::vector<ItemSpec> pass_items(const DProgram<C>& hp, bool fwd, const PassPlan
     // cut [a, b) into pieces of at most pp.T, each with its own halo
     for (long long x = a; x < b; x += pp.T) {
       const long long y = std::min(b, x + pp.T);
-      const long long h = lds_halo(hp, fwd, m0, pp.k, x, y, cap);
+      const long long h = lds_halo(hp, fwd, m0, pp.k, x, y, cap, bc);
       if (h < 0) fail(LFX_EUNSUPPORTED, "no halo found for a band-end tile");
       out.push_back(ItemSpec{kSpecTile, m0, pp.k, x, y, h});
     }

@@ -114,7 +114,7 @@ static void run(const WaveletDefL& w, bool use_reg, int tf, int bc, long long n,
       const PassPlan& pp = sc.passes[i];
       const bool last = (i + 1 == sc.passes.size()) && sc.L_line == 0;
       std::vector<C> sk(band_len(pp.m0, pp.k));
-      for (const ItemSpec& it : pass_items(prog, true, pp, use_reg, storage)) {
+      for (const ItemSpec& it : pass_items(prog, true, pp, use_reg, storage, bc)) {
         C* skp = last ? out.data() : sk.data();
         if (it.kind == kSpecReg && !std::getenv("LFX_SIM_REG_AS_LDS")) {
           std::fprintf(stderr, "  reg [%lld, %lld) H=%lld\n", it.a, it.b, it.H);
@@ -142,7 +142,7 @@ static void run(const WaveletDefL& w, bool use_reg, int tf, int bc, long long n,
     const PassPlan& pp = sc.passes[j];
     std::vector<C> band(j == 0 ? 0 : pp.m0);
     C* dst = j == 0 ? out.data() : band.data();
-    for (const ItemSpec& it : pass_items(prog, false, pp, use_reg, storage)) {
+    for (const ItemSpec& it : pass_items(prog, false, pp, use_reg, storage, bc)) {
       const C* sp = s_user ? in.data() : s.data();
       if (it.kind == kSpecReg && !std::getenv("LFX_SIM_REG_AS_LDS")) {
         std::fprintf(stderr, "  reg [%lld, %lld) Hk=%lld\n", it.a, it.b, it.H);

@@ -60,6 +60,24 @@ def test_tiled_all_wavelets(sim, oracle, w):
         np.testing.assert_array_equal(got, ref, err_msg=f"w={w} bc={bc} tf={tf} n={n}\n{info}")
 
 
+@pytest.mark.parametrize("w", [w for w in WAVELETS if w != 0])
+def test_tiled_one_sample_end_tiles(sim, oracle, w):
+    """Last tiles a single sample wide: a window touching the band end holds
+    only the halo, so mirrored reads (Symmetric / Reflect) and adjoint folds
+    must be covered by the halo rule, with plain LDS tiles and with register
+    passes whose band-end pieces are cut from the tail."""
+    g = np.random.default_rng(w + 3)
+    for bc, tf, n in itertools.product((1, 3, 4), (0, 1, 2, 3), (2561, 4097)):
+        L = min(4, oracle.max_level(w, n))
+        if not ok_bands(bc, n, L):
+            continue
+        x = g.standard_normal(n).astype(np.float32)
+        ref = oracle.lwt(x, w, tf, bc, level=[L])
+        for tile, k in ((512, 0), (64, 2)):
+            got, info = run_sim(sim, w, tf, bc, x, L, limit=200, tile=tile, k=k)
+            np.testing.assert_array_equal(got, ref, err_msg=f"w={w} bc={bc} tf={tf} n={n} tile={tile}\n{info}")
+
+
 @pytest.mark.parametrize("tile,k,limit", [(128, 1, 100), (256, 2, 60), (1024, 4, 300), (64, 0, 50)])
 def test_tiled_shapes_cdf97_float32(sim, oracle, tile, k, limit):
     g = np.random.default_rng(tile + k)
