@@ -17,6 +17,14 @@ per level) for comparison; and the CPU restatement of the reference timed on
 this host.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-plain] [--no-traffic]
+
+Secondary workloads (one JSON line each; not the headline):
+    --workload c2   4096^2 float32 sum of Gaussians, CDF 9/7, 5 levels, F / I / FA,
+                    separable and level-by-level, fused and plain
+    --workload c3   512^3 float32, CDF 9/7 levels (4, 4, 4): forward, soft-threshold
+                    loss and backward (the forward-adjoint) through autograd
+    --workload c5   16 x 8192^2 1/f^2 fields stored bf16 / fp16 / fp32, CDF 9/7, 6
+                    levels, forward + inverse: errors against an fp64 run and GB/s
 """
 import argparse
 import csv
@@ -248,6 +256,135 @@ def pmc_child():
     print("pmc child done", flush=True)
 
 
+def _time_calls(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / steps
+
+
+def workload_c2(steps, warmup, cpu):
+    """4096^2 float32, sum of 32 Gaussians + 0.01 N(0,1) noise; CDF 9/7, 5 levels."""
+    import torch
+    import lifted_amd as la
+    rng = np.random.default_rng(20261201 + 1)
+    n = 4096
+    yy, xx = np.mgrid[0:n, 0:n].astype(np.float32)
+    img = np.zeros((n, n), np.float32)
+    for _ in range(32):
+        cy, cx = rng.uniform(0, n, 2)
+        sig = np.exp(rng.uniform(np.log(64), np.log(1024)))
+        amp = rng.standard_normal()
+        img += (amp * np.exp(-((yy - cy) ** 2 + (xx - cx) ** 2) / (2 * sig * sig))).astype(np.float32)
+    img += 0.01 * rng.standard_normal((n, n)).astype(np.float32)
+    x = torch.tensor(img, device="cuda")
+    out = torch.empty_like(x)
+    N = x.numel()
+    rows = []
+    for mode_name, level in (("separable", [5, 5]), ("level-by-level", 5)):
+        for tf in (0, 1, 2):
+            for plain in (False, True):
+                ms = _time_calls(lambda: la._lwt(x, la.Wavelet.CDF9_7, 0, tf, (0, 1), level, out=out, plain=plain),
+                                 steps, warmup)
+                rows.append({"mode": mode_name, "transform": ["F", "I", "FA"][tf], "plain": plain,
+                             "ms": round(ms, 4), "GBps_compulsory": round(2 * N * 4 / (ms * 1e-3) / 1e9, 1)})
+    res = {"workload": "c2: 4096^2 float32 sum of 32 Gaussians + noise, CDF9_7, 5 levels, Zero BC",
+           "rows": rows,
+           "note": "GBps_compulsory = 2 N 4 B / t (each sample read and written once); a one-pass-per-axis "
+                   "design moves 2 N 4 B per axis pass, so its ceiling is half of that"}
+    for r in rows:
+        if not r["plain"]:
+            p = [q for q in rows if q["mode"] == r["mode"] and q["transform"] == r["transform"] and q["plain"]][0]
+            r["vs_plain"] = round(p["ms"] / r["ms"], 3)
+    if cpu:
+        import oracle as O
+        threads = min(os.cpu_count() or 1, 16)
+        t = time.perf_counter()
+        O.lwt(img, la.Wavelet.CDF9_7, O.FORWARD, O.ZERO, level=[5, 5], n_threads=threads)
+        res["cpu_baseline"] = {"ms": round((time.perf_counter() - t) * 1e3, 1), "cores": threads, "kind": "port",
+                               "sample": "separable forward, once"}
+    return res
+
+
+def workload_c3(steps, warmup):
+    """512^3 float32 N(0,1), CDF 9/7 levels (4, 4, 4): forward, loss, backward."""
+    import torch
+    import lifted_amd.autograd as AG
+    g = torch.Generator(device="cuda").manual_seed(20261201 + 2)
+    x = torch.randn(512, 512, 512, device="cuda", generator=g, requires_grad=True)
+    N = x.numel()
+
+    def step():
+        c = AG.forward(x, 100907, 0, axes=(0, 1, 2), level=(4, 4, 4))
+        loss = (torch.sign(c) * torch.clamp(c.abs() - 0.1, min=0.0)).abs().sum()
+        x.grad = None
+        loss.backward()
+
+    ms = _time_calls(step, steps, warmup)
+    c = AG.forward(x.detach(), 100907, 0, axes=(0, 1, 2), level=(4, 4, 4))
+    ms_f = _time_calls(lambda: AG.forward(x.detach(), 100907, 0, axes=(0, 1, 2), level=(4, 4, 4)), steps, warmup)
+    import lifted_amd as la
+    ms_a = _time_calls(lambda: la.lwt_forward_adjoint(c, 100907, 0, axes=(0, 1, 2), level=[4, 4, 4]), steps, warmup)
+    return {"workload": "c3: 512^3 float32 N(0,1), CDF9_7 levels (4,4,4), Zero BC, autograd F + loss + FA",
+            "ms_per_iteration": round(ms, 3), "forward_ms": round(ms_f, 3), "adjoint_ms": round(ms_a, 3),
+            "GBps_compulsory_forward": round(2 * N * 4 / (ms_f * 1e-3) / 1e9, 1),
+            "GBps_compulsory_adjoint": round(2 * N * 4 / (ms_a * 1e-3) / 1e9, 1),
+            "GBps_per_axis_pass_forward": round(3 * 2 * N * 4 / (ms_f * 1e-3) / 1e9, 1)}
+
+
+def workload_c5(steps, warmup):
+    """16 x 8192^2 1/f^2 fields, CDF 9/7, 6 levels (level-by-level), F + I."""
+    import torch
+    import lifted_amd as la
+    B, n = 16, 8192
+    g = torch.Generator(device="cuda").manual_seed(20261201 + 4)
+    x64 = torch.empty(B, n, n, dtype=torch.float64, device="cuda")
+    ky = torch.fft.fftfreq(n, device="cuda").view(n, 1)
+    kx = torch.fft.rfftfreq(n, device="cuda").view(1, n // 2 + 1)
+    k2 = ky * ky + kx * kx
+    filt = torch.where(k2 > 0, 1.0 / k2, torch.zeros_like(k2))
+    for b in range(B):
+        w = torch.randn(n, n, device="cuda", generator=g)
+        f = torch.fft.irfft2(torch.fft.rfft2(w) * filt, s=(n, n))
+        x64[b] = ((f - f.mean()) / f.std()).double()
+        del w, f
+    axes = (1, 2)
+    ref = la.lwt_forward(x64, 100907, 0, axes=axes, level=6)
+    rows = []
+    for name, dt in (("fp32", torch.float32), ("bf16", torch.bfloat16), ("fp16", torch.float16)):
+        xs = x64.to(dt)
+        y = la.lwt_forward(xs, 100907, 0, axes=axes, level=6)
+        z = la.lwt_inverse(y, 100907, 0, axes=axes, level=6)
+        e_coef = float((y.double() - ref).norm() / ref.norm())
+        e_rt = float((z.double() - xs.double()).norm() / xs.double().norm())
+        e_rt64 = float((z.double() - x64).norm() / x64.norm())
+        yo = torch.empty_like(xs)
+        zo = torch.empty_like(xs)
+
+        def fi():
+            la.lwt_forward(xs, 100907, 0, axes=axes, level=6, out=yo)
+            la.lwt_inverse(yo, 100907, 0, axes=axes, level=6, out=zo)
+
+        ms = _time_calls(fi, steps, warmup)
+        es = xs.element_size()
+        rows.append({"storage": name, "coef_rel_err_vs_fp64": e_coef, "roundtrip_rel_err": e_rt,
+                     "roundtrip_rel_err_vs_fp64_input": e_rt64, "ms_F_plus_I": round(ms, 3),
+                     "GBps_compulsory": round(4 * xs.numel() * es / (ms * 1e-3) / 1e9, 1),
+                     "Gsamples_per_s": round(2 * xs.numel() / (ms * 1e-3) / 1e9, 2)})
+        del xs, y, z, yo, zo
+        torch.cuda.empty_cache()
+    return {"workload": "c5: 16 x 8192^2 1/f^2 fields (unit variance), CDF9_7, level=6 (level by level), "
+                        "Zero BC, forward + inverse; errors are relative L2 against the same kernels in fp64",
+            "rows": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -260,6 +397,8 @@ def main():
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a device copy of the workload's input size (HBM counter calibration)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--workload", choices=["c4", "c2", "c3", "c5"], default="c4",
+                    help="c4 (default): the batched headline; c2 / c3 / c5: secondary workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); this build has no multi-GPU path: "
@@ -268,6 +407,14 @@ def main():
         sys.exit(2)
     if args.pmc_child:
         pmc_child()
+        return
+    if args.workload != "c4":
+        import torch
+        torch.cuda.set_device(0)
+        steps = min(args.steps, 10)
+        fn = {"c2": lambda: workload_c2(steps, 2, not args.no_cpu), "c3": lambda: workload_c3(steps, 2),
+              "c5": lambda: workload_c5(steps, 1)}[args.workload]
+        print(json.dumps(fn()))
         return
     import torch
     import lifted_amd as la

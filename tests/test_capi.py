@@ -138,3 +138,14 @@ def test_custom_program_validation():
     w = la.CustomWavelet([{"kind": "generic", "target": "d", "offset": -1, "taps": [0.5, 0.5]}], width=3)
     p = la.program_dump(w, 1)
     assert p.shape[0] == 1 and p[0, 9] == -0.5 and p[0, 10] == -0.5
+
+
+def test_cpp_dropin_header_compiles_and_throws():
+    """include/lifted.hpp: a reference-style caller compiles against it and
+    links the library; invalid arguments throw std::invalid_argument before
+    any device work."""
+    d = os.path.join(REPO, "tests", "cpp_shim")
+    subprocess.run(["make", "-s", "-C", d], check=True)
+    r = subprocess.run([os.path.join(d, "build", "shim_main")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "errors_thrown 3/3 max_level_db2_65536 14" in r.stdout

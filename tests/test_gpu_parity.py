@@ -7,6 +7,7 @@ same fused multiply-adds, so the expected difference is zero; the measured
 maximum is asserted against those tolerances and printed.
 """
 import itertools
+import os
 
 import numpy as np
 import pytest
@@ -210,3 +211,16 @@ def test_long_lines_2d_strided_axis(gpu, oracle):
         _check(la.lwt(t, 100907, tf, mode=3, axes=[0], level=[6]).cpu().numpy(), ref, np.float32, f"tf={tf}")
         la.lwt(t, 100907, tf, mode=3, axes=[0], level=[6], out=t)
         _check(t.cpu().numpy(), ref, np.float32, f"in-place tf={tf}")
+
+
+@pytest.mark.gpu
+def test_cpp_dropin_header_round_trips(gpu):
+    """The four lifted::lwt overloads of include/lifted.hpp on host arrays:
+    Inverse(Forward(x)) == x for both paths and types, and the dot-product
+    identity of Forward / ForwardAdjoint."""
+    import subprocess
+    d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp_shim")
+    subprocess.run(["make", "-s", "-C", d], check=True)
+    r = subprocess.run([os.path.join(d, "build", "shim_main"), "all"], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
