@@ -597,8 +597,10 @@ static std::vector<Group> signal_groups(const WaveletDefL& w, int tf, int storag
   std::vector<Group> g;
   const int k_hint = plain ? 1 : (g_tuning.k > 0 ? g_tuning.k : (reg ? kRegLevels : 0));
   // with register tiles, bands longer than the whole-line threshold are tiled so
-  // that whole-line items stay small (several workgroups per CU)
-  const long long limit = reg ? (g_tuning.line > 0 ? g_tuning.line : 4096) : 0;
+  // that whole-line items stay small (several workgroups per CU); with a Zero
+  // boundary register tiles also cover band ends, so only short bands are left
+  // to the (latency-bound) whole-line kernels
+  const long long limit = reg ? (g_tuning.line > 0 ? g_tuning.line : (bc == kZero ? 256 : 4096)) : 0;
   const LineSchedule sc = schedule(hp, fwd, n, L, storage, bc, g_tuning.tile, k_hint, bc == kPeriodic ? 0 : limit);
   for (auto& p : sc.passes) g.push_back(Group{true, p.m0, p.k, p.T, p.H, p.ntiles});
   if (sc.L_line > 0) {

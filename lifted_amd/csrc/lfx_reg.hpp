@@ -78,10 +78,40 @@ struct LaneCpu {
     for (int l = 0; l < kLanes; ++l)
       if (pred(l)) dev::store_as<S, float>(p, idx(l), v.v[l]);
   }
+  // bit e of lane l: position p0 + E l + e lies in [0, n)
+  struct M {
+    uint32_t m[kLanes];
+  };
+  static M band_mask(int E, long long p0, long long n) {
+    M r;
+    for (int l = 0; l < kLanes; ++l) {
+      r.m[l] = 0;
+      for (int e = 0; e < E; ++e) {
+        const long long g = p0 + (long long)E * l + e;
+        if (g >= 0 && g < n) r.m[l] |= 1u << e;
+      }
+    }
+    return r;
+  }
+  static V keep(const M& m, int e, V v) {
+    for (int l = 0; l < kLanes; ++l)
+      if (!((m.m[l] >> e) & 1u)) v.v[l] = 0.0f;
+    return v;
+  }
+  template <class S, class F, class G>
+  static V ld(const S* p, F idx, G pred) {
+    V r;
+    for (int l = 0; l < kLanes; ++l) r.v[l] = pred(l) ? dev::load_as<S, float>(p, idx(l)) : 0.0f;
+    return r;
+  }
+  // x[e] of lane l = p[pos0 + E l + e] where that position lies in [0, n), else 0
   template <int E, class S>
-  static void ld_chunks(const S* p, long long base, V (&x)[E], float*) {
+  static void ld_chunks(const S* p, long long pos0, long long n, V (&x)[E], float*) {
     for (int e = 0; e < E; ++e)
-      for (int l = 0; l < kLanes; ++l) x[e].v[l] = dev::load_as<S, float>(p, base + (long long)E * l + e);
+      for (int l = 0; l < kLanes; ++l) {
+        const long long g = pos0 + (long long)E * l + e;
+        x[e].v[l] = (g >= 0 && g < n) ? dev::load_as<S, float>(p, g) : 0.0f;
+      }
   }
   template <int E, class S>
   static void st_chunks(S* p, long long pos0, const V (&v)[E], long long g0, long long lo, long long hi, float*) {
@@ -118,16 +148,39 @@ struct LaneGpu {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
-  // x[e] = p[base + E * lane + e] for e < E: coalesced global loads (lane l
-  // reads base + 64 j + l), transposed through the wave's LDS buffer (row
-  // stride E + 1 keeps the chunk reads conflict-free).
-  template <int E, class S>
-  static __device__ __forceinline__ void ld_chunks(const S* p, long long base, V (&x)[E], float* lds) {
+  // bit e: position p0 + E lane + e lies in [0, n)
+  using M = uint32_t;
+  static __device__ __forceinline__ M band_mask(int E, long long p0, long long n) {
+    const long long g = p0 + (long long)E * lane();
+    M r = 0;
+    for (int e = 0; e < E; ++e) r |= (g + e >= 0 && g + e < n) ? (1u << e) : 0u;
+    return r;
+  }
+  static __device__ __forceinline__ V keep(M m, int e, V v) { return ((m >> e) & 1u) ? v : 0.0f; }
+  template <class S, class F, class G>
+  static __device__ __forceinline__ V ld(const S* p, F idx, G pred) {
     const int l = lane();
-    const S* q = p + base;
+    return pred(l) ? dev::load_as<S, float>(p, idx(l)) : 0.0f;
+  }
+  // x[e] = p[pos0 + E * lane + e] (0 outside [0, n)): coalesced global loads
+  // (lane l reads pos0 + 64 j + l), transposed through the wave's LDS buffer
+  // (row stride E + 1 keeps the chunk reads conflict-free). The bounds checks
+  // run only for windows that overlap a band end (a wave-uniform test).
+  template <int E, class S>
+  static __device__ __forceinline__ void ld_chunks(const S* p, long long pos0, long long n, V (&x)[E], float* lds) {
+    const int l = lane();
     float t[E];
+    if (pos0 >= 0 && pos0 + E * kLanes <= n) {
+      const S* q = p + pos0;
 #pragma unroll
-    for (int j = 0; j < E; ++j) t[j] = dev::load_as<S, float>(q, j * kLanes + l);  // all loads in flight
+      for (int j = 0; j < E; ++j) t[j] = dev::load_as<S, float>(q, j * kLanes + l);  // all loads in flight
+    } else {
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        const long long g = pos0 + j * kLanes + l;
+        t[j] = (g >= 0 && g < n) ? dev::load_as<S, float>(p, g) : 0.0f;
+      }
+    }
 #pragma unroll
     for (int j = 0; j < E; ++j) {
       const int u = j * kLanes + l;
@@ -214,9 +267,13 @@ __device__ __forceinline__ void rstep(typename P::V (&y)[E], const typename P::V
 }
 
 // All steps of one level (wavelet W, transform TF) on E elements per lane.
+// With masks (windows overlapping a band end, Zero boundary), the updated band
+// is zeroed outside its range after every step: the next steps then read exact
+// zeros there, which is what skipping out-of-range terms computes.
 template <class P, class W, int TF, int E>
 __device__ __forceinline__ void level(typename P::V (&s)[E], typename P::V (&d)[E],
-                                      const DProgram<float>* __restrict__ prog) {
+                                      const DProgram<float>* __restrict__ prog, bool masked,
+                                      const typename P::M& ms, const typename P::M& md) {
   using L = typename W::list;
   sfor<L::size>([&](auto I) {
     constexpr int i = decltype(I)::value;
@@ -230,8 +287,10 @@ __device__ __forceinline__ void level(typename P::V (&s)[E], typename P::V (&d)[
       });
     } else if constexpr (ST::y_is_d) {
       rstep<P, E, ST::kind, ST::off, ST::n, ST::sub>(d, s, cf);
+      if (masked) sfor<E>([&](auto e) { d[e] = P::keep(md, decltype(e)::value, d[e]); });
     } else {
       rstep<P, E, ST::kind, ST::off, ST::n, ST::sub>(s, d, cf);
+      if (masked) sfor<E>([&](auto e) { s[e] = P::keep(ms, decltype(e)::value, s[e]); });
     }
   });
 }
@@ -268,93 +327,137 @@ __device__ __forceinline__ void store_band(S* p, long long pos0, const typename 
 }
 
 // Forward tile (Forward / InverseAdjoint) of K (1..4) levels: owned [a, b)
-// of the input band of length m0, window [a - H, a - H + 1024) inside the
-// band. Writes d_l of the owned range to out[ns_l + i] (l = 1..K) and s_K to
-// sout[i]. K is uniform over the wavefront.
+// of the input band of length m0 (b == m0 for the last tile), window
+// [w0, w0 + 1024) with w0 = a - H. Writes d_l of the owned range to
+// out[ns_l + i] (l = 1..K) and s_K to sout[i]. A window inside the band reads
+// no boundary; one that overlaps a band end (Zero boundary only) runs masked.
+// K is uniform over the wavefront.
 template <class P, class W, int TF, class SI, class SO, class SK>
 __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long m0, int K, long long a, long long b,
                                          long long H, const SI* in, SO* out, SK* sout, float* lds) {
   using V = typename P::V;
-  const long long w0 = a - H;
+  using M = typename P::M;
+  const long long w0 = a - H;  // multiple of 16
+  const bool masked = w0 < 0 || w0 + kWindow > m0;
   V x[16];
-  P::template ld_chunks<16>(in, w0, x, lds);
+  P::template ld_chunks<16>(in, w0, m0, x, lds);
   V s1[8], d1[8], s2[4], d2[4], s3[2], d3[2], s4[1], d4[1];
+  M ms{}, md{};
+  long long m = m0, ns = m - m / 2, nd = m / 2;
+  // owned end of a band after l levels: the band end for the last tile
+  auto hi_d = [&](int l) { return b >= m0 ? nd : (b >> l); };
+  auto hi_s = [&](int l) { return b >= m0 ? ns : (b >> l); };
   split<P, 8>(x, s1, d1);
-  level<P, W, TF, 8>(s1, d1, prog);
-  long long m = m0;
-  long long ns = m - m / 2;
-  P::template st_chunks<8>(out, ns, d1, w0 / 2, a >> 1, b >> 1, lds);
+  if (masked) {
+    ms = P::band_mask(8, w0 / 2, ns);
+    md = P::band_mask(8, w0 / 2, nd);
+  }
+  level<P, W, TF, 8>(s1, d1, prog, masked, ms, md);
+  P::template st_chunks<8>(out, ns, d1, w0 / 2, a >> 1, hi_d(1), lds);
   if (K == 1) {
-    P::template st_chunks<8>(sout, 0, s1, w0 / 2, a >> 1, b >> 1, lds);
+    P::template st_chunks<8>(sout, 0, s1, w0 / 2, a >> 1, hi_s(1), lds);
     return;
   }
+  m = ns;
+  ns = m - m / 2;
+  nd = m / 2;
   split<P, 4>(s1, s2, d2);
-  level<P, W, TF, 4>(s2, d2, prog);
-  m = ns;
-  ns = m - m / 2;
-  P::template st_chunks<4>(out, ns, d2, w0 / 4, a >> 2, b >> 2, lds);
+  if (masked) {
+    ms = P::band_mask(4, w0 / 4, ns);
+    md = P::band_mask(4, w0 / 4, nd);
+  }
+  level<P, W, TF, 4>(s2, d2, prog, masked, ms, md);
+  P::template st_chunks<4>(out, ns, d2, w0 / 4, a >> 2, hi_d(2), lds);
   if (K == 2) {
-    P::template st_chunks<4>(sout, 0, s2, w0 / 4, a >> 2, b >> 2, lds);
+    P::template st_chunks<4>(sout, 0, s2, w0 / 4, a >> 2, hi_s(2), lds);
     return;
   }
+  m = ns;
+  ns = m - m / 2;
+  nd = m / 2;
   split<P, 2>(s2, s3, d3);
-  level<P, W, TF, 2>(s3, d3, prog);
-  m = ns;
-  ns = m - m / 2;
-  P::template st_chunks<2>(out, ns, d3, w0 / 8, a >> 3, b >> 3, lds);
+  if (masked) {
+    ms = P::band_mask(2, w0 / 8, ns);
+    md = P::band_mask(2, w0 / 8, nd);
+  }
+  level<P, W, TF, 2>(s3, d3, prog, masked, ms, md);
+  P::template st_chunks<2>(out, ns, d3, w0 / 8, a >> 3, hi_d(3), lds);
   if (K == 3) {
-    P::template st_chunks<2>(sout, 0, s3, w0 / 8, a >> 3, b >> 3, lds);
+    P::template st_chunks<2>(sout, 0, s3, w0 / 8, a >> 3, hi_s(3), lds);
     return;
   }
-  split<P, 1>(s3, s4, d4);
-  level<P, W, TF, 1>(s4, d4, prog);
   m = ns;
   ns = m - m / 2;
-  store_band<P, 1>(out, ns, d4, w0 / 16, a >> 4, b >> 4);
-  store_band<P, 1>(sout, 0, s4, w0 / 16, a >> 4, b >> 4);
+  nd = m / 2;
+  split<P, 1>(s3, s4, d4);
+  if (masked) {
+    ms = P::band_mask(1, w0 / 16, ns);
+    md = P::band_mask(1, w0 / 16, nd);
+  }
+  level<P, W, TF, 1>(s4, d4, prog, masked, ms, md);
+  store_band<P, 1>(out, ns, d4, w0 / 16, a >> 4, hi_d(4));
+  store_band<P, 1>(sout, 0, s4, w0 / 16, a >> 4, hi_s(4));
 }
 
 // Reverse tile (Inverse / ForwardAdjoint) of K (1..4) levels: owned [a, b) of
-// the output band of length m0. Windows are counted in level-4 units whatever
-// K is: with g0 = (a >> 4) - Hk, the level-l windows are
-// [2^(4-l) g0, 2^(4-l) (g0 + 64)), inside their bands. Reads s_K from sin[i]
-// and d_l from in[ns_l + i]; writes the owned band values to out[q].
+// the output band of length m0 (b == m0 for the last tile). Windows are
+// counted in level-4 units whatever K is: with g0 = (a >> 4) - Hk, the level-l
+// windows are [2^(4-l) g0, 2^(4-l) (g0 + 64)). Reads s_K from sin[i] and d_l
+// from in[ns_l + i]; writes the owned band values to out[q]. Windows that
+// overlap a band end (Zero boundary only) run masked.
 template <class P, class W, int TF, class SI, class SK, class SO>
 __device__ __forceinline__ void rev_tile(const DProgram<float>* prog, long long m0, int K, long long a, long long b,
                                          long long Hk, const SI* in, const SK* sin, SO* out, float* lds) {
   using V = typename P::V;
+  using M = typename P::M;
   const long long g0 = (a >> 4) - Hk;
-  const long long ns1 = m0 - m0 / 2;
-  const long long ns2 = ns1 - ns1 / 2;
-  const long long ns3 = ns2 - ns2 / 2;
-  const long long ns4 = ns3 - ns3 / 2;
+  const bool masked = g0 < 0 || 16 * (g0 + 64) > m0;
+  const long long m1 = m0, m2 = m1 - m1 / 2, m3 = m2 - m2 / 2, m4 = m3 - m3 / 2;  // band split by level l
+  const long long ns1 = m1 - m1 / 2, ns2 = m2 - m2 / 2, ns3 = m3 - m3 / 2, ns4 = m4 - m4 / 2;
   V s4[1], d4[1], s3[2], d3[2], s2[4], d2[4], s1[8], d1[8], x[16];
+  M ms{}, md{};
   if (K == 4) {
-    s4[0] = P::ld(sin, [&](int l) { return g0 + l; });
-    d4[0] = P::ld(in, [&](int l) { return ns4 + g0 + l; });
-    level<P, W, TF, 1>(s4, d4, prog);
+    s4[0] = P::ld(sin, [&](int l) { return g0 + l; }, [&](int l) { return g0 + l >= 0 && g0 + l < ns4; });
+    d4[0] = P::ld(in + ns4, [&](int l) { return g0 + l; }, [&](int l) { return g0 + l >= 0 && g0 + l < m4 / 2; });
+    if (masked) {
+      ms = P::band_mask(1, g0, ns4);
+      md = P::band_mask(1, g0, m4 / 2);
+    }
+    level<P, W, TF, 1>(s4, d4, prog, masked, ms, md);
     merge<P, 1>(s4, d4, s3);
   } else if (K == 3) {
-    P::template ld_chunks<2>(sin, 2 * g0, s3, lds);
+    P::template ld_chunks<2>(sin, 2 * g0, ns3, s3, lds);
   }
   if (K >= 3) {
-    P::template ld_chunks<2>(in, ns3 + 2 * g0, d3, lds);
-    level<P, W, TF, 2>(s3, d3, prog);
+    P::template ld_chunks<2>(in + ns3, 2 * g0, m3 / 2, d3, lds);
+    if (masked) {
+      ms = P::band_mask(2, 2 * g0, ns3);
+      md = P::band_mask(2, 2 * g0, m3 / 2);
+    }
+    level<P, W, TF, 2>(s3, d3, prog, masked, ms, md);
     merge<P, 2>(s3, d3, s2);
   } else if (K == 2) {
-    P::template ld_chunks<4>(sin, 4 * g0, s2, lds);
+    P::template ld_chunks<4>(sin, 4 * g0, ns2, s2, lds);
   }
   if (K >= 2) {
-    P::template ld_chunks<4>(in, ns2 + 4 * g0, d2, lds);
-    level<P, W, TF, 4>(s2, d2, prog);
+    P::template ld_chunks<4>(in + ns2, 4 * g0, m2 / 2, d2, lds);
+    if (masked) {
+      ms = P::band_mask(4, 4 * g0, ns2);
+      md = P::band_mask(4, 4 * g0, m2 / 2);
+    }
+    level<P, W, TF, 4>(s2, d2, prog, masked, ms, md);
     merge<P, 4>(s2, d2, s1);
   } else {
-    P::template ld_chunks<8>(sin, 8 * g0, s1, lds);
+    P::template ld_chunks<8>(sin, 8 * g0, ns1, s1, lds);
   }
-  P::template ld_chunks<8>(in, ns1 + 8 * g0, d1, lds);
-  level<P, W, TF, 8>(s1, d1, prog);
+  P::template ld_chunks<8>(in + ns1, 8 * g0, m1 / 2, d1, lds);
+  if (masked) {
+    ms = P::band_mask(8, 8 * g0, ns1);
+    md = P::band_mask(8, 8 * g0, m1 / 2);
+  }
+  level<P, W, TF, 8>(s1, d1, prog, masked, ms, md);
   merge<P, 8>(s1, d1, x);
-  P::template st_chunks<16>(out, 0, x, 16 * g0, a, b, lds);
+  P::template st_chunks<16>(out, 0, x, 16 * g0, a, b >= m0 ? m0 : b, lds);
 }
 
 }  // namespace reg

@@ -140,6 +140,23 @@ long long reg_halo(const DProgram<C>& hp, bool fwd, int K, int bc) {
   return -1;
 }
 
+// Register tiles may cover a band's ends (Zero boundary, masked windows) when
+// every band of the pass is long enough that no step's stencil spans the band:
+// the reference updates some elements twice there (n_front > nd - n_back), which
+// zero padding would not reproduce.
+template <class C>
+bool zero_cover_ok(const DProgram<C>& hp, long long m0, int k) {
+  long long reach = 0;
+  for (int i = 0; i < hp.nsteps; ++i)
+    if (hp.steps[i].kind != kScale) reach = std::max<long long>(reach, hp.steps[i].nf + hp.steps[i].nb);
+  long long m = m0;
+  for (int l = 1; l <= k; ++l) {
+    if (m / 2 <= reach + 1) return false;
+    m = m - m / 2;
+  }
+  return true;
+}
+
 // Smallest LDS-tile halo for the owned range [a, b) (-1: none below the cap).
 template <class C>
 long long lds_halo(const DProgram<C>& hp, bool fwd, long long m0, int k, long long a, long long b, long long cap,
@@ -160,6 +177,25 @@ std::vector<ItemSpec> pass_items(const DProgram<C>& hp, bool fwd, const PassPlan
   const long long m0 = pp.m0;
   long long r0 = 0, r1 = 0, Tr = 0, H = -1;
   if (reg && pp.k >= 1 && pp.k <= kRegLevels) H = reg_halo(hp, fwd, pp.k, bc);
+  if (H >= 0 && bc == kZero && zero_cover_ok(hp, m0, pp.k)) {
+    // Zero boundary: register windows may overlap the band ends (they run
+    // masked, reading zeros there), so register tiles cover the whole band
+    const long long Tr = fwd ? kRegWindow - 2 * H : 16 * (64 - 2 * H);
+    const long long nt = (m0 + Tr - 1) / Tr;
+    auto ok = [&](long long t) {
+      const long long a = t * Tr, b = std::min(a + Tr, m0);
+      return fwd ? fwd_tile_ok(hp, m0, pp.k, a, b, H, bc)
+                 : rev_tile_ok(hp, m0, pp.k, a, b, H << (kRegLevels - pp.k), bc);
+    };
+    bool all = true;
+    for (long long t : probe_tiles(nt)) all = all && ok(t);
+    if (all) {
+      for (long long t = 0; t < nt; ++t)
+        out.push_back(ItemSpec{kSpecReg, m0, pp.k, t * Tr, std::min(t * Tr + Tr, m0), H});
+      (void)storage;
+      return out;
+    }
+  }
   if (H >= 0) {
     // Register windows stay strictly inside the band at every level (they
     // apply no boundary map), with a margin of 64 samples at the finest level.

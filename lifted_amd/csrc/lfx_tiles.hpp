@@ -69,11 +69,9 @@ LFX_HD void step_shrink(const DStep& st, long long ns, long long nd, Ranges& r, 
       if (xlo <= 0 && xhi < 1) ylo = lmax(ylo, -(long long)st.off);
       if (xhi >= nx && xlo > nx - 1) yhi = lmin(yhi, nx - st.off - st.N + 1);
     }
-  } else {
-    const bool folds = bc != kZero;
-    if (st.fold_off < 0 && (xlo > 0 || (folds && xhi < -(long long)st.fold_off)))
-      ylo = lmax(ylo, 1 - (long long)st.fold_off);
-    if (xhi < nx || (folds && xlo > lmax(0, nd + st.off))) yhi = lmin(yhi, 2 * ny - nx + st.off - 1);
+  } else if (bc != kZero) {  // a Zero boundary folds nothing
+    if (st.fold_off < 0 && (xlo > 0 || xhi < -(long long)st.fold_off)) ylo = lmax(ylo, 1 - (long long)st.fold_off);
+    if (xhi < nx || xlo > lmax(0, nd + st.off)) yhi = lmin(yhi, 2 * ny - nx + st.off - 1);
   }
   if (yhi < ylo) yhi = ylo;
 }

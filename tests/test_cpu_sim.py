@@ -154,6 +154,22 @@ def test_register_tiles_all_wavelets(sim, oracle, w):
 
 
 @pytest.mark.parametrize("w", [w for w in WAVELETS if w != 0])
+def test_register_tiles_zero_boundary_whole_band(sim, oracle, w):
+    """Zero boundary: register windows overlap the band ends and run masked,
+    so register tiles cover whole bands, down to bands of a few hundred
+    samples (one window holds the band) -- no LDS pieces at the ends."""
+    g = np.random.default_rng(w + 13)
+    for tf, n in itertools.product((0, 1, 2, 3), (5003, 700, 1024, 333)):
+        L = min(9, oracle.max_level(w, n))
+        x = g.standard_normal(n).astype(np.float32)
+        ref = oracle.lwt(x, w, tf, 0, level=[L])
+        got, info = run_sim(sim, w, tf, 0, x, L, limit=64, tile=8192, k=4)
+        np.testing.assert_array_equal(got, ref, err_msg=f"w={w} tf={tf} n={n}\n{info}")
+        if n >= 1024:
+            assert "reg [0," in info, info
+
+
+@pytest.mark.parametrize("w", [w for w in WAVELETS if w != 0])
 def test_register_tiles_fewer_levels(sim, oracle, w):
     """Register passes of one to three levels (level counts that are not a
     multiple of four; the last pass of a 5-level transform is one level)."""
