@@ -224,6 +224,7 @@ struct Tuning {
   int k = 0;             // 0: default levels per pass
   long long line = 0;    // 0: default whole-line threshold of register-tiled plans
   int streams = -1;      // side streams of a batched stage (-1: default)
+  int nd_batch = -1;     // ND lines through batched plans (0: off)
   long long small = -1;  // LDS bytes up to which batch items run as one-wave workgroups (-1: default)
 };
 static Tuning env_tuning() {
@@ -233,9 +234,13 @@ static Tuning env_tuning() {
   if (const char* e = std::getenv("LFX_LINE")) t.line = std::atoll(e);
   if (const char* e = std::getenv("LFX_STREAMS")) t.streams = std::atoi(e);
   if (const char* e = std::getenv("LFX_SMALL")) t.small = std::atoll(e);
+  if (const char* e = std::getenv("LFX_ND_BATCH")) t.nd_batch = std::atoi(e);
   return t;
 }
 static thread_local Tuning g_tuning = env_tuning();
+
+static bool lines_fast(const Plan& p, const long long* shp, int ax, long long L, const void* src, const long long* sst,
+                       void* dst, const long long* dstr, hipStream_t st);
 
 template <class C>
 static void transform_lines_t(const Plan& p, const WaveletDefL& w, const void* prog, bool fwd, const long long* shp,
@@ -246,6 +251,7 @@ static void transform_lines_t(const Plan& p, const WaveletDefL& w, const void* p
   for (int k = 0; k < p.ndim; ++k)
     if (k != ax) nlines *= shp[k];
   if (nlines == 0 || n == 0 || L == 0) return;
+  if (lines_fast(p, shp, ax, L, src, sst, dst, dstr, st)) return;
   const DProgram<C> hp = resolve<C>(w, p.tf);
   // LFX_FLAG_PLAIN: one tiled pass per level (one launch per level, the band
   // round-trips through memory between levels); Periodic keeps whole lines
@@ -619,8 +625,16 @@ static std::vector<Group> signal_groups(const WaveletDefL& w, int tf, int storag
 
 static void free_plan(BatchPlanImpl* p);
 static void plan_streams(BatchPlanImpl* p);
+static void batch_stage(BatchPlanImpl* p, int st, const void* in, void* out, hipStream_t s,
+                        const std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev);
+static void batch_stage_all(BatchPlanImpl* p, const void* in, void* out, hipStream_t s) {
+  for (int i = 0; i < (int)p->stages.size(); ++i) batch_stage(p, i, in, out, s, nullptr);
+}
 
-static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int bc, int storage, uint32_t flags) {
+// nrep > 1: the signal list is repeated nrep times, copy r shifted by
+// r * rep_stride elements (many equal lines: items are computed once).
+static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int bc, int storage, uint32_t flags,
+                                   long long nrep = 1, long long rep_stride = 0) {
   if (tf < 0 || tf > 3) fail(LFX_EINVAL, "unknown transform");
   if (bc < 0 || bc > 4) fail(LFX_EINVAL, "Unknown Boundary Condition");
   if (storage < 0 || storage > 3) fail(LFX_EINVAL, "unknown storage type");
@@ -809,6 +823,34 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
       }
     }
   }
+  if (nrep > 1) {
+    // replicate: user-array offsets shift by rep_stride, scratch by scr_elems
+    auto shift = [&](BatchItem it, long long r) {
+      for (int a = 0; a < 3; ++a) it.off[a] += r * (it.arr[a] == kArrScr ? scr_elems : rep_stride);
+      return it;
+    };
+    for (size_t st = 0; st < per_stage.size(); ++st) {
+      const size_t n0 = per_stage[st].size();
+      per_stage[st].reserve(n0 * nrep);
+      for (long long r = 1; r < nrep; ++r)
+        for (size_t j = 0; j < n0; ++j) {
+          per_stage[st].push_back(shift(per_stage[st][j], r));
+          cost[st].push_back(cost[st][j]);
+          item_lds[st].push_back(item_lds[st][j]);
+          item_bytes[st].push_back(item_bytes[st][j]);
+        }
+      bytes[st] *= (double)nrep;
+    }
+    for (size_t st = 0; st < reg_stage.size(); ++st) {
+      const size_t n0 = reg_stage[st].size();
+      reg_stage[st].reserve(n0 * nrep);
+      for (long long r = 1; r < nrep; ++r)
+        for (size_t j = 0; j < n0; ++j) reg_stage[st].push_back(shift(reg_stage[st][j], r));
+      reg_bytes[st] *= (double)nrep;
+    }
+    scr_elems *= nrep;
+    plan->total *= nrep;
+  }
   // each stage: items grouped into LDS classes (one launch per class, so small
   // tiles are not launched with the LDS reservation of the longest line), each
   // class ordered by decreasing cost; upload. Items of the smallest class
@@ -911,7 +953,7 @@ static void plan_streams(BatchPlanImpl* p) {
 // ev (optional): one start/end event pair per launch, recorded on the
 // launch's stream (LDS launches in order, then the register launch)
 static void batch_stage(BatchPlanImpl* p, int st, const void* in, void* out, hipStream_t s,
-                        const std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev = nullptr) {
+                        const std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev) {
   if (st < 0 || st >= (int)p->stages.size()) fail(LFX_EINVAL, "stage out of range");
   int dev = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
@@ -956,6 +998,122 @@ static void batch_stage(BatchPlanImpl* p, int st, const void* in, void* out, hip
       hip_check(hipStreamWaitEvent(s, p->join[i], 0), "hipStreamWaitEvent");
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Lines of an ND box through batched plans: the box is copied into a dense
+// float workspace with the transform axis contiguous (a tiled transpose when
+// another dim is contiguous in the box, with the storage conversion), the
+// lines run as one batched plan (register tiles), and the result is copied
+// back. Plans are cached per (device, lines, length, wavelet, levels,
+// transform, boundary).
+
+struct LinePlanKey {
+  int dev;
+  long long nlines, n, L;
+  uint32_t wavelet;
+  int tf, bc;
+  bool operator<(const LinePlanKey& o) const {
+    return std::tie(dev, nlines, n, L, wavelet, tf, bc) < std::tie(o.dev, o.nlines, o.n, o.L, o.wavelet, o.tf, o.bc);
+  }
+};
+static std::map<LinePlanKey, std::pair<BatchPlanImpl*, unsigned long long>> g_line_plans;
+static unsigned long long g_line_tick = 0;
+static constexpr size_t kLinePlanCache = 32;
+
+static BatchPlanImpl* line_plan(const LinePlanKey& k) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_line_plans.find(k);
+  if (it != g_line_plans.end()) {
+    it->second.second = ++g_line_tick;
+    return it->second.first;
+  }
+  if (g_line_plans.size() >= kLinePlanCache) {
+    auto old = g_line_plans.begin();
+    for (auto j = g_line_plans.begin(); j != g_line_plans.end(); ++j)
+      if (j->second.second < old->second.second) old = j;
+    (void)hipDeviceSynchronize();  // the evicted plan may still be in flight on a caller's stream
+    free_plan(old->second.first);
+    g_line_plans.erase(old);
+  }
+  lfx_signal sg{0, k.n, k.wavelet, (int32_t)k.L};
+  BatchPlanImpl* bp = batch_create(&sg, 1, k.tf, k.bc, kF32, 0, k.nlines, k.n);
+  g_line_plans[k] = {bp, ++g_line_tick};
+  return bp;
+}
+
+static PermMap perm_map(int ndim, const long long* shp, const long long* s_in, const long long* s_out, int fi,
+                        int fo) {
+  PermMap pm{};
+  pm.same = fi == fo;
+  pm.ni = shp[fi];
+  pm.si_i = s_in[fi];
+  pm.so_i = s_out[fi];
+  pm.no = pm.same ? 1 : shp[fo];
+  pm.si_o = pm.same ? 0 : s_in[fo];
+  pm.so_o = pm.same ? 0 : s_out[fo];
+  pm.nbatch = 1;
+  for (int k = 0; k < ndim; ++k) {
+    if (k == fi || k == fo) continue;
+    pm.bshape[pm.nb] = shp[k];
+    pm.bss[pm.nb] = s_in[k];
+    pm.bds[pm.nb] = s_out[k];
+    ++pm.nb;
+    pm.nbatch *= shp[k];
+  }
+  return pm;
+}
+
+static int fast_dim(int ndim, const long long* shp, const long long* str) {
+  int f = -1;
+  for (int k = 0; k < ndim; ++k) {
+    if (shp[k] <= 1) continue;
+    const long long a = str[k] < 0 ? -str[k] : str[k];
+    if (f < 0 || a < (str[f] < 0 ? -str[f] : str[f])) f = k;
+  }
+  return f < 0 ? ndim - 1 : f;
+}
+
+static bool lines_fast(const Plan& p, const long long* shp, int ax, long long L, const void* src, const long long* sst,
+                       void* dst, const long long* dstr, hipStream_t st) {
+  if (g_tuning.nd_batch == 0) return false;
+  if (p.storage == kF64 || p.bc == kPeriodic || (p.flags & LFX_FLAG_PLAIN) || p.wavelet >= kCustomBase ||
+      p.wavelet == 0)
+    return false;
+  const long long n = shp[ax];
+  long long total = 1;
+  for (int k = 0; k < p.ndim; ++k) total *= shp[k];
+  if (n < 2 || total <= 0) return false;
+  const long long nlines = total / n;
+  if (nlines > (1ll << 24)) return false;
+  // dense workspace layout: the other dims in order, then the axis
+  std::vector<long long> ws_str(p.ndim);
+  {
+    long long acc = n;
+    ws_str[ax] = 1;
+    for (int k = p.ndim - 1; k >= 0; --k) {
+      if (k == ax) continue;
+      ws_str[k] = acc;
+      acc *= shp[k];
+    }
+  }
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  BatchPlanImpl* bp = line_plan(LinePlanKey{dev, nlines, n, L, p.wavelet, p.tf, p.bc});
+  const size_t bytes = (size_t)total * sizeof(float);
+  const size_t pad = (bytes + 255) / 256 * 256;
+  WsLease lease;
+  lease_workspace(2 * pad, st, lease);
+  float* ws1 = (float*)lease.p;
+  float* ws2 = (float*)((char*)lease.p + pad);
+  const int fi = fast_dim(p.ndim, shp, sst);
+  hip_check(launch_permute(p.storage, kF32, src, ws1, perm_map(p.ndim, shp, sst, ws_str.data(), fi, ax), st),
+            "permute kernel");
+  batch_stage_all(bp, ws1, ws2, st);
+  const int fo = fast_dim(p.ndim, shp, dstr);
+  hip_check(launch_permute(kF32, p.storage, ws2, dst, perm_map(p.ndim, shp, ws_str.data(), dstr, ax, fo), st),
+            "permute kernel");
+  return true;
 }
 
 }  // namespace lfx
@@ -1078,7 +1236,7 @@ int lfx_batch_execute(lfx_batch_plan plan, const void* in_dev, void* out_dev, vo
     BatchPlanImpl* p = reinterpret_cast<BatchPlanImpl*>(plan);
     if (!p) fail(LFX_EINVAL, "null plan");
     if (in_dev == out_dev) fail(LFX_EINVAL, "batched transforms need distinct input and output arrays");
-    for (int s = 0; s < (int)p->stages.size(); ++s) batch_stage(p, s, in_dev, out_dev, (hipStream_t)stream);
+    for (int s = 0; s < (int)p->stages.size(); ++s) batch_stage(p, s, in_dev, out_dev, (hipStream_t)stream, nullptr);
   });
 }
 
@@ -1092,7 +1250,7 @@ int lfx_batch_execute_stage(lfx_batch_plan plan, int32_t stage, const void* in_d
     BatchPlanImpl* p = reinterpret_cast<BatchPlanImpl*>(plan);
     if (!p) fail(LFX_EINVAL, "null plan");
     if (in_dev == out_dev) fail(LFX_EINVAL, "batched transforms need distinct input and output arrays");
-    batch_stage(p, stage, in_dev, out_dev, (hipStream_t)stream);
+    batch_stage(p, stage, in_dev, out_dev, (hipStream_t)stream, nullptr);
   });
 }
 

@@ -45,6 +45,19 @@ struct CopyMap {
   long long dstr[kMaxDims];
 };
 
+// Strided copy with type conversion between two boxes of the same shape where
+// a different dim is contiguous on each side: dim fi is the source's fast dim,
+// dim fo the destination's (fi == fo: a plain strided copy along fi). The other
+// dims are the batch (bshape, with their source / destination strides).
+struct PermMap {
+  int32_t nb;
+  int32_t same;  // fi == fo
+  long long ni, no;
+  long long si_i, si_o, so_i, so_o;
+  long long bshape[kMaxDims], bss[kMaxDims], bds[kMaxDims];
+  long long nbatch;
+};
+
 // LDS bytes of the whole-line kernels for a line of n samples
 inline size_t line_lds_bytes(long long n, size_t csize) { return (size_t)(n + (n - n / 2)) * csize; }
 

@@ -3,6 +3,7 @@
 // LineMap3) and their launchers.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <mutex>
 #include <set>
@@ -124,6 +125,78 @@ __global__ void k_copy_nd(const S* src, S* dst, CopyMap cm, long long total) {
     }
     dst[dof] = src[so];
   }
+}
+
+// Box permutation (see PermMap): 64 x 64 tiles through LDS, so that reads run
+// along the source's fast dim and writes along the destination's.
+__device__ __forceinline__ void perm_batch(const PermMap& pm, long long b, long long& os, long long& od) {
+  os = 0;
+  od = 0;
+  for (int k = pm.nb - 1; k >= 0; --k) {
+    const long long q = b % pm.bshape[k];
+    b /= pm.bshape[k];
+    os += q * pm.bss[k];
+    od += q * pm.bds[k];
+  }
+}
+
+template <class SI, class SO, class C>
+__global__ void __launch_bounds__(256) k_permute(const SI* __restrict__ src, SO* __restrict__ dst, PermMap pm) {
+  __shared__ C tile[64][65];
+  const long long i0 = (long long)blockIdx.x * 64, o0 = (long long)blockIdx.y * 64;
+  const int tx = (int)(threadIdx.x & 63), ty = (int)(threadIdx.x >> 6);
+  for (long long b = blockIdx.z; b < pm.nbatch; b += gridDim.z) {
+    long long os, od;
+    perm_batch(pm, b, os, od);
+#pragma unroll 4
+    for (int r = ty; r < 64; r += 4) {
+      const long long i = i0 + tx, o = o0 + r;
+      if (i < pm.ni && o < pm.no) tile[r][tx] = dev::load_as<SI, C>(src, os + i * pm.si_i + o * pm.si_o);
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int r = ty; r < 64; r += 4) {
+      const long long i = i0 + r, o = o0 + tx;
+      if (i < pm.ni && o < pm.no) dev::store_as<SO, C>(dst, od + i * pm.so_i + o * pm.so_o, tile[tx][r]);
+    }
+    __syncthreads();
+  }
+}
+
+// fi == fo: both sides contiguous along dim i
+template <class SI, class SO, class C>
+__global__ void __launch_bounds__(256) k_convert(const SI* __restrict__ src, SO* __restrict__ dst, PermMap pm) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= pm.ni) return;
+  for (long long b = blockIdx.y; b < pm.nbatch; b += gridDim.y) {
+    long long os, od;
+    perm_batch(pm, b, os, od);
+    dev::store_as<SO, C>(dst, od + i * pm.so_i, dev::load_as<SI, C>(src, os + i * pm.si_i));
+  }
+}
+
+template <class SI, class SO, class C>
+static hipError_t permute_t(const void* src, void* dst, const PermMap& pm, hipStream_t st) {
+  const unsigned z = (unsigned)std::min<long long>(pm.nbatch, 65535);
+  if (pm.same) {
+    hipLaunchKernelGGL((k_convert<SI, SO, C>), dim3((unsigned)((pm.ni + 255) / 256), z), dim3(256), 0, st,
+                       reinterpret_cast<const SI*>(src), reinterpret_cast<SO*>(dst), pm);
+  } else {
+    hipLaunchKernelGGL((k_permute<SI, SO, C>), dim3((unsigned)((pm.ni + 63) / 64), (unsigned)((pm.no + 63) / 64), z),
+                       dim3(256), 0, st, reinterpret_cast<const SI*>(src), reinterpret_cast<SO*>(dst), pm);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_permute(int sin, int sout, const void* src, void* dst, const PermMap& pm, hipStream_t st) {
+  if (pm.ni <= 0 || pm.no <= 0 || pm.nbatch <= 0) return hipSuccess;
+  if (sin == kF64 && sout == kF64) return permute_t<double, double, double>(src, dst, pm, st);
+  if (sin == kF32 && sout == kF32) return permute_t<float, float, float>(src, dst, pm, st);
+  if (sin == kBF16 && sout == kF32) return permute_t<bf16_t, float, float>(src, dst, pm, st);
+  if (sin == kF32 && sout == kBF16) return permute_t<float, bf16_t, float>(src, dst, pm, st);
+  if (sin == kF16 && sout == kF32) return permute_t<f16_t, float, float>(src, dst, pm, st);
+  if (sin == kF32 && sout == kF16) return permute_t<float, f16_t, float>(src, dst, pm, st);
+  return hipErrorInvalidValue;
 }
 
 // ---------------------------------------------------------------------------

@@ -35,5 +35,9 @@ inline hipError_t launch_reg(int tf, const void* progs, const BatchItem* items, 
 }
 hipError_t launch_copy(int storage, const void* src, void* dst, const CopyMap& cm, long long total,
                        hipStream_t st);
+// Box copy between storage types (kF32 <-> kBF16 / kF16, or same type f32 /
+// f64) where each side is contiguous along its own dim (PermMap).
+hipError_t launch_permute(int storage_in, int storage_out, const void* src, void* dst, const PermMap& pm,
+                          hipStream_t st);
 
 }  // namespace lfx
