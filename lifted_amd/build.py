@@ -1,6 +1,8 @@
 """Build lifted_amd/liblifted_hip.so for gfx950 with hipcc (in-tree).
 
-    python -m lifted_amd.build        (or lifted_amd.build.build())
+    python -m lifted_amd.build            full rebuild (what build() does by default)
+    python -m lifted_amd.build --incr     recompile only objects whose sources or
+                                          headers changed (compiler dep files)
 """
 import concurrent.futures
 import os
@@ -39,26 +41,50 @@ def _hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False):
+def _up_to_date(obj, dep):
+    """True when obj exists and is newer than every file its dep file lists."""
+    if not (os.path.exists(obj) and os.path.exists(dep)):
+        return False
+    t = os.path.getmtime(obj)
+    with open(dep) as fh:
+        text = fh.read().replace("\\\n", " ")
+    files = text.split(":", 1)[1].split() if ":" in text else []
+    if not files:
+        return False
+    for f in files:
+        if not os.path.exists(f) or os.path.getmtime(f) > t:
+            return False
+    return True
+
+
+def build(verbose=False, incremental=False):
     hipcc = _hipcc()
     os.makedirs(OBJDIR, exist_ok=True)
     objs, cmds = [], []
     for src, device in SOURCES:
-        obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
+        base = os.path.join(OBJDIR, os.path.splitext(src)[0])
+        obj, dep = base + ".o", base + ".d"
+        objs.append(obj)
+        if incremental and _up_to_date(obj, dep):
+            continue
         cmd = [hipcc, "-c"] + COMMON
         cmd += DEVICE if device else ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
+        # dependency list for incremental builds
+        cmd += ["-MMD", "-MF", dep]
         cmd += [os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd))
         cmds.append(cmd)
-        objs.append(obj)
     # compile the translation units in parallel
     jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
-    with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
-        for r in ex.map(lambda c: subprocess.run(c, capture_output=True, text=True), cmds):
-            if r.returncode != 0:
-                sys.stderr.write(r.stdout + r.stderr)
-                raise subprocess.CalledProcessError(r.returncode, r.args)
+    if cmds:
+        with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
+            for r in ex.map(lambda c: subprocess.run(c, capture_output=True, text=True), cmds):
+                if r.returncode != 0:
+                    sys.stderr.write(r.stdout + r.stderr)
+                    raise subprocess.CalledProcessError(r.returncode, r.args)
+    if incremental and not cmds and os.path.exists(OUT):
+        return OUT
     tmp = OUT + ".tmp"
     cmd = [hipcc, "-shared", "-o", tmp] + objs + [f"--offload-arch={ARCH}"]
     if verbose:
@@ -69,5 +95,5 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    build(verbose="-v" in sys.argv)
+    build(verbose="-v" in sys.argv, incremental="--incr" in sys.argv)
     print(OUT)

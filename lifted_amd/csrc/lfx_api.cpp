@@ -1020,9 +1020,14 @@ struct LinePlanKey {
 static std::map<LinePlanKey, std::pair<BatchPlanImpl*, unsigned long long>> g_line_plans;
 static unsigned long long g_line_tick = 0;
 static constexpr size_t kLinePlanCache = 32;
+// Guards the line-plan cache and each cached plan's launches (a plan's fork /
+// join events must not interleave between host threads). Distinct from g_mu:
+// batch_create takes g_mu (device programs), so holding g_mu here would
+// self-deadlock.
+static std::mutex g_line_mu;
 
+// Caller holds g_line_mu.
 static BatchPlanImpl* line_plan(const LinePlanKey& k) {
-  std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_line_plans.find(k);
   if (it != g_line_plans.end()) {
     it->second.second = ++g_line_tick;
@@ -1099,6 +1104,7 @@ static bool lines_fast(const Plan& p, const long long* shp, int ax, long long L,
   }
   int dev = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_line_mu);
   BatchPlanImpl* bp = line_plan(LinePlanKey{dev, nlines, n, L, p.wavelet, p.tf, p.bc});
   const size_t bytes = (size_t)total * sizeof(float);
   const size_t pad = (bytes + 255) / 256 * 256;

@@ -6,6 +6,7 @@ tracking, boundary folds at tile edges, multi-pass schedules -- without a GPU:
 the line-length limit is lowered so that short lines take the tiled path.
 Results must be bit-identical to the oracle (same operations, same order).
 """
+import concurrent.futures
 import itertools
 import os
 import subprocess
@@ -17,6 +18,29 @@ from helpers import MOCK_WAVELETS, WAVELETS, WIDTHS, oracle_steps
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SIM = os.path.join(HERE, "cpu_sim", "build", "lfx_sim")
+
+# The simulator is an AddressSanitizer build run once per case, so the full
+# grid (every table wavelet and mock program) takes about an hour of CPU time.
+# By default the tests cover a subset chosen to reach every step shape:
+# Daubechies1 (unit steps, sqrt2 scale), Daubechies2 (the 1-D workload),
+# Symlet5 (negative scale), Bior2_2 (repeat steps), Bior3_1 (two s-updates),
+# Bior6_8 (widest stencil, width 21) and CDF9_7; and six mock programs with
+# negative, zero and positive offsets of every step kind.
+# LFX_FULL_TESTS=1 runs everything.
+FULL = os.environ.get("LFX_FULL_TESTS") == "1"
+SIM_WAVELETS = [w for w in WAVELETS if w != 0] if FULL else [1, 2, 105, 10202, 10301, 10608, 100907]
+SIM_MOCKS = list(MOCK_WAVELETS) if FULL else [m for m in ("DUpdateNegOff", "SUpdatePosOff", "SUnitUpdateNegOff",
+                                                             "DRepeatUpdatePosOff", "MultiScale", "MultiStep")
+                                               if m in MOCK_WAVELETS]
+JOBS = min(8, os.cpu_count() or 1)
+
+
+def run_cases(cases):
+    """Run case callables on a thread pool (each spends its time in a
+    subprocess or in the oracle's C code); re-raise the first failure."""
+    with concurrent.futures.ThreadPoolExecutor(JOBS) as ex:
+        for f in [ex.submit(c) for c in cases]:
+            f.result()
 
 
 @pytest.fixture(scope="module")
@@ -46,47 +70,58 @@ def ok_bands(bc, n, L):
     return True
 
 
-@pytest.mark.parametrize("w", [w for w in WAVELETS if w != 0])
+def _case(sim, oracle, w, tf, bc, x, L, limit, tile, k, msg, ref_level=None, steps=None, spec=None, check=None):
+    """One simulator run against the oracle (bit-exact)."""
+    def run():
+        ref = oracle.lwt(x, w if steps is None else 0, tf, bc, level=[L] if ref_level is None else ref_level,
+                         steps=steps)
+        got, info = run_sim(sim, w, tf, bc, x, L, limit=limit, tile=tile, k=k, steps=spec)
+        np.testing.assert_array_equal(got, ref, err_msg=f"{msg}\n{info}")
+        if check is not None:
+            check(info)
+    return run
+
+
+@pytest.mark.parametrize("w", SIM_WAVELETS)
 def test_tiled_all_wavelets(sim, oracle, w):
     g = np.random.default_rng(w)
+    cases = []
     for bc, tf, n in itertools.product((0, 1, 3, 4), (0, 1, 2, 3), (3001, 2048)):
         L = min(7, max(1, oracle.max_level(w, n)))
         if not ok_bands(bc, n, L):
             continue
         x = g.standard_normal(n)
-        ref = oracle.lwt(x, w, tf, bc, level=[L])
-        got, info = run_sim(sim, w, tf, bc, x, L, limit=200, tile=512, k=0)
-        assert "pass" in info, info
-        np.testing.assert_array_equal(got, ref, err_msg=f"w={w} bc={bc} tf={tf} n={n}\n{info}")
+        cases.append(_case(sim, oracle, w, tf, bc, x, L, 200, 512, 0, f"w={w} bc={bc} tf={tf} n={n}",
+                           check=lambda info: None if "pass" in info else pytest.fail(info)))
+    run_cases(cases)
 
 
-@pytest.mark.parametrize("w", [w for w in WAVELETS if w != 0])
+@pytest.mark.parametrize("w", SIM_WAVELETS)
 def test_tiled_one_sample_end_tiles(sim, oracle, w):
     """Last tiles a single sample wide: a window touching the band end holds
     only the halo, so mirrored reads (Symmetric / Reflect) and adjoint folds
     must be covered by the halo rule, with plain LDS tiles and with register
     passes whose band-end pieces are cut from the tail."""
     g = np.random.default_rng(w + 3)
+    cases = []
     for bc, tf, n in itertools.product((1, 3, 4), (0, 1, 2, 3), (2561, 4097)):
         L = min(4, oracle.max_level(w, n))
         if not ok_bands(bc, n, L):
             continue
         x = g.standard_normal(n).astype(np.float32)
-        ref = oracle.lwt(x, w, tf, bc, level=[L])
         for tile, k in ((512, 0), (64, 2)):
-            got, info = run_sim(sim, w, tf, bc, x, L, limit=200, tile=tile, k=k)
-            np.testing.assert_array_equal(got, ref, err_msg=f"w={w} bc={bc} tf={tf} n={n} tile={tile}\n{info}")
+            cases.append(_case(sim, oracle, w, tf, bc, x, L, 200, tile, k, f"w={w} bc={bc} tf={tf} n={n} tile={tile}"))
+    run_cases(cases)
 
 
 @pytest.mark.parametrize("tile,k,limit", [(128, 1, 100), (256, 2, 60), (1024, 4, 300), (64, 0, 50)])
 def test_tiled_shapes_cdf97_float32(sim, oracle, tile, k, limit):
     g = np.random.default_rng(tile + k)
+    cases = []
     for bc, tf, n in itertools.product((0, 1, 3, 4), (0, 1, 2, 3), (1777, 4096, 999)):
-        L = 6
         x = g.standard_normal(n).astype(np.float32)
-        ref = oracle.lwt(x, 100907, tf, bc, level=[L])
-        got, info = run_sim(sim, 100907, tf, bc, x, L, limit=limit, tile=tile, k=k)
-        np.testing.assert_array_equal(got, ref, err_msg=f"bc={bc} tf={tf} n={n}\n{info}")
+        cases.append(_case(sim, oracle, 100907, tf, bc, x, 6, limit, tile, k, f"bc={bc} tf={tf} n={n}"))
+    run_cases(cases)
 
 
 def _spec(steps):
@@ -99,87 +134,93 @@ def _spec(steps):
     return ";".join(parts)
 
 
-@pytest.mark.parametrize("name", list(MOCK_WAVELETS))
+@pytest.mark.parametrize("name", SIM_MOCKS)
 def test_tiled_mock_wavelets(sim, oracle, name):
     """The reference's mock step programs (large positive / negative offsets
     exercise folds that land next to tile edges)."""
     st = oracle_steps(MOCK_WAVELETS[name])
     spec = _spec(MOCK_WAVELETS[name])
     g = np.random.default_rng(5)
+    cases = []
     for bc, tf, n in itertools.product((0, 1, 3, 4), (0, 1, 2, 3), (1531, 1024)):
         x = g.standard_normal(n)
-        ref = oracle.lwt(x, 0, tf, bc, level=[3], steps=st)
-        got, info = run_sim(sim, 0, tf, bc, x, 3, limit=100, tile=256, k=0, steps=spec)
-        np.testing.assert_array_equal(got, ref, err_msg=f"{name} bc={bc} tf={tf} n={n}\n{info}")
+        cases.append(_case(sim, oracle, 0, tf, bc, x, 3, 100, 256, 0, f"{name} bc={bc} tf={tf} n={n}", steps=st,
+                           spec=spec))
+    run_cases(cases)
 
 
-@pytest.mark.parametrize("name", list(MOCK_WAVELETS))
+@pytest.mark.parametrize("name", SIM_MOCKS)
 def test_whole_line_all_bcs_mock(sim, oracle, name):
     """Whole-line path (no tiling) with all five BCs, including Periodic folds
     at non-power-of-two lengths whose targets land mid-band."""
     st = oracle_steps(MOCK_WAVELETS[name])
     spec = _spec(MOCK_WAVELETS[name])
     g = np.random.default_rng(6)
+    cases = []
     for bc, tf, n in itertools.product(range(5), (0, 1, 2, 3), (153, 510, 1000, 37)):
         x = g.standard_normal(n)
-        ref = oracle.lwt(x, 0, tf, bc, level=[2], steps=st)
-        got, info = run_sim(sim, 0, tf, bc, x, 2, limit=1 << 20, tile=0, k=0, steps=spec)
-        np.testing.assert_array_equal(got, ref, err_msg=f"{name} bc={bc} tf={tf} n={n}\n{info}")
+        cases.append(_case(sim, oracle, 0, tf, bc, x, 2, 1 << 20, 0, 0, f"{name} bc={bc} tf={tf} n={n}", steps=st,
+                           spec=spec))
+    run_cases(cases)
 
 
-@pytest.mark.parametrize("w", [w for w in WAVELETS if w != 0])
+@pytest.mark.parametrize("w", SIM_WAVELETS)
 def test_whole_line_all_bcs(sim, oracle, w):
     g = np.random.default_rng(w + 1)
+    cases = []
     for bc, tf, n in itertools.product(range(5), (0, 1, 2, 3), (1000, 333)):
         L = min(4, max(1, oracle.max_level(w, n)))
         if not ok_bands(bc, n, L):
             continue
         x = g.standard_normal(n).astype(np.float32)
-        ref = oracle.lwt(x, w, tf, bc, level=[L])
-        got, info = run_sim(sim, w, tf, bc, x, L, limit=1 << 20, tile=0, k=0)
-        np.testing.assert_array_equal(got, ref, err_msg=f"w={w} bc={bc} tf={tf} n={n}")
+        cases.append(_case(sim, oracle, w, tf, bc, x, L, 1 << 20, 0, 0, f"w={w} bc={bc} tf={tf} n={n}"))
+    run_cases(cases)
 
 
-@pytest.mark.parametrize("w", [w for w in WAVELETS if w != 0])
+@pytest.mark.parametrize("w", SIM_WAVELETS)
 def test_register_tiles_all_wavelets(sim, oracle, w):
     """Passes of four levels whose interior runs as register tiles (64-lane
     model of a wavefront) and whose band ends run as LDS tiles."""
     g = np.random.default_rng(w + 7)
+    cases = []
     for bc, tf, n in itertools.product((0, 3), (0, 1, 2, 3), (30011, 24576)):
         L = min(6, oracle.max_level(w, n))
         x = g.standard_normal(n).astype(np.float32)
-        ref = oracle.lwt(x, w, tf, bc, level=[L])
-        got, info = run_sim(sim, w, tf, bc, x, L, limit=2000, tile=8192, k=4)
-        np.testing.assert_array_equal(got, ref, err_msg=f"w={w} bc={bc} tf={tf} n={n}\n{info}")
+        cases.append(_case(sim, oracle, w, tf, bc, x, L, 2000, 8192, 4, f"w={w} bc={bc} tf={tf} n={n}"))
+    run_cases(cases)
 
 
-@pytest.mark.parametrize("w", [w for w in WAVELETS if w != 0])
+@pytest.mark.parametrize("w", SIM_WAVELETS)
 def test_register_tiles_zero_boundary_whole_band(sim, oracle, w):
     """Zero boundary: register windows overlap the band ends and run masked,
     so register tiles cover whole bands, down to bands of a few hundred
     samples (one window holds the band) -- no LDS pieces at the ends."""
     g = np.random.default_rng(w + 13)
+    cases = []
     for tf, n in itertools.product((0, 1, 2, 3), (5003, 700, 1024, 333)):
         L = min(9, oracle.max_level(w, n))
         x = g.standard_normal(n).astype(np.float32)
-        ref = oracle.lwt(x, w, tf, 0, level=[L])
-        got, info = run_sim(sim, w, tf, 0, x, L, limit=64, tile=8192, k=4)
-        np.testing.assert_array_equal(got, ref, err_msg=f"w={w} tf={tf} n={n}\n{info}")
-        if n >= 1024:
-            assert "reg [0," in info, info
+
+        def whole(info, n=n):
+            if n >= 1024:
+                assert "reg [0," in info, info
+        cases.append(_case(sim, oracle, w, tf, 0, x, L, 64, 8192, 4, f"w={w} tf={tf} n={n}", check=whole))
+    run_cases(cases)
 
 
-@pytest.mark.parametrize("w", [w for w in WAVELETS if w != 0])
+@pytest.mark.parametrize("w", SIM_WAVELETS)
 def test_register_tiles_fewer_levels(sim, oracle, w):
     """Register passes of one to three levels (level counts that are not a
     multiple of four; the last pass of a 5-level transform is one level)."""
     g = np.random.default_rng(w + 11)
     n = 20011
+    cases = []
     for L, tf in itertools.product((1, 2, 3, 5), (0, 1, 2, 3)):
         if L > oracle.max_level(w, n):
             continue
         x = g.standard_normal(n).astype(np.float32)
-        ref = oracle.lwt(x, w, tf, 3, level=[L])
-        got, info = run_sim(sim, w, tf, 3, x, L, limit=600, tile=8192, k=4)
-        assert "reg [" in info, info
-        np.testing.assert_array_equal(got, ref, err_msg=f"w={w} L={L} tf={tf}\n{info}")
+
+        def has_reg(info):
+            assert "reg [" in info, info
+        cases.append(_case(sim, oracle, w, tf, 3, x, L, 600, 8192, 4, f"w={w} L={L} tf={tf}", check=has_reg))
+    run_cases(cases)
