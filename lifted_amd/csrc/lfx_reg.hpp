@@ -7,7 +7,7 @@
 // (s[e] = x[2e], d[e] = x[2e+1]); level l + 1 splits s_l the same way, so after
 // four levels each lane holds one s_4 and one d_4. A lifting step
 //     y[e] += sum_k c[k] * x[e + off + k]
-// reads x values past the lane's own chunk from neighbouring lanes (shuffles
+// reads x values past the lane's own chunk from neighbouring lanes (DPP shifts
 // by whole chunks). The step shapes (kind, band, offset, taps) are compile-time
 // (lfx_wavelet_types.hpp); the coefficients come from the runtime program.
 // The window's outer lanes compute values that depend on samples outside the
@@ -17,7 +17,7 @@
 // band next to its ends run through the LDS tile bodies.
 //
 // The lane policy P abstracts the wavefront: on the GPU a value is one float
-// per lane and cross-lane reads are shuffles; in the CPU simulation a value is
+// per lane and cross-lane reads are lane shifts; in the CPU simulation a value is
 // an array of 64 floats.
 #pragma once
 #include <cmath>
@@ -41,13 +41,14 @@ struct LaneCpu {
   struct V {
     float v[kLanes];
   };
-  // value of x held by lane (lane + q); lanes outside the wavefront keep their
-  // own value, like the GPU shuffles
-  static V from(const V& x, int q) {
+  // value of x held by lane (lane + Q), the source lane clamped to the
+  // wavefront (as the GPU's chained one-lane shifts give)
+  template <int Q>
+  static V from(const V& x) {
     V r;
     for (int l = 0; l < kLanes; ++l) {
-      const int src = l + q;
-      r.v[l] = (src >= 0 && src < kLanes) ? x.v[src] : x.v[l];
+      const int src = l + Q < 0 ? 0 : (l + Q >= kLanes ? kLanes - 1 : l + Q);
+      r.v[l] = x.v[src];
     }
     return r;
   }
@@ -104,14 +105,31 @@ struct LaneCpu {
     for (int l = 0; l < kLanes; ++l) r.v[l] = pred(l) ? dev::load_as<S, float>(p, idx(l)) : 0.0f;
     return r;
   }
-  // x[e] of lane l = p[pos0 + E l + e] where that position lies in [0, n), else 0
+  // x[e] of lane l = p[pos0 + E l + e] where that position lies in [0, n), else 0.
+  // The model has no coalescing order: ld_raw yields the chunk values and
+  // xpose_in copies them.
   template <int E, class S>
-  static void ld_chunks(const S* p, long long pos0, long long n, V (&x)[E], float*) {
+  static void ld_raw(const S* p, long long pos0, long long n, V (&t)[E]) {
     for (int e = 0; e < E; ++e)
       for (int l = 0; l < kLanes; ++l) {
         const long long g = pos0 + (long long)E * l + e;
-        x[e].v[l] = (g >= 0 && g < n) ? dev::load_as<S, float>(p, g) : 0.0f;
+        t[e].v[l] = (g >= 0 && g < n) ? dev::load_as<S, float>(p, g) : 0.0f;
       }
+  }
+  template <int E>
+  static void xpose_in(const V (&t)[E], V (&x)[E], float*) {
+    for (int e = 0; e < E; ++e) x[e] = t[e];
+  }
+  template <int E, class S>
+  static void ld_chunks(const S* p, long long pos0, long long n, V (&x)[E], float* lds) {
+    V t[E];
+    ld_raw<E>(p, pos0, n, t);
+    xpose_in<E>(t, x, lds);
+  }
+  // the N step coefficients of a program, read once per tile
+  template <int N>
+  static void coefs(const float* p, float* c) {
+    for (int i = 0; i < N; ++i) c[i] = p[i];
   }
   template <int E, class S>
   static void st_chunks(S* p, long long pos0, const V (&v)[E], long long g0, long long lo, long long hi, float*) {
@@ -126,9 +144,32 @@ struct LaneCpu {
 struct LaneGpu {
   using V = float;
   static __device__ __forceinline__ int lane() { return (int)(threadIdx.x & (kLanes - 1)); }
-  // value of v held by lane (lane + q)
-  static __device__ __forceinline__ V from(V v, int q) {
-    return q > 0 ? __shfl_down(v, (unsigned)q, kLanes) : __shfl_up(v, (unsigned)(-q), kLanes);
+  // One-lane whole-wave shifts (DPP wave_shl:1 / wave_shr:1): lane l gets the
+  // value of lane l + 1 (l - 1); the lane with no source keeps its own value.
+  static __device__ __forceinline__ V shl1(V v) {
+    const int b = __builtin_bit_cast(int, v);
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, 0x130, 0xF, 0xF, false));
+  }
+  static __device__ __forceinline__ V shr1(V v) {
+    const int b = __builtin_bit_cast(int, v);
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, 0x138, 0xF, 0xF, false));
+  }
+  // value of v held by lane (lane + Q), the source lane clamped to the
+  // wavefront: chains of one-lane DPP shifts for |Q| <= 3 (a few cycles each,
+  // no LDS round trip), a clamped ds_bpermute beyond
+  template <int Q>
+  static __device__ __forceinline__ V from(V v) {
+    if constexpr (Q == 0) {
+      return v;
+    } else if constexpr (Q > 0 && Q <= 3) {
+      return from<Q - 1>(shl1(v));
+    } else if constexpr (Q < 0 && Q >= -3) {
+      return from<Q + 1>(shr1(v));
+    } else {
+      int src = lane() + Q;
+      src = src < 0 ? 0 : (src >= kLanes ? kLanes - 1 : src);
+      return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src << 2, __builtin_bit_cast(int, v)));
+    }
   }
   static __device__ __forceinline__ V fma(float c, V x, V acc) { return __builtin_fmaf(c, x, acc); }
   static __device__ __forceinline__ V add(V a, V b) { return a + b; }
@@ -166,10 +207,12 @@ struct LaneGpu {
   // (lane l reads pos0 + 64 j + l), transposed through the wave's LDS buffer
   // (row stride E + 1 keeps the chunk reads conflict-free). The bounds checks
   // run only for windows that overlap a band end (a wave-uniform test).
+  // Split in two so that a tile can issue all its global loads first and
+  // transpose each band when a level needs it: ld_raw leaves t[j] =
+  // p[pos0 + 64 j + lane], xpose_in moves it to the chunk layout.
   template <int E, class S>
-  static __device__ __forceinline__ void ld_chunks(const S* p, long long pos0, long long n, V (&x)[E], float* lds) {
+  static __device__ __forceinline__ void ld_raw(const S* p, long long pos0, long long n, V (&t)[E]) {
     const int l = lane();
-    float t[E];
     if (pos0 >= 0 && pos0 + E * kLanes <= n) {
       const S* q = p + pos0;
 #pragma unroll
@@ -181,6 +224,10 @@ struct LaneGpu {
         t[j] = (g >= 0 && g < n) ? dev::load_as<S, float>(p, g) : 0.0f;
       }
     }
+  }
+  template <int E>
+  static __device__ __forceinline__ void xpose_in(const V (&t)[E], V (&x)[E], float* lds) {
+    const int l = lane();
 #pragma unroll
     for (int j = 0; j < E; ++j) {
       const int u = j * kLanes + l;
@@ -190,6 +237,23 @@ struct LaneGpu {
 #pragma unroll
     for (int e = 0; e < E; ++e) x[e] = lds[l * (E + 1) + e];
     wave_sync();
+  }
+  template <int E, class S>
+  static __device__ __forceinline__ void ld_chunks(const S* p, long long pos0, long long n, V (&x)[E], float* lds) {
+    V t[E];
+    ld_raw<E>(p, pos0, n, t);
+    xpose_in<E>(t, x, lds);
+  }
+  // The N step coefficients of a program, read once per tile through the
+  // constant address space: the address is wave-uniform, so these become
+  // scalar loads into SGPRs, issued before any global store (a vector load
+  // after a store would wait for the store on the shared memory counter).
+  template <int N>
+  static __device__ __forceinline__ void coefs(const float* p, float* c) {
+    typedef const __attribute__((address_space(4))) float* cptr;
+    const cptr q = (cptr)p;
+#pragma unroll
+    for (int i = 0; i < N; ++i) c[i] = q[i];
   }
   // p[pos0 + g] = v for the owned globals g = g0 + E * lane + e in [lo, hi),
   // transposed through LDS so that the global stores are coalesced.
@@ -236,7 +300,7 @@ __device__ __forceinline__ typename P::V xat(const typename P::V (&x)[E]) {
   constexpr int q = fdiv<J, E>();
   constexpr int r = J - q * E;
   if constexpr (q == 0) return x[r];
-  else return P::from(x[r], q);
+  else return P::template from<q>(x[r]);
 }
 
 // One lifting step on E elements per lane: y[e] (+)= stencil of x.
@@ -270,15 +334,15 @@ __device__ __forceinline__ void rstep(typename P::V (&y)[E], const typename P::V
 // With masks (windows overlapping a band end, Zero boundary), the updated band
 // is zeroed outside its range after every step: the next steps then read exact
 // zeros there, which is what skipping out-of-range terms computes.
+// cf: the program's coefficients (wt::NCoef of them), held in registers.
 template <class P, class W, int TF, int E>
-__device__ __forceinline__ void level(typename P::V (&s)[E], typename P::V (&d)[E],
-                                      const DProgram<float>* __restrict__ prog, bool masked,
-                                      const typename P::M& ms, const typename P::M& md) {
+__device__ __forceinline__ void level(typename P::V (&s)[E], typename P::V (&d)[E], const float* cf_all,
+                                      bool masked, const typename P::M& ms, const typename P::M& md) {
   using L = typename W::list;
   sfor<L::size>([&](auto I) {
     constexpr int i = decltype(I)::value;
     using ST = wt::Exec<L, TF, i>;
-    const float* cf = prog->coef + prog->steps[i].coef;
+    const float* cf = cf_all + wt::CoefOff<L, TF, i>::value;
     if constexpr (ST::kind == kScale) {
       const float sm = cf[0], dm = cf[1];
       sfor<E>([&](auto e) {
@@ -339,6 +403,9 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long 
   using M = typename P::M;
   const long long w0 = a - H;  // multiple of 16
   const bool masked = w0 < 0 || w0 + kWindow > m0;
+  constexpr int NC = wt::NCoef<typename W::list, TF>::value;
+  float cf[NC > 0 ? NC : 1];
+  P::template coefs<NC>(prog->coef, cf);
   V x[16];
   P::template ld_chunks<16>(in, w0, m0, x, lds);
   V s1[8], d1[8], s2[4], d2[4], s3[2], d3[2], s4[1], d4[1];
@@ -352,7 +419,7 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long 
     ms = P::band_mask(8, w0 / 2, ns);
     md = P::band_mask(8, w0 / 2, nd);
   }
-  level<P, W, TF, 8>(s1, d1, prog, masked, ms, md);
+  level<P, W, TF, 8>(s1, d1, cf, masked, ms, md);
   P::template st_chunks<8>(out, ns, d1, w0 / 2, a >> 1, hi_d(1), lds);
   if (K == 1) {
     P::template st_chunks<8>(sout, 0, s1, w0 / 2, a >> 1, hi_s(1), lds);
@@ -366,7 +433,7 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long 
     ms = P::band_mask(4, w0 / 4, ns);
     md = P::band_mask(4, w0 / 4, nd);
   }
-  level<P, W, TF, 4>(s2, d2, prog, masked, ms, md);
+  level<P, W, TF, 4>(s2, d2, cf, masked, ms, md);
   P::template st_chunks<4>(out, ns, d2, w0 / 4, a >> 2, hi_d(2), lds);
   if (K == 2) {
     P::template st_chunks<4>(sout, 0, s2, w0 / 4, a >> 2, hi_s(2), lds);
@@ -380,7 +447,7 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long 
     ms = P::band_mask(2, w0 / 8, ns);
     md = P::band_mask(2, w0 / 8, nd);
   }
-  level<P, W, TF, 2>(s3, d3, prog, masked, ms, md);
+  level<P, W, TF, 2>(s3, d3, cf, masked, ms, md);
   P::template st_chunks<2>(out, ns, d3, w0 / 8, a >> 3, hi_d(3), lds);
   if (K == 3) {
     P::template st_chunks<2>(sout, 0, s3, w0 / 8, a >> 3, hi_s(3), lds);
@@ -394,7 +461,7 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long 
     ms = P::band_mask(1, w0 / 16, ns);
     md = P::band_mask(1, w0 / 16, nd);
   }
-  level<P, W, TF, 1>(s4, d4, prog, masked, ms, md);
+  level<P, W, TF, 1>(s4, d4, cf, masked, ms, md);
   store_band<P, 1>(out, ns, d4, w0 / 16, a >> 4, hi_d(4));
   store_band<P, 1>(sout, 0, s4, w0 / 16, a >> 4, hi_s(4));
 }
@@ -404,60 +471,92 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long 
 // counted in level-4 units whatever K is: with g0 = (a >> 4) - Hk, the level-l
 // windows are [2^(4-l) g0, 2^(4-l) (g0 + 64)). Reads s_K from sin[i] and d_l
 // from in[ns_l + i]; writes the owned band values to out[q]. Windows that
-// overlap a band end (Zero boundary only) run masked.
-template <class P, class W, int TF, class SI, class SK, class SO>
-__device__ __forceinline__ void rev_tile(const DProgram<float>* prog, long long m0, int K, long long a, long long b,
-                                         long long Hk, const SI* in, const SK* sin, SO* out, float* lds) {
+// overlap a band end (Zero boundary only) run masked. All global loads of the
+// tile are issued first (the coarse levels then compute while the fine bands
+// are in flight); each band is transposed to the chunk layout when its level
+// runs.
+template <class P, class W, int TF, int K, class SI, class SK, class SO>
+__device__ __forceinline__ void rev_tile_k(const DProgram<float>* prog, long long m0, long long a, long long b,
+                                           long long Hk, const SI* in, const SK* sin, SO* out, float* lds) {
   using V = typename P::V;
   using M = typename P::M;
   const long long g0 = (a >> 4) - Hk;
   const bool masked = g0 < 0 || 16 * (g0 + 64) > m0;
   const long long m1 = m0, m2 = m1 - m1 / 2, m3 = m2 - m2 / 2, m4 = m3 - m3 / 2;  // band split by level l
   const long long ns1 = m1 - m1 / 2, ns2 = m2 - m2 / 2, ns3 = m3 - m3 / 2, ns4 = m4 - m4 / 2;
-  V s4[1], d4[1], s3[2], d3[2], s2[4], d2[4], s1[8], d1[8], x[16];
+  constexpr int NC = wt::NCoef<typename W::list, TF>::value;
+  float cf[NC > 0 ? NC : 1];
+  P::template coefs<NC>(prog->coef, cf);
+  // raw (coalesced-order) loads: d_1 .. d_K and s_K
+  constexpr int EK = 1 << (4 - K);  // elements per lane of the level-K bands
+  V rs[EK], r1[8], r2[4], r3[2], r4[1];
+  if constexpr (K == 4) {
+    rs[0] = P::ld(sin, [&](int l) { return g0 + l; }, [&](int l) { return g0 + l >= 0 && g0 + l < ns4; });
+    r4[0] = P::ld(in + ns4, [&](int l) { return g0 + l; }, [&](int l) { return g0 + l >= 0 && g0 + l < m4 / 2; });
+  } else if constexpr (K == 3) {
+    P::template ld_raw<2>(sin, 2 * g0, ns3, rs);
+  } else if constexpr (K == 2) {
+    P::template ld_raw<4>(sin, 4 * g0, ns2, rs);
+  } else {
+    P::template ld_raw<8>(sin, 8 * g0, ns1, rs);
+  }
+  if constexpr (K >= 3) P::template ld_raw<2>(in + ns3, 2 * g0, m3 / 2, r3);
+  if constexpr (K >= 2) P::template ld_raw<4>(in + ns2, 4 * g0, m2 / 2, r2);
+  P::template ld_raw<8>(in + ns1, 8 * g0, m1 / 2, r1);
+  V s3[2], d3[2], s2[4], d2[4], s1[8], d1[8], x[16];
   M ms{}, md{};
-  if (K == 4) {
-    s4[0] = P::ld(sin, [&](int l) { return g0 + l; }, [&](int l) { return g0 + l >= 0 && g0 + l < ns4; });
-    d4[0] = P::ld(in + ns4, [&](int l) { return g0 + l; }, [&](int l) { return g0 + l >= 0 && g0 + l < m4 / 2; });
+  if constexpr (K == 4) {
+    V s4[1] = {rs[0]}, d4[1] = {r4[0]};
     if (masked) {
       ms = P::band_mask(1, g0, ns4);
       md = P::band_mask(1, g0, m4 / 2);
     }
-    level<P, W, TF, 1>(s4, d4, prog, masked, ms, md);
+    level<P, W, TF, 1>(s4, d4, cf, masked, ms, md);
     merge<P, 1>(s4, d4, s3);
-  } else if (K == 3) {
-    P::template ld_chunks<2>(sin, 2 * g0, ns3, s3, lds);
+  } else if constexpr (K == 3) {
+    P::template xpose_in<2>(rs, s3, lds);
   }
-  if (K >= 3) {
-    P::template ld_chunks<2>(in + ns3, 2 * g0, m3 / 2, d3, lds);
+  if constexpr (K >= 3) {
+    P::template xpose_in<2>(r3, d3, lds);
     if (masked) {
       ms = P::band_mask(2, 2 * g0, ns3);
       md = P::band_mask(2, 2 * g0, m3 / 2);
     }
-    level<P, W, TF, 2>(s3, d3, prog, masked, ms, md);
+    level<P, W, TF, 2>(s3, d3, cf, masked, ms, md);
     merge<P, 2>(s3, d3, s2);
-  } else if (K == 2) {
-    P::template ld_chunks<4>(sin, 4 * g0, ns2, s2, lds);
+  } else if constexpr (K == 2) {
+    P::template xpose_in<4>(rs, s2, lds);
   }
-  if (K >= 2) {
-    P::template ld_chunks<4>(in + ns2, 4 * g0, m2 / 2, d2, lds);
+  if constexpr (K >= 2) {
+    P::template xpose_in<4>(r2, d2, lds);
     if (masked) {
       ms = P::band_mask(4, 4 * g0, ns2);
       md = P::band_mask(4, 4 * g0, m2 / 2);
     }
-    level<P, W, TF, 4>(s2, d2, prog, masked, ms, md);
+    level<P, W, TF, 4>(s2, d2, cf, masked, ms, md);
     merge<P, 4>(s2, d2, s1);
   } else {
-    P::template ld_chunks<8>(sin, 8 * g0, ns1, s1, lds);
+    P::template xpose_in<8>(rs, s1, lds);
   }
-  P::template ld_chunks<8>(in + ns1, 8 * g0, m1 / 2, d1, lds);
+  P::template xpose_in<8>(r1, d1, lds);
   if (masked) {
     ms = P::band_mask(8, 8 * g0, ns1);
     md = P::band_mask(8, 8 * g0, m1 / 2);
   }
-  level<P, W, TF, 8>(s1, d1, prog, masked, ms, md);
+  level<P, W, TF, 8>(s1, d1, cf, masked, ms, md);
   merge<P, 8>(s1, d1, x);
   P::template st_chunks<16>(out, 0, x, 16 * g0, a, b >= m0 ? m0 : b, lds);
+}
+
+template <class P, class W, int TF, class SI, class SK, class SO>
+__device__ __forceinline__ void rev_tile(const DProgram<float>* prog, long long m0, int K, long long a, long long b,
+                                         long long Hk, const SI* in, const SK* sin, SO* out, float* lds) {
+  switch (K) {
+    case 4: rev_tile_k<P, W, TF, 4>(prog, m0, a, b, Hk, in, sin, out, lds); return;
+    case 3: rev_tile_k<P, W, TF, 3>(prog, m0, a, b, Hk, in, sin, out, lds); return;
+    case 2: rev_tile_k<P, W, TF, 2>(prog, m0, a, b, Hk, in, sin, out, lds); return;
+    default: rev_tile_k<P, W, TF, 1>(prog, m0, a, b, Hk, in, sin, out, lds); return;
+  }
 }
 
 }  // namespace reg

@@ -63,6 +63,24 @@ struct Exec {
   static constexpr bool sub = D::kind == kUnit ? ((D::sign > 0) != fwd_tf) : false;
 };
 
+// Index of step I's first coefficient in the resolved program (the runtime
+// resolution packs them in execution order: scale 2, generic N, repeat 1,
+// unit none), and the program's coefficient count.
+template <class L, int TF, int I>
+struct CoefOff {
+  using P = Exec<L, TF, I - 1>;
+  static constexpr int value =
+      CoefOff<L, TF, I - 1>::value + (P::kind == kScale ? 2 : (P::kind == kGeneric ? P::n : (P::kind == kRepeat ? 1 : 0)));
+};
+template <class L, int TF>
+struct CoefOff<L, TF, 0> {
+  static constexpr int value = 0;
+};
+template <class L, int TF>
+struct NCoef {
+  static constexpr int value = CoefOff<L, TF, L::size>::value;
+};
+
 #define LFX_NARGS_(a1, a2, a3, a4, a5, a6, a7, a8, a9, a10, a11, a12, a13, a14, a15, a16, n, ...) n
 #define LFX_NARGS(...) LFX_NARGS_(__VA_ARGS__, 16, 15, 14, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2, 1, 0)
 
