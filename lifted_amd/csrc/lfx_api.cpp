@@ -658,6 +658,14 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
   std::vector<DProgram<float>> reg_progs;  // program table of register items
   std::map<uint32_t, int> reg_pidx;
   long long scr_elems = 0;
+  // Level groups of every signal first. Forward plans put each signal's
+  // trailing whole-line groups in the plan's last stages (aligned on the
+  // deepest signal), so that the latency-bound LDS items of all signals share
+  // one launch instead of trailing every stage; a signal's groups keep their
+  // order, so its dependencies are unchanged.
+  std::vector<std::vector<Group>> groups(nsig);
+  std::vector<long long> levels(nsig, 0);
+  int Gmax = 0;
   for (int i = 0; i < nsig; ++i) {
     const lfx_signal& sg = sigs[i];
     if (sg.length < 0 || sg.offset < 0 || sg.levels < 0) fail(LFX_EINVAL, "bad signal " + std::to_string(i));
@@ -668,15 +676,23 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
       if (L < 0) fail(LFX_EINVAL, "level 0 (max level) is undefined for a width-1 wavelet");
     }
     check_bands(bc, has_lift_steps(w), sg.length, L);
+    if (L == 0 && sg.length > 0) fail(LFX_EINVAL, "batched signals need at least one level");
+    levels[i] = L;
+    if (sg.length == 0 || L == 0) continue;
+    const bool reg = storage == kF32 && !plain && sg.wavelet < kCustomBase;
+    groups[i] = cs == 8 ? signal_groups<double>(w, tf, storage, bc, sg.length, L, plain, reg)
+                        : signal_groups<float>(w, tf, storage, bc, sg.length, L, plain, reg);
+    Gmax = std::max(Gmax, (int)groups[i].size());
+  }
+  for (int i = 0; i < nsig; ++i) {
+    const lfx_signal& sg = sigs[i];
+    const WaveletDefL w = find_wavelet(sg.wavelet);
+    const long long L = levels[i];
     plan->total += sg.length;
-    if (sg.length == 0 || L == 0) {
-      if (L == 0) fail(LFX_EINVAL, "batched signals need at least one level");
-      continue;
-    }
+    if (sg.length == 0 || L == 0) continue;
     const void* prog = device_program(sg.wavelet, tf, storage);
     const bool reg = storage == kF32 && !plain && sg.wavelet < kCustomBase;
-    std::vector<Group> gs = cs == 8 ? signal_groups<double>(w, tf, storage, bc, sg.length, L, plain, reg)
-                                    : signal_groups<float>(w, tf, storage, bc, sg.length, L, plain, reg);
+    const std::vector<Group>& gs = groups[i];
     const DProgram<float> hpf = resolve<float>(w, tf);
     const DProgram<double> hpd = resolve<double>(w, tf);
     auto items_of = [&](const Group& gr) {
@@ -773,8 +789,8 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
           it.arr[0] = in_arr; it.off[0] = in_off;
           it.arr[1] = kArrOut; it.off[1] = sg.offset;
           it.arr[2] = in_arr; it.off[2] = in_off;
-          add_item(g, it, (double)gr.m0, line_lds_bytes(gr.m0, cs), gr.m0 * (arr_bytes(in_arr) + (double)es),
-                   sg.wavelet);
+          add_item(g + (Gmax - G), it, (double)gr.m0, line_lds_bytes(gr.m0, cs),
+                   gr.m0 * (arr_bytes(in_arr) + (double)es), sg.wavelet);
         }
         in_arr = s_arr;
         in_off = s_off;

@@ -333,3 +333,32 @@ def test_planted_mistakes_are_detected(oracle):
     # columns of even samples i -> their effect on d
     block = W[n:, 0::2]
     np.testing.assert_allclose(block, good, atol=1e-15)
+
+
+# ---------------- Periodic boundary quirk ----------------
+
+def test_periodic_unsigned_modulo_quirk(oracle):
+    """Periodic maps a negative index i to (size_t)i % n (src/lifted-common.hpp:745-749):
+    (2^64 + i) mod n, which equals the true wrap n + i only when n divides 2^64
+    (a power of two). One step d[i] += c * s[BC(i - 1)] on bands of 4..9
+    samples: the index each d[0] reads is fixed by arithmetic on 2^64."""
+    c = 0.5
+    steps = oracle_steps([{"kind": "generic", "target": "d", "offset": -1, "taps": [c]}])
+    for n in range(8, 20):
+        x = fill_sin(n, -3.0, 5.0)
+        s, d = x[0::2].copy(), x[1::2].copy()
+        ns = len(s)
+        want = d.copy()
+        for i in range(len(d)):
+            j = i - 1
+            k = (j + 2 ** 64) % ns if j < 0 else j % ns
+            want[i] = d[i] + c * s[k]
+        got = oracle.lwt(x, 0, F, oracle.PERIODIC, level=1, steps=steps)
+        np.testing.assert_array_equal(got[:ns], s)
+        np.testing.assert_allclose(got[ns:], want, rtol=0, atol=1e-15)  # one fma vs mul+add: <= 1 ulp
+        # the true wrap differs exactly when ns is not a power of two
+        true_wrap = np.array([d[i] + c * s[(i - 1) % ns] for i in range(len(d))])
+        differs = not np.allclose(got[ns:], true_wrap, rtol=0, atol=1e-12)
+        assert differs == ((2 ** 64 - 1) % ns != ns - 1), (n, ns)
+        if ns & (ns - 1) == 0:
+            assert not differs, (n, ns)
