@@ -25,6 +25,13 @@ constexpr int kMaxDims = 8;
 // most kMaxLdsBytes
 constexpr size_t kProgLdsBytes = 3072;
 constexpr size_t kMaxLdsBytes = 160 * 1024 - kProgLdsBytes;
+// Register tiles: one wavefront holds a window of kRegWindow samples,
+// kRegPerLane consecutive ones per lane, and runs up to kRegLevels levels;
+// reverse tiles count their window in level-kRegLevels units (kRegUnits).
+constexpr int kRegLevels = 4;
+constexpr int kRegPerLane = 32;
+constexpr long long kRegWindow = 64ll * kRegPerLane;
+constexpr long long kRegUnits = kRegWindow >> kRegLevels;
 
 // Base offsets of a line in three arrays: the line index is decomposed over
 // the non-transformed dims (last fastest); str[a][k] is array a's stride of

@@ -24,6 +24,7 @@
 #include <cstdint>
 #include <utility>
 
+#include "lfx_common.hpp"
 #include "lfx_device.hpp"
 #include "lfx_wavelet_types.hpp"
 
@@ -31,12 +32,18 @@ namespace lfx {
 namespace reg {
 
 constexpr int kLanes = 64;
-constexpr int kLevels = 4;              // levels per register tile
-constexpr int kPerLane = 1 << kLevels;  // samples per lane at the finest level
-constexpr int kWindow = kLanes * kPerLane;
+constexpr int kLevels = kRegLevels;           // levels per register tile
+constexpr int kPerLane = kRegPerLane;         // samples per lane at the finest level
+constexpr int kWindow = (int)kRegWindow;      // kLanes * kPerLane samples
+constexpr int kUnits = (int)kRegUnits;        // the window in level-4 units
+static_assert(kLevels == 4 && kWindow == kLanes * kPerLane && (kPerLane == 16 || kPerLane == 32),
+              "register tiles run four levels on 16 or 32 samples per lane");
 
 #ifdef LFX_CPU_SIM
-// CPU simulation: a value is the 64 lanes of the wavefront.
+// CPU simulation: a value is the 64 lanes of the wavefront. The elementwise
+// operations stay out of line: inlining a 64-lane loop into every unrolled
+// step makes the simulation too large to compile in reasonable time.
+#define LFX_SIM_NOINLINE __attribute__((noinline))
 struct LaneCpu {
   struct V {
     float v[kLanes];
@@ -44,7 +51,7 @@ struct LaneCpu {
   // value of x held by lane (lane + Q), the source lane clamped to the
   // wavefront (as the GPU's chained one-lane shifts give)
   template <int Q>
-  static V from(const V& x) {
+  LFX_SIM_NOINLINE static V from(const V& x) {
     V r;
     for (int l = 0; l < kLanes; ++l) {
       const int src = l + Q < 0 ? 0 : (l + Q >= kLanes ? kLanes - 1 : l + Q);
@@ -52,19 +59,19 @@ struct LaneCpu {
     }
     return r;
   }
-  static V fma(float c, V x, V acc) {
+  LFX_SIM_NOINLINE static V fma(float c, V x, V acc) {
     for (int l = 0; l < kLanes; ++l) acc.v[l] = std::fma(c, x.v[l], acc.v[l]);
     return acc;
   }
-  static V add(V a, const V& b) {
+  LFX_SIM_NOINLINE static V add(V a, const V& b) {
     for (int l = 0; l < kLanes; ++l) a.v[l] = a.v[l] + b.v[l];
     return a;
   }
-  static V sub(V a, const V& b) {
+  LFX_SIM_NOINLINE static V sub(V a, const V& b) {
     for (int l = 0; l < kLanes; ++l) a.v[l] = a.v[l] - b.v[l];
     return a;
   }
-  static V mul(V a, float c) {
+  LFX_SIM_NOINLINE static V mul(V a, float c) {
     for (int l = 0; l < kLanes; ++l) a.v[l] = a.v[l] * c;
     return a;
   }
@@ -94,7 +101,7 @@ struct LaneCpu {
     }
     return r;
   }
-  static V keep(const M& m, int e, V v) {
+  LFX_SIM_NOINLINE static V keep(const M& m, int e, V v) {
     for (int l = 0; l < kLanes; ++l)
       if (!((m.m[l] >> e) & 1u)) v.v[l] = 0.0f;
     return v;
@@ -225,18 +232,38 @@ struct LaneGpu {
       }
     }
   }
+  // Up to 16 values per lane go through the buffer at once (64 rows of E + 1
+  // floats); 32 per lane go in two rounds of 1024 values, round h filling
+  // the chunks of lanes [32 h, 32 h + 32) (32 rows of 33 floats), so the
+  // buffer never exceeds 64 x 17 floats.
   template <int E>
   static __device__ __forceinline__ void xpose_in(const V (&t)[E], V (&x)[E], float* lds) {
     const int l = lane();
+    if constexpr (E <= 16) {
 #pragma unroll
-    for (int j = 0; j < E; ++j) {
-      const int u = j * kLanes + l;
-      lds[(u / E) * (E + 1) + (u % E)] = t[j];
+      for (int j = 0; j < E; ++j) {
+        const int u = j * kLanes + l;
+        lds[(u / E) * (E + 1) + (u % E)] = t[j];
+      }
+      wave_sync();
+#pragma unroll
+      for (int e = 0; e < E; ++e) x[e] = lds[l * (E + 1) + e];
+      wave_sync();
+    } else {
+      static_assert(E == 32, "register chunks hold at most 32 samples per lane");
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        // value j of round h: u = (16 h + j) 64 + l -> lane u / 32 = 32 h + 2 j + l / 32, slot l % 32
+#pragma unroll
+        for (int j = 0; j < 16; ++j) lds[(2 * j + (l >> 5)) * 33 + (l & 31)] = t[16 * h + j];
+        wave_sync();
+        if ((l >> 5) == h) {
+#pragma unroll
+          for (int e = 0; e < 32; ++e) x[e] = lds[(l & 31) * 33 + e];
+        }
+        wave_sync();
+      }
     }
-    wave_sync();
-#pragma unroll
-    for (int e = 0; e < E; ++e) x[e] = lds[l * (E + 1) + e];
-    wave_sync();
   }
   template <int E, class S>
   static __device__ __forceinline__ void ld_chunks(const S* p, long long pos0, long long n, V (&x)[E], float* lds) {
@@ -261,20 +288,40 @@ struct LaneGpu {
   static __device__ __forceinline__ void st_chunks(S* p, long long pos0, const V (&v)[E], long long g0, long long lo,
                                                    long long hi, float* lds) {
     const int l = lane();
-#pragma unroll
-    for (int e = 0; e < E; ++e) lds[l * (E + 1) + e] = v[e];
-    wave_sync();
     // owned window-relative range [t0, t1) of the E * 64 values
     const int t0 = (int)(lo - g0 < 0 ? 0 : lo - g0);
     const int t1 = (int)(hi - g0 > E * kLanes ? E * kLanes : hi - g0);
     S* q = p + pos0 + g0;
+    if constexpr (E <= 16) {
 #pragma unroll
-    for (int j = 0; j < E; ++j) {
-      const int u = j * kLanes + l;
-      const float val = lds[(u / E) * (E + 1) + (u % E)];
-      if (u >= t0 && u < t1) dev::store_as<S, float>(q, u, val);
+      for (int e = 0; e < E; ++e) lds[l * (E + 1) + e] = v[e];
+      wave_sync();
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        const int u = j * kLanes + l;
+        const float val = lds[(u / E) * (E + 1) + (u % E)];
+        if (u >= t0 && u < t1) dev::store_as<S, float>(q, u, val);
+      }
+      wave_sync();
+    } else {
+      static_assert(E == 32, "register chunks hold at most 32 samples per lane");
+      // two rounds of 1024 values (see xpose_in)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if ((l >> 5) == h) {
+#pragma unroll
+          for (int e = 0; e < 32; ++e) lds[(l & 31) * 33 + e] = v[e];
+        }
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const int u = (16 * h + j) * kLanes + l;
+          const float val = lds[(2 * j + (l >> 5)) * 33 + (l & 31)];
+          if (u >= t0 && u < t1) dev::store_as<S, float>(q, u, val);
+        }
+        wave_sync();
+      }
     }
-    wave_sync();
   }
 };
 #endif
@@ -392,160 +439,181 @@ __device__ __forceinline__ void store_band(S* p, long long pos0, const typename 
 
 // Forward tile (Forward / InverseAdjoint) of K (1..4) levels: owned [a, b)
 // of the input band of length m0 (b == m0 for the last tile), window
-// [w0, w0 + 1024) with w0 = a - H. Writes d_l of the owned range to
-// out[ns_l + i] (l = 1..K) and s_K to sout[i]. A window inside the band reads
-// no boundary; one that overlaps a band end (Zero boundary only) runs masked.
-// K is uniform over the wavefront.
+// [w0, w0 + kWindow) with w0 = a - H (a multiple of 16). Lane l holds the
+// kPerLane samples w0 + kPerLane l + e; after level l it holds kPerLane / 2^l
+// values of s_l and of d_l. Writes d_l of the owned range to out[ns_l + i]
+// (l = 1..K) and s_K to sout[i]. A window inside the band reads no boundary;
+// one that overlaps a band end (Zero boundary only) runs masked. K is uniform
+// over the wavefront.
 template <class P, class W, int TF, class SI, class SO, class SK>
 __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long m0, int K, long long a, long long b,
                                          long long H, const SI* in, SO* out, SK* sout, float* lds) {
   using V = typename P::V;
   using M = typename P::M;
-  const long long w0 = a - H;  // multiple of 16
+  constexpr int E1 = kPerLane / 2, E2 = kPerLane / 4, E3 = kPerLane / 8, E4 = kPerLane / 16;
+  const long long w0 = a - H;
   const bool masked = w0 < 0 || w0 + kWindow > m0;
   constexpr int NC = wt::NCoef<typename W::list, TF>::value;
   float cf[NC > 0 ? NC : 1];
   P::template coefs<NC>(prog->coef, cf);
-  V x[16];
-  P::template ld_chunks<16>(in, w0, m0, x, lds);
-  V s1[8], d1[8], s2[4], d2[4], s3[2], d3[2], s4[1], d4[1];
+  V x[kPerLane];
+  P::template ld_chunks<kPerLane>(in, w0, m0, x, lds);
+  V s1[E1], d1[E1], s2[E2], d2[E2], s3[E3], d3[E3], s4[E4], d4[E4];
   M ms{}, md{};
   long long m = m0, ns = m - m / 2, nd = m / 2;
   // owned end of a band after l levels: the band end for the last tile
   auto hi_d = [&](int l) { return b >= m0 ? nd : (b >> l); };
   auto hi_s = [&](int l) { return b >= m0 ? ns : (b >> l); };
-  split<P, 8>(x, s1, d1);
+  split<P, E1>(x, s1, d1);
   if (masked) {
-    ms = P::band_mask(8, w0 / 2, ns);
-    md = P::band_mask(8, w0 / 2, nd);
+    ms = P::band_mask(E1, w0 / 2, ns);
+    md = P::band_mask(E1, w0 / 2, nd);
   }
-  level<P, W, TF, 8>(s1, d1, cf, masked, ms, md);
-  P::template st_chunks<8>(out, ns, d1, w0 / 2, a >> 1, hi_d(1), lds);
+  level<P, W, TF, E1>(s1, d1, cf, masked, ms, md);
+  P::template st_chunks<E1>(out, ns, d1, w0 / 2, a >> 1, hi_d(1), lds);
   if (K == 1) {
-    P::template st_chunks<8>(sout, 0, s1, w0 / 2, a >> 1, hi_s(1), lds);
+    P::template st_chunks<E1>(sout, 0, s1, w0 / 2, a >> 1, hi_s(1), lds);
     return;
   }
   m = ns;
   ns = m - m / 2;
   nd = m / 2;
-  split<P, 4>(s1, s2, d2);
+  split<P, E2>(s1, s2, d2);
   if (masked) {
-    ms = P::band_mask(4, w0 / 4, ns);
-    md = P::band_mask(4, w0 / 4, nd);
+    ms = P::band_mask(E2, w0 / 4, ns);
+    md = P::band_mask(E2, w0 / 4, nd);
   }
-  level<P, W, TF, 4>(s2, d2, cf, masked, ms, md);
-  P::template st_chunks<4>(out, ns, d2, w0 / 4, a >> 2, hi_d(2), lds);
+  level<P, W, TF, E2>(s2, d2, cf, masked, ms, md);
+  P::template st_chunks<E2>(out, ns, d2, w0 / 4, a >> 2, hi_d(2), lds);
   if (K == 2) {
-    P::template st_chunks<4>(sout, 0, s2, w0 / 4, a >> 2, hi_s(2), lds);
+    P::template st_chunks<E2>(sout, 0, s2, w0 / 4, a >> 2, hi_s(2), lds);
     return;
   }
   m = ns;
   ns = m - m / 2;
   nd = m / 2;
-  split<P, 2>(s2, s3, d3);
+  split<P, E3>(s2, s3, d3);
   if (masked) {
-    ms = P::band_mask(2, w0 / 8, ns);
-    md = P::band_mask(2, w0 / 8, nd);
+    ms = P::band_mask(E3, w0 / 8, ns);
+    md = P::band_mask(E3, w0 / 8, nd);
   }
-  level<P, W, TF, 2>(s3, d3, cf, masked, ms, md);
-  P::template st_chunks<2>(out, ns, d3, w0 / 8, a >> 3, hi_d(3), lds);
+  level<P, W, TF, E3>(s3, d3, cf, masked, ms, md);
+  P::template st_chunks<E3>(out, ns, d3, w0 / 8, a >> 3, hi_d(3), lds);
   if (K == 3) {
-    P::template st_chunks<2>(sout, 0, s3, w0 / 8, a >> 3, hi_s(3), lds);
+    P::template st_chunks<E3>(sout, 0, s3, w0 / 8, a >> 3, hi_s(3), lds);
     return;
   }
   m = ns;
   ns = m - m / 2;
   nd = m / 2;
-  split<P, 1>(s3, s4, d4);
+  split<P, E4>(s3, s4, d4);
   if (masked) {
-    ms = P::band_mask(1, w0 / 16, ns);
-    md = P::band_mask(1, w0 / 16, nd);
+    ms = P::band_mask(E4, w0 / 16, ns);
+    md = P::band_mask(E4, w0 / 16, nd);
   }
-  level<P, W, TF, 1>(s4, d4, cf, masked, ms, md);
-  store_band<P, 1>(out, ns, d4, w0 / 16, a >> 4, hi_d(4));
-  store_band<P, 1>(sout, 0, s4, w0 / 16, a >> 4, hi_s(4));
+  level<P, W, TF, E4>(s4, d4, cf, masked, ms, md);
+  if constexpr (E4 == 1) {
+    store_band<P, 1>(out, ns, d4, w0 / 16, a >> 4, hi_d(4));
+    store_band<P, 1>(sout, 0, s4, w0 / 16, a >> 4, hi_s(4));
+  } else {
+    P::template st_chunks<E4>(out, ns, d4, w0 / 16, a >> 4, hi_d(4), lds);
+    P::template st_chunks<E4>(sout, 0, s4, w0 / 16, a >> 4, hi_s(4), lds);
+  }
 }
 
 // Reverse tile (Inverse / ForwardAdjoint) of K (1..4) levels: owned [a, b) of
 // the output band of length m0 (b == m0 for the last tile). Windows are
 // counted in level-4 units whatever K is: with g0 = (a >> 4) - Hk, the level-l
-// windows are [2^(4-l) g0, 2^(4-l) (g0 + 64)). Reads s_K from sin[i] and d_l
-// from in[ns_l + i]; writes the owned band values to out[q]. Windows that
-// overlap a band end (Zero boundary only) run masked. All global loads of the
-// tile are issued first (the coarse levels then compute while the fine bands
-// are in flight); each band is transposed to the chunk layout when its level
-// runs.
+// windows are [2^(4-l) g0, 2^(4-l) (g0 + kUnits)), kPerLane / 2^l values per
+// lane. Reads s_K from sin[i] and d_l from in[ns_l + i]; writes the owned band
+// values to out[q]. Windows that overlap a band end (Zero boundary only) run
+// masked. All global loads of the tile are issued first (the coarse levels
+// then compute while the fine bands are in flight); each band is transposed
+// to the chunk layout when its level runs.
 template <class P, class W, int TF, int K, class SI, class SK, class SO>
 __device__ __forceinline__ void rev_tile_k(const DProgram<float>* prog, long long m0, long long a, long long b,
                                            long long Hk, const SI* in, const SK* sin, SO* out, float* lds) {
   using V = typename P::V;
   using M = typename P::M;
+  constexpr int E1 = kPerLane / 2, E2 = kPerLane / 4, E3 = kPerLane / 8, E4 = kPerLane / 16;
   const long long g0 = (a >> 4) - Hk;
-  const bool masked = g0 < 0 || 16 * (g0 + 64) > m0;
+  const bool masked = g0 < 0 || 16 * (g0 + kUnits) > m0;
   const long long m1 = m0, m2 = m1 - m1 / 2, m3 = m2 - m2 / 2, m4 = m3 - m3 / 2;  // band split by level l
   const long long ns1 = m1 - m1 / 2, ns2 = m2 - m2 / 2, ns3 = m3 - m3 / 2, ns4 = m4 - m4 / 2;
   constexpr int NC = wt::NCoef<typename W::list, TF>::value;
   float cf[NC > 0 ? NC : 1];
   P::template coefs<NC>(prog->coef, cf);
   // raw (coalesced-order) loads: d_1 .. d_K and s_K
-  constexpr int EK = 1 << (4 - K);  // elements per lane of the level-K bands
-  V rs[EK], r1[8], r2[4], r3[2], r4[1];
+  constexpr int EK = kPerLane >> K;  // values per lane of the level-K bands
+  V rs[EK], r1[E1], r2[E2], r3[E3], r4[E4];
   if constexpr (K == 4) {
-    rs[0] = P::ld(sin, [&](int l) { return g0 + l; }, [&](int l) { return g0 + l >= 0 && g0 + l < ns4; });
-    r4[0] = P::ld(in + ns4, [&](int l) { return g0 + l; }, [&](int l) { return g0 + l >= 0 && g0 + l < m4 / 2; });
+    if constexpr (E4 == 1) {
+      rs[0] = P::ld(sin, [&](int l) { return g0 + l; }, [&](int l) { return g0 + l >= 0 && g0 + l < ns4; });
+      r4[0] = P::ld(in + ns4, [&](int l) { return g0 + l; }, [&](int l) { return g0 + l >= 0 && g0 + l < m4 / 2; });
+    } else {
+      P::template ld_raw<E4>(sin, g0, ns4, rs);
+      P::template ld_raw<E4>(in + ns4, g0, m4 / 2, r4);
+    }
   } else if constexpr (K == 3) {
-    P::template ld_raw<2>(sin, 2 * g0, ns3, rs);
+    P::template ld_raw<E3>(sin, 2 * g0, ns3, rs);
   } else if constexpr (K == 2) {
-    P::template ld_raw<4>(sin, 4 * g0, ns2, rs);
+    P::template ld_raw<E2>(sin, 4 * g0, ns2, rs);
   } else {
-    P::template ld_raw<8>(sin, 8 * g0, ns1, rs);
+    P::template ld_raw<E1>(sin, 8 * g0, ns1, rs);
   }
-  if constexpr (K >= 3) P::template ld_raw<2>(in + ns3, 2 * g0, m3 / 2, r3);
-  if constexpr (K >= 2) P::template ld_raw<4>(in + ns2, 4 * g0, m2 / 2, r2);
-  P::template ld_raw<8>(in + ns1, 8 * g0, m1 / 2, r1);
-  V s3[2], d3[2], s2[4], d2[4], s1[8], d1[8], x[16];
+  if constexpr (K >= 3) P::template ld_raw<E3>(in + ns3, 2 * g0, m3 / 2, r3);
+  if constexpr (K >= 2) P::template ld_raw<E2>(in + ns2, 4 * g0, m2 / 2, r2);
+  P::template ld_raw<E1>(in + ns1, 8 * g0, m1 / 2, r1);
+  V s3[E3], d3[E3], s2[E2], d2[E2], s1[E1], d1[E1], x[kPerLane];
   M ms{}, md{};
   if constexpr (K == 4) {
-    V s4[1] = {rs[0]}, d4[1] = {r4[0]};
-    if (masked) {
-      ms = P::band_mask(1, g0, ns4);
-      md = P::band_mask(1, g0, m4 / 2);
+    V s4[E4], d4[E4];
+    if constexpr (E4 == 1) {
+      s4[0] = rs[0];
+      d4[0] = r4[0];
+    } else {
+      P::template xpose_in<E4>(rs, s4, lds);
+      P::template xpose_in<E4>(r4, d4, lds);
     }
-    level<P, W, TF, 1>(s4, d4, cf, masked, ms, md);
-    merge<P, 1>(s4, d4, s3);
+    if (masked) {
+      ms = P::band_mask(E4, g0, ns4);
+      md = P::band_mask(E4, g0, m4 / 2);
+    }
+    level<P, W, TF, E4>(s4, d4, cf, masked, ms, md);
+    merge<P, E4>(s4, d4, s3);
   } else if constexpr (K == 3) {
-    P::template xpose_in<2>(rs, s3, lds);
+    P::template xpose_in<E3>(rs, s3, lds);
   }
   if constexpr (K >= 3) {
-    P::template xpose_in<2>(r3, d3, lds);
+    P::template xpose_in<E3>(r3, d3, lds);
     if (masked) {
-      ms = P::band_mask(2, 2 * g0, ns3);
-      md = P::band_mask(2, 2 * g0, m3 / 2);
+      ms = P::band_mask(E3, 2 * g0, ns3);
+      md = P::band_mask(E3, 2 * g0, m3 / 2);
     }
-    level<P, W, TF, 2>(s3, d3, cf, masked, ms, md);
-    merge<P, 2>(s3, d3, s2);
+    level<P, W, TF, E3>(s3, d3, cf, masked, ms, md);
+    merge<P, E3>(s3, d3, s2);
   } else if constexpr (K == 2) {
-    P::template xpose_in<4>(rs, s2, lds);
+    P::template xpose_in<E2>(rs, s2, lds);
   }
   if constexpr (K >= 2) {
-    P::template xpose_in<4>(r2, d2, lds);
+    P::template xpose_in<E2>(r2, d2, lds);
     if (masked) {
-      ms = P::band_mask(4, 4 * g0, ns2);
-      md = P::band_mask(4, 4 * g0, m2 / 2);
+      ms = P::band_mask(E2, 4 * g0, ns2);
+      md = P::band_mask(E2, 4 * g0, m2 / 2);
     }
-    level<P, W, TF, 4>(s2, d2, cf, masked, ms, md);
-    merge<P, 4>(s2, d2, s1);
+    level<P, W, TF, E2>(s2, d2, cf, masked, ms, md);
+    merge<P, E2>(s2, d2, s1);
   } else {
-    P::template xpose_in<8>(rs, s1, lds);
+    P::template xpose_in<E1>(rs, s1, lds);
   }
-  P::template xpose_in<8>(r1, d1, lds);
+  P::template xpose_in<E1>(r1, d1, lds);
   if (masked) {
-    ms = P::band_mask(8, 8 * g0, ns1);
-    md = P::band_mask(8, 8 * g0, m1 / 2);
+    ms = P::band_mask(E1, 8 * g0, ns1);
+    md = P::band_mask(E1, 8 * g0, m1 / 2);
   }
-  level<P, W, TF, 8>(s1, d1, cf, masked, ms, md);
-  merge<P, 8>(s1, d1, x);
-  P::template st_chunks<16>(out, 0, x, 16 * g0, a, b >= m0 ? m0 : b, lds);
+  level<P, W, TF, E1>(s1, d1, cf, masked, ms, md);
+  merge<P, E1>(s1, d1, x);
+  P::template st_chunks<kPerLane>(out, 0, x, 16 * g0, a, b >= m0 ? m0 : b, lds);
 }
 
 template <class P, class W, int TF, class SI, class SK, class SO>

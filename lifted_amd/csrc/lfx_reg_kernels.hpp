@@ -26,7 +26,8 @@ __device__ __forceinline__ void reg_item(const BatchItem& it, const DProgram<flo
 template <int TF>
 __global__ void __launch_bounds__(256) k_reg(const BatchItem* items, long long nitems, const DProgram<float>* progs,
                                              const float* uin, float* uout, float* scr) {
-  // per-wave transposition buffer: 16 x 64 samples, rows padded to 17
+  // per-wave transposition buffer: 64 rows of 17 floats (16 values per lane,
+  // or 32 rows of 33 floats per round of the 32-per-lane transposes)
   __shared__ float stage[4][kStageFloats];
   const long long w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
   if (w >= nitems) return;

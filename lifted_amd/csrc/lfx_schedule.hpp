@@ -108,7 +108,7 @@ LineSchedule schedule(const DProgram<C>& hp, bool fwd, long long n, long long L,
 // Work items of one group of levels of one signal.
 //
 // A tiled group of one to four levels of a float32 table wavelet runs its
-// interior as register tiles (one wavefront per 1024-sample window,
+// interior as register tiles (one wavefront per kRegWindow-sample window,
 // lfx_reg.hpp) and the parts next to the band ends as LDS tiles; other groups
 // are LDS tiles of the schedule's tile length, or one whole-line item.
 
@@ -120,13 +120,10 @@ struct ItemSpec {
   long long a, b, H;
 };
 
-constexpr int kRegLevels = 4;
-constexpr long long kRegWindow = 1024;  // samples per register tile window
-
 // Halo of an interior register tile of K levels: forward H (finest-level
-// samples, multiple of 16, owned 1024 - 2H); reverse Hk (level-4 units whatever
-// K is, owned 16 (64 - 2 Hk)). -1 when the stencil is too wide for the register
-// window to pay off.
+// samples, multiple of 16, owned kRegWindow - 2H); reverse Hk (level-4 units
+// whatever K is, owned 16 (kRegUnits - 2 Hk)). -1 when the stencil is too wide
+// for the register window to pay off.
 template <class C>
 long long reg_halo(const DProgram<C>& hp, bool fwd, int K, int bc) {
   const long long m0 = 1ll << 40, a0 = 1ll << 36;
@@ -134,8 +131,8 @@ long long reg_halo(const DProgram<C>& hp, bool fwd, int K, int bc) {
     for (long long H = 0; 2 * H <= kRegWindow - 256; H += 16)
       if (fwd_tile_ok(hp, m0, K, a0, a0 + kRegWindow - 2 * H, H, bc)) return H;
   } else {
-    for (long long Hk = 0; 16 * (64 - 2 * Hk) >= 256; ++Hk)
-      if (rev_tile_ok(hp, m0, K, a0, a0 + 16 * (64 - 2 * Hk), Hk << (kRegLevels - K), bc)) return Hk;
+    for (long long Hk = 0; 16 * (kRegUnits - 2 * Hk) >= 256; ++Hk)
+      if (rev_tile_ok(hp, m0, K, a0, a0 + 16 * (kRegUnits - 2 * Hk), Hk << (kRegLevels - K), bc)) return Hk;
   }
   return -1;
 }
@@ -180,7 +177,7 @@ std::vector<ItemSpec> pass_items(const DProgram<C>& hp, bool fwd, const PassPlan
   if (H >= 0 && bc == kZero && zero_cover_ok(hp, m0, pp.k)) {
     // Zero boundary: register windows may overlap the band ends (they run
     // masked, reading zeros there), so register tiles cover the whole band
-    const long long Tr = fwd ? kRegWindow - 2 * H : 16 * (64 - 2 * H);
+    const long long Tr = fwd ? kRegWindow - 2 * H : 16 * (kRegUnits - 2 * H);
     const long long nt = (m0 + Tr - 1) / Tr;
     auto ok = [&](long long t) {
       const long long a = t * Tr, b = std::min(a + Tr, m0);
@@ -202,21 +199,21 @@ std::vector<ItemSpec> pass_items(const DProgram<C>& hp, bool fwd, const PassPlan
     if (fwd) {
       Tr = kRegWindow - 2 * H;
       r0 = H + 64;
-      const long long last = m0 - kRegWindow + H - 64;  // window [a - H, a - H + 1024)
+      const long long last = m0 - kRegWindow + H - 64;  // window [a - H, a - H + kRegWindow)
       if (last >= r0) r1 = r0 + ((last - r0) / Tr + 1) * Tr;
     } else {
-      Tr = 16 * (64 - 2 * H);
+      Tr = 16 * (kRegUnits - 2 * H);
       r0 = 16 * (H + 4);
-      // every level's window [2^(4-l) g0, 2^(4-l) (g0 + 64)) inside its d band, and s_k's inside ns_k
+      // every level's window [2^(4-l) g0, 2^(4-l) (g0 + kRegUnits)) inside its d band, and s_k's inside ns_k
       long long gmax = -1;
       long long m = m0;
       bool ok = true;
       for (int l = 1; l <= pp.k; ++l) {
         const long long nd = m / 2, ns = m - nd;
         const long long span = 1ll << (kRegLevels - l);
-        const long long g = nd / span - 68;  // g0 + 64 < nd / span, with a margin
+        const long long g = nd / span - (kRegUnits + 4);  // g0 + kRegUnits < nd / span, with a margin
         gmax = (gmax < 0 || g < gmax) ? g : gmax;
-        if (l == pp.k) gmax = std::min(gmax, ns / span - 68);
+        if (l == pp.k) gmax = std::min(gmax, ns / span - (kRegUnits + 4));
         m = ns;
         if (g < 0) ok = false;
       }

@@ -17,48 +17,29 @@
 #include <vector>
 
 #include "../../lifted_amd/csrc/lfx_bodies.hpp"
-#include "../../lifted_amd/csrc/lfx_reg.hpp"
 #include "../../lifted_amd/csrc/lfx_schedule.hpp"
 #include "../../lifted_amd/csrc/lfx_table.hpp"
 
 using namespace lfx;
 
-// register tile of wavelet `wid` (compile-time type) on the 64-lane CPU model
-template <int TF>
-static bool reg_item(uint32_t wid, const DProgram<float>& prog, const ItemSpec& it, const float* in, float* out,
-                     float* sk) {
-  constexpr bool fwd = TF == kForward || TF == kInverseAdjoint;
-  switch (wid) {
-#define LFX_WAVELET(nm, idv, w)                                                                          \
-  case idv:                                                                                              \
-    if constexpr (fwd) reg::fwd_tile<reg::LaneCpu, wt::WL_##nm, TF>(&prog, it.m0, it.k, it.a, it.b, it.H, in, out, sk, nullptr); \
-    else reg::rev_tile<reg::LaneCpu, wt::WL_##nm, TF>(&prog, it.m0, it.k, it.a, it.b, it.H, in, sk, out, nullptr); \
-    return true;
-#define LFX_GENERIC(...)
-#define LFX_UNIT(...)
-#define LFX_REPEAT(...)
-#define LFX_SCALE(...)
-#define LFX_END(nm)
-#include "../../lifted_amd/csrc/wavelet_table.inc"
-#undef LFX_WAVELET
-#undef LFX_GENERIC
-#undef LFX_UNIT
-#undef LFX_REPEAT
-#undef LFX_SCALE
-#undef LFX_END
-  }
-  return false;
-}
+// register tiles of every table wavelet on the 64-lane CPU model: one
+// translation unit per transform type (sim_reg_tf.inc), compiled in parallel
+namespace lfx {
+bool sim_reg_tf0(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*);
+bool sim_reg_tf1(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*);
+bool sim_reg_tf2(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*);
+bool sim_reg_tf3(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*);
+}  // namespace lfx
 
 template <class C>
 static void reg_dispatch(int tf, uint32_t wid, const DProgram<C>& prog, const ItemSpec& it, const C* in, C* out, C* sk) {
   if constexpr (sizeof(C) == 4) {
     bool ok = false;
     switch (tf) {
-      case 0: ok = reg_item<0>(wid, prog, it, in, out, sk); break;
-      case 1: ok = reg_item<1>(wid, prog, it, in, out, sk); break;
-      case 2: ok = reg_item<2>(wid, prog, it, in, out, sk); break;
-      case 3: ok = reg_item<3>(wid, prog, it, in, out, sk); break;
+      case 0: ok = sim_reg_tf0(wid, prog, it, in, out, sk); break;
+      case 1: ok = sim_reg_tf1(wid, prog, it, in, out, sk); break;
+      case 2: ok = sim_reg_tf2(wid, prog, it, in, out, sk); break;
+      case 3: ok = sim_reg_tf3(wid, prog, it, in, out, sk); break;
     }
     if (!ok) fail(LFX_EINVAL, "no register kernel for this wavelet");
   } else {

@@ -1,0 +1,4 @@
+// Register tiles of transform type 3 on the CPU model (see sim_reg_tf.inc).
+#define SIM_TF 3
+#define SIM_FN sim_reg_tf3
+#include "sim_reg_tf.inc"
