@@ -435,9 +435,16 @@ def main():
     x = torch.randn(total, generator=gen, device=dev, dtype=torch.float32)
     y = torch.empty_like(x)
     z = torch.empty_like(x)
+    # planning knobs chosen by the autotuner on this data (defaults first)
+    from lifted_amd.tune import tune_batch
+    t_tune = time.perf_counter()
+    best, tried = tune_batch(sigs, (la.Transform.Forward, la.Transform.ForwardAdjoint), x, reps=3, warmup=1)
+    t_tune = time.perf_counter() - t_tune
+    print(f"[bench] autotune {t_tune:.1f}s: " + ", ".join(f"{k}={ms:.3f}ms" for k, ms in tried), file=sys.stderr,
+          flush=True)
     t_plan = time.perf_counter()
-    fwd = BatchPlan(sigs, la.Transform.Forward, dtype=torch.float32)
-    adj = BatchPlan(sigs, la.Transform.ForwardAdjoint, dtype=torch.float32)
+    fwd = BatchPlan(sigs, la.Transform.Forward, dtype=torch.float32, tuning=best)
+    adj = BatchPlan(sigs, la.Transform.ForwardAdjoint, dtype=torch.float32, tuning=best)
     t_plan = time.perf_counter() - t_plan
     plans = [fwd, adj]
     print(f"[bench] {len(sigs)} signals, {total} samples, plans in {t_plan:.1f}s, stages "
@@ -467,6 +474,8 @@ def main():
                                "random non-Lazy wavelet, levels U[3,max], Zero BC",
                    "signals": len(sigs), "samples": total, "seed": SEED},
         "hbm_frac_whole_step": round(4 * total * 4 / (ms * 1e-3) / HBM_PEAK, 4),
+        "tuning": {"chosen": best, "candidates": [{"knobs": k, "ms_per_step": round(t, 4)} for k, t in tried],
+                   "seconds": round(t_tune, 1)},
         "copy_GBps": round(copy_gbs, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,

@@ -160,8 +160,19 @@ int lfx_batch_execute_stage_timed(lfx_batch_plan plan, int32_t stage, const void
 
 /* Tile length (samples at a pass's finest level, 0 = default) and levels per
  * tiled pass (0 = default) used by plans and transforms created afterwards on
- * this thread; set by the autotuner. */
+ * this thread. */
 int lfx_set_tuning(int64_t tile, int32_t levels_per_pass);
+
+/* All planning knobs of the calling thread, for plans and transforms created
+ * afterwards; lifted_amd.tune searches them by timing candidate plans. */
+typedef struct {
+  int64_t tile;            /* LDS tile length (samples at a pass's finest level), 0 = default */
+  int32_t levels_per_pass; /* levels per tiled pass (1..12), 0 = default (4 for register tiles) */
+  int32_t side_streams;    /* side streams of a batched plan (0..8), -1 = default (3) */
+  int64_t line_limit;      /* register-tiled plans: longest band handed to the whole-line kernels, 0 = default */
+} lfx_tuning;
+int lfx_set_tuning_ex(const lfx_tuning* t);
+int lfx_get_tuning(lfx_tuning* t);
 
 /* Message of the last error on this thread. */
 const char* lfx_last_error(void);

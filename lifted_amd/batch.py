@@ -49,10 +49,17 @@ class BatchPlan:
 
     signals: iterable of (offset, length, wavelet, levels); levels 0 = max level.
     dtype: torch dtype of the packed arrays (float32, float64, bfloat16, float16).
+    tuning: planning knobs from lifted_amd.tune.tune_batch (None: defaults).
     """
 
-    def __init__(self, signals, transform, mode=0, dtype=None, plain=False):
+    def __init__(self, signals, transform, mode=0, dtype=None, plain=False, tuning=None):
+        """tuning: planning knobs (see lifted_amd.tune) used for this plan only."""
         import torch
+        if tuning:
+            from . import tune
+            with tune.tuning(**tuning):
+                self.__init__(signals, transform, mode, dtype, plain)
+            return
         L = _bind()
         sigs = list(signals)
         arr = (LfxSignal * max(len(sigs), 1))()

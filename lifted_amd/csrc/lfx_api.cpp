@@ -1333,12 +1333,39 @@ void lfx_batch_plan_destroy(lfx_batch_plan plan) {
   free_plan(p);
 }
 
+static void check_tuning(long long tile, int levels_per_pass) {
+  if (tile < 0 || levels_per_pass < 0 || levels_per_pass > 12) fail(LFX_EINVAL, "bad tuning values");
+  if (tile > 0 && (tile < 64 || tile > (1 << 16))) fail(LFX_EINVAL, "tile must be in [64, 65536]");
+}
+
 int lfx_set_tuning(int64_t tile, int32_t levels_per_pass) {
   return guarded([&] {
-    if (tile < 0 || levels_per_pass < 0 || levels_per_pass > 12) fail(LFX_EINVAL, "bad tuning values");
-    if (tile > 0 && (tile < 64 || tile > (1 << 16))) fail(LFX_EINVAL, "tile must be in [64, 65536]");
+    check_tuning(tile, levels_per_pass);
     g_tuning.tile = tile;
     g_tuning.k = levels_per_pass;
+  });
+}
+
+int lfx_set_tuning_ex(const lfx_tuning* t) {
+  return guarded([&] {
+    if (!t) fail(LFX_EINVAL, "null tuning");
+    check_tuning(t->tile, t->levels_per_pass);
+    if (t->side_streams < -1 || t->side_streams > 8) fail(LFX_EINVAL, "side_streams must be in [-1, 8]");
+    if (t->line_limit < 0 || (t->line_limit > 0 && t->line_limit < 16)) fail(LFX_EINVAL, "line_limit must be 0 or >= 16");
+    g_tuning.tile = t->tile;
+    g_tuning.k = t->levels_per_pass;
+    g_tuning.streams = t->side_streams;
+    g_tuning.line = t->line_limit;
+  });
+}
+
+int lfx_get_tuning(lfx_tuning* t) {
+  return guarded([&] {
+    if (!t) fail(LFX_EINVAL, "null tuning");
+    t->tile = g_tuning.tile;
+    t->levels_per_pass = g_tuning.k;
+    t->side_streams = g_tuning.streams;
+    t->line_limit = g_tuning.line;
   });
 }
 

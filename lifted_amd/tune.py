@@ -1,0 +1,132 @@
+"""Autotuner of batched plans.
+
+A batched plan's schedule depends on planning knobs that the library reads
+when the plan is created (lfx_set_tuning_ex in the C ABI):
+
+    levels_per_pass  levels fused per tiled pass (register tiles take 1..4)
+    line_limit       longest band that register-tiled plans hand to the
+                     whole-line (LDS) kernels; shorter bands finish there
+    side_streams     side streams that run a stage's LDS launches next to
+                     its register launch
+    tile             tile length of LDS-tiled passes (float64 and other plans
+                     without register tiles)
+
+tune_batch() builds the plans of a signal list for every candidate setting,
+times them on the device with the caller's data, and returns the fastest.
+The reference has no counterpart (its only knob is the thread count).
+"""
+import contextlib
+import ctypes
+import itertools
+import time
+
+from . import _lib
+
+FIELDS = ("tile", "levels_per_pass", "side_streams", "line_limit")
+
+
+class LfxTuning(ctypes.Structure):
+    _fields_ = [("tile", ctypes.c_int64), ("levels_per_pass", ctypes.c_int32), ("side_streams", ctypes.c_int32),
+                ("line_limit", ctypes.c_int64)]
+
+
+def _bind():
+    L = _lib.lib()
+    if not hasattr(L, "_tune_bound"):
+        L.lfx_set_tuning_ex.argtypes = [ctypes.POINTER(LfxTuning)]
+        L.lfx_set_tuning_ex.restype = ctypes.c_int
+        L.lfx_get_tuning.argtypes = [ctypes.POINTER(LfxTuning)]
+        L.lfx_get_tuning.restype = ctypes.c_int
+        L._tune_bound = True
+    return L
+
+
+def get_tuning():
+    """The calling thread's planning knobs (0 / -1 = library default)."""
+    t = LfxTuning()
+    _lib.check(_bind().lfx_get_tuning(ctypes.byref(t)))
+    return {f: getattr(t, f) for f in FIELDS}
+
+
+def set_tuning(tile=0, levels_per_pass=0, side_streams=-1, line_limit=0):
+    """Planning knobs for plans created afterwards on this thread."""
+    t = LfxTuning(int(tile), int(levels_per_pass), int(side_streams), int(line_limit))
+    _lib.check(_bind().lfx_set_tuning_ex(ctypes.byref(t)))
+
+
+@contextlib.contextmanager
+def tuning(**knobs):
+    """Temporarily set planning knobs (unnamed ones keep their defaults)."""
+    old = get_tuning()
+    cfg = {"tile": 0, "levels_per_pass": 0, "side_streams": -1, "line_limit": 0}
+    cfg.update(knobs)
+    set_tuning(**cfg)
+    try:
+        yield
+    finally:
+        set_tuning(**old)
+
+
+def default_space(dtype):
+    """Candidate knob values searched by tune_batch for a storage type."""
+    import torch
+    if dtype == torch.float64:
+        return {"tile": [0, 2048, 8192], "levels_per_pass": [0, 4], "side_streams": [-1, 0]}
+    return {"levels_per_pass": [0, 3], "line_limit": [0, 1024], "side_streams": [-1, 0]}
+
+
+def _time_plans(plans, x, bufs, reps, warmup):
+    import torch
+    stream = torch.cuda.current_stream(x.device)
+
+    def run():
+        src = x
+        for p, b in zip(plans, bufs):
+            p.execute(src, b)
+            src = b
+
+    for _ in range(warmup):
+        run()
+    torch.cuda.synchronize(x.device)
+    times = []
+    for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        run()
+        b.record(stream)
+        b.synchronize()
+        times.append(a.elapsed_time(b))
+    times.sort()
+    return times[len(times) // 2]
+
+
+def tune_batch(signals, transforms, x, mode=0, space=None, reps=5, warmup=2, verbose=False):
+    """Search planning knobs for batched plans of `signals`.
+
+    transforms: transform types run in sequence (x -> out_0 -> out_1 ...),
+    e.g. (Forward, ForwardAdjoint). x: the packed input (device tensor, the
+    plans' storage type). Returns (best knobs dict, [(knobs, ms), ...]) with
+    the candidates in the order timed; the library defaults are always the
+    first candidate, so the result is never slower than them on this data."""
+    import torch
+    from .batch import BatchPlan
+    space = dict(space if space is not None else default_space(x.dtype))
+    keys = list(space)
+    results = []
+    bufs = [torch.empty_like(x) for _ in transforms]
+    for values in itertools.product(*(space[k] for k in keys)):
+        knobs = dict(zip(keys, values))
+        t0 = time.perf_counter()
+        with tuning(**knobs):
+            plans = [BatchPlan(signals, tf, mode=mode, dtype=x.dtype) for tf in transforms]
+        try:
+            ms = _time_plans(plans, x, bufs, reps, warmup)
+        finally:
+            for p in plans:
+                p.close()
+        results.append((knobs, ms))
+        if verbose:
+            print(f"[tune] {knobs}: {ms:.3f} ms (plans {time.perf_counter() - t0:.2f} s)", flush=True)
+    best = min(results, key=lambda r: r[1])[0]
+    return best, results

@@ -149,3 +149,18 @@ def test_cpp_dropin_header_compiles_and_throws():
     r = subprocess.run([os.path.join(d, "build", "shim_main")], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "errors_thrown 3/3 max_level_db2_65536 14" in r.stdout
+
+
+def test_tuning_knobs_round_trip_and_validation():
+    """lfx_set_tuning_ex / lfx_get_tuning (no device needed): values round
+    trip, out-of-range values are rejected, the context manager restores."""
+    from lifted_amd import tune
+    from lifted_amd._lib import LfxError
+    base = tune.get_tuning()
+    with tune.tuning(levels_per_pass=3, line_limit=1024, side_streams=0):
+        assert tune.get_tuning() == {"tile": 0, "levels_per_pass": 3, "side_streams": 0, "line_limit": 1024}
+    assert tune.get_tuning() == base
+    for bad in ({"tile": 8}, {"levels_per_pass": 13}, {"side_streams": 9}, {"line_limit": 5}):
+        with pytest.raises(LfxError):
+            tune.set_tuning(**bad)
+    assert tune.get_tuning() == base

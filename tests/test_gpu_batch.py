@@ -106,3 +106,25 @@ def test_batch_fp32_graph_replay(gpu, oracle):
         torch.cuda.synchronize()
         assert torch.equal(y, ye) and torch.equal(z, ze), r
         assert torch.isfinite(w)
+
+
+def test_autotuned_plans_match_oracle(gpu, oracle):
+    """tune_batch times candidate planning knobs (defaults first) and the plans
+    built with the chosen knobs -- and with every candidate's knobs -- still
+    equal the oracle per signal."""
+    import torch
+    from lifted_amd.batch import BatchPlan
+    from lifted_amd.tune import tune_batch
+    sigs, total = make_signals(21, 48, 200, 150000, oracle)
+    xn = np.random.default_rng(5).standard_normal(total).astype(np.float32)
+    x = torch.from_numpy(xn).to(gpu)
+    best, tried = tune_batch(sigs, (0, 2), x, reps=1, warmup=1)
+    assert tried[0][0] == {"levels_per_pass": 0, "line_limit": 0, "side_streams": -1}
+    assert best in [k for k, _ in tried]
+    for knobs, _ in tried:
+        for tf in (0, 2):
+            y = BatchPlan(sigs, tf, dtype=torch.float32, tuning=knobs).execute(x).cpu().numpy()
+            for off, n, w, L in sigs:
+                ref = oracle.lwt(xn[off:off + n], w, tf, 0, level=[L])
+                scale = max(np.abs(ref).max(), 1e-30)
+                assert np.abs(y[off:off + n] - ref).max() <= 4 * np.finfo(np.float32).eps * scale, (knobs, tf, w)
