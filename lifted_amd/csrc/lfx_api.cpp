@@ -1122,19 +1122,42 @@ static bool lines_fast(const Plan& p, const long long* shp, int ax, long long L,
   hip_check(hipGetDevice(&dev), "hipGetDevice");
   std::lock_guard<std::mutex> lk(g_line_mu);
   BatchPlanImpl* bp = line_plan(LinePlanKey{dev, nlines, n, L, p.wavelet, p.tf, p.bc});
+  // A float32 side whose lines are already packed along the axis (its strides
+  // equal the dense layout's on every dim longer than 1) is used directly:
+  // the plan reads the input there, or writes the output there, saving that
+  // side's transpose. The plan's input and output must not alias, so an
+  // in-place call copies its input once.
+  auto packed = [&](const long long* str) {
+    if (p.storage != kF32) return false;
+    for (int k = 0; k < p.ndim; ++k)
+      if (shp[k] > 1 && str[k] != ws_str[k]) return false;
+    return true;
+  };
+  const bool in_packed = packed(sst) && src != dst;
+  const bool out_packed = packed(dstr);
   const size_t bytes = (size_t)total * sizeof(float);
   const size_t pad = (bytes + 255) / 256 * 256;
+  const int nws = (in_packed ? 0 : 1) + (out_packed ? 0 : 1);
+  // the lease is taken even when no workspace is needed: its event also
+  // orders this call's use of the cached plan's scratch after earlier calls
+  // on other streams
   WsLease lease;
-  lease_workspace(2 * pad, st, lease);
-  float* ws1 = (float*)lease.p;
-  float* ws2 = (float*)((char*)lease.p + pad);
-  const int fi = fast_dim(p.ndim, shp, sst);
-  hip_check(launch_permute(p.storage, kF32, src, ws1, perm_map(p.ndim, shp, sst, ws_str.data(), fi, ax), st),
-            "permute kernel");
-  batch_stage_all(bp, ws1, ws2, st);
-  const int fo = fast_dim(p.ndim, shp, dstr);
-  hip_check(launch_permute(kF32, p.storage, ws2, dst, perm_map(p.ndim, shp, ws_str.data(), dstr, ax, fo), st),
-            "permute kernel");
+  lease_workspace(std::max<size_t>((size_t)nws * pad, 256), st, lease);
+  float* ws1 = in_packed ? nullptr : (float*)lease.p;
+  float* ws2 = out_packed ? nullptr : (float*)((char*)lease.p + (in_packed ? 0 : pad));
+  const void* pin = src;
+  if (!in_packed) {
+    const int fi = fast_dim(p.ndim, shp, sst);
+    hip_check(launch_permute(p.storage, kF32, src, ws1, perm_map(p.ndim, shp, sst, ws_str.data(), fi, ax), st),
+              "permute kernel");
+    pin = ws1;
+  }
+  batch_stage_all(bp, pin, out_packed ? dst : (void*)ws2, st);
+  if (!out_packed) {
+    const int fo = fast_dim(p.ndim, shp, dstr);
+    hip_check(launch_permute(kF32, p.storage, ws2, dst, perm_map(p.ndim, shp, ws_str.data(), dstr, ax, fo), st),
+              "permute kernel");
+  }
   return true;
 }
 
