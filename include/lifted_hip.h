@@ -141,14 +141,14 @@ void lfx_batch_plan_destroy(lfx_batch_plan plan);
 
 /* One kernel launch of a batched stage. */
 typedef struct {
-  int32_t kind;      /* 0: LDS items (tiles, whole lines), 1: register tiles */
+  int32_t kind;      /* 0: LDS items (tiles, whole lines), 1: register tiles, 2: row items (four equal short lines per wavefront) */
   int32_t threads;   /* workgroup size */
   int64_t nitems;    /* work items (workgroups for kind 0, wavefronts for kind 1) */
   double alg_bytes;  /* algorithmic bytes: owned samples read + written */
   int64_t lds_bytes; /* dynamic LDS per workgroup (kind 0) */
 } lfx_launch_info;
 
-/* Launches of a stage (LDS launches first, then the register launch): fills
+/* Launches of a stage (LDS launches first, then the register launch, then the row launch): fills
  * info[0..min(n, max_launches)) and returns n, or an error code. */
 int lfx_batch_launch_info(lfx_batch_plan plan, int32_t stage, int32_t max_launches, lfx_launch_info* info);
 

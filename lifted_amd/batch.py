@@ -112,11 +112,12 @@ class BatchPlan:
 
     def launch_info(self, stage):
         """Kernel launches of a stage (LDS launches first, then the register
-        launch): dicts with kind ("lds" / "register"), threads, items,
-        alg_bytes, lds_bytes."""
+        launch, then the row launch): dicts with kind ("lds" / "register" /
+        "rows"), threads, items, alg_bytes, lds_bytes."""
         arr = (LaunchInfo * 16)()
         n = _lib.check(_bind().lfx_batch_launch_info(self._h, int(stage), 16, arr))
-        return [{"kind": "register" if arr[i].kind == 1 else "lds", "threads": arr[i].threads,
+        kinds = {0: "lds", 1: "register", 2: "rows"}
+        return [{"kind": kinds[arr[i].kind], "threads": arr[i].threads,
                  "items": arr[i].nitems, "alg_bytes": arr[i].alg_bytes, "lds_bytes": arr[i].lds_bytes}
                 for i in range(n)]
 

@@ -29,6 +29,10 @@ SOURCES = [
     ("lfx_reg_tf1.hip", True),
     ("lfx_reg_tf2.hip", True),
     ("lfx_reg_tf3.hip", True),
+    ("lfx_rows_tf0.hip", True),
+    ("lfx_rows_tf1.hip", True),
+    ("lfx_rows_tf2.hip", True),
+    ("lfx_rows_tf3.hip", True),
     ("lfx_api.cpp", False),
     ("lfx_table.cpp", False),
 ]

@@ -568,6 +568,8 @@ struct BatchStage {
   std::vector<BatchLaunch> launches;
   long long reg0 = 0, nreg = 0;  // register-tile items (one launch, all wavelets)
   double reg_bytes = 0;
+  long long rows0 = 0, nrows = 0;  // row items: four equal short lines per wavefront (one launch)
+  double rows_bytes = 0;
 };
 
 struct BatchPlanImpl {
@@ -597,7 +599,7 @@ struct Group {
 
 template <class C>
 static std::vector<Group> signal_groups(const WaveletDefL& w, int tf, int storage, int bc, long long n, long long L,
-                                        bool plain, bool reg) {
+                                        bool plain, bool reg, bool rows = false) {
   const DProgram<C> hp = resolve<C>(w, tf);
   const bool fwd = tf == kForward || tf == kInverseAdjoint;
   std::vector<Group> g;
@@ -606,7 +608,9 @@ static std::vector<Group> signal_groups(const WaveletDefL& w, int tf, int storag
   // that whole-line items stay small (several workgroups per CU); with a Zero
   // boundary register tiles also cover band ends, so only short bands are left
   // to the (latency-bound) whole-line kernels
-  const long long limit = reg ? (g_tuning.line > 0 ? g_tuning.line : (bc == kZero ? 256 : 4096)) : 0;
+  // (many equal lines, Zero boundary: row items take bands down to 32 samples)
+  const long long zl = rows ? 32 : 256;
+  const long long limit = reg ? (g_tuning.line > 0 ? g_tuning.line : (bc == kZero ? zl : 4096)) : 0;
   const LineSchedule sc = schedule(hp, fwd, n, L, storage, bc, g_tuning.tile, k_hint, bc == kPeriodic ? 0 : limit);
   for (auto& p : sc.passes) g.push_back(Group{true, p.m0, p.k, p.T, p.H, p.ntiles});
   if (sc.L_line > 0) {
@@ -653,9 +657,12 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
   std::vector<std::vector<size_t>> item_lds;
   std::vector<double> bytes;
   std::vector<std::vector<BatchItem>> reg_stage;
+  std::vector<std::vector<double>> reg_ibytes;  // algorithmic bytes of each register item
   std::vector<std::vector<double>> item_bytes;
   std::vector<double> reg_bytes;
   std::vector<DProgram<float>> reg_progs;  // program table of register items
+  std::vector<std::vector<BatchItem>> rows_stage;
+  std::vector<double> rows_bytes;
   std::map<uint32_t, int> reg_pidx;
   long long scr_elems = 0;
   // Level groups of every signal first. Forward plans put each signal's
@@ -680,8 +687,9 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
     levels[i] = L;
     if (sg.length == 0 || L == 0) continue;
     const bool reg = storage == kF32 && !plain && sg.wavelet < kCustomBase;
+    const bool rows = reg && nrep >= 4 && bc == kZero;
     groups[i] = cs == 8 ? signal_groups<double>(w, tf, storage, bc, sg.length, L, plain, reg)
-                        : signal_groups<float>(w, tf, storage, bc, sg.length, L, plain, reg);
+                        : signal_groups<float>(w, tf, storage, bc, sg.length, L, plain, reg, rows);
     Gmax = std::max(Gmax, (int)groups[i].size());
   }
   for (int i = 0; i < nsig; ++i) {
@@ -718,6 +726,8 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
         ri.wavelet = wid;
         ri.pidx = f->second;
         reg_stage[stage].push_back(ri);
+        if ((int)reg_ibytes.size() <= stage) reg_ibytes.resize(stage + 1);
+        reg_ibytes[stage].push_back(b);
         if ((int)bytes.size() <= stage) {
           per_stage.resize(stage + 1);
           cost.resize(stage + 1);
@@ -857,12 +867,39 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
         }
       bytes[st] *= (double)nrep;
     }
+    // Register items that hold a whole band of at most 512 samples (Zero
+    // boundary) become row items: four copies per wavefront, one per 16-lane
+    // row, starting at band position 0 (no halo: reads past a row read zeros).
+    rows_stage.resize(reg_stage.size());
+    rows_bytes.resize(reg_stage.size(), 0.0);
     for (size_t st = 0; st < reg_stage.size(); ++st) {
-      const size_t n0 = reg_stage[st].size();
-      reg_stage[st].reserve(n0 * nrep);
+      std::vector<BatchItem> keep;
+      double kept = 0;
+      for (size_t j = 0; j < reg_stage[st].size(); ++j) {
+        const BatchItem& it = reg_stage[st][j];
+        const bool row = nrep >= 4 && bc == kZero && it.a == 0 && it.b == it.m0 && it.m0 <= kRegWindow / 4;
+        if (!row) {
+          keep.push_back(it);
+          kept += reg_ibytes[st][j];
+          continue;
+        }
+        for (long long r = 0; r < nrep; r += 4) {
+          BatchItem ri = shift(it, r);
+          ri.rows = (int32_t)std::min<long long>(4, nrep - r);
+          for (int a = 0; a < 3; ++a) ri.rstride[a] = ri.arr[a] == kArrScr ? scr_elems : rep_stride;
+          ri.a = 0;
+          ri.b = ri.m0;
+          ri.H = 0;
+          rows_stage[st].push_back(ri);
+        }
+        rows_bytes[st] += reg_ibytes[st][j] * (double)nrep;
+      }
+      const size_t n0 = keep.size();
+      keep.reserve(n0 * nrep);
       for (long long r = 1; r < nrep; ++r)
-        for (size_t j = 0; j < n0; ++j) reg_stage[st].push_back(shift(reg_stage[st][j], r));
-      reg_bytes[st] *= (double)nrep;
+        for (size_t j = 0; j < n0; ++j) keep.push_back(shift(keep[j], r));
+      reg_stage[st] = std::move(keep);
+      reg_bytes[st] = kept * (double)nrep;
     }
     scr_elems *= nrep;
     plan->total *= nrep;
@@ -910,6 +947,13 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
       bs.reg_bytes = reg_bytes[st];
       all.insert(all.end(), rs.begin(), rs.end());
       bs.nitems += bs.nreg;
+    }
+    if (st < rows_stage.size() && !rows_stage[st].empty()) {
+      bs.rows0 = (long long)all.size();
+      bs.nrows = (long long)rows_stage[st].size();
+      bs.rows_bytes = rows_bytes[st];
+      all.insert(all.end(), rows_stage[st].begin(), rows_stage[st].end());
+      bs.nitems += bs.nrows;
     }
     plan->stages.push_back(bs);
   }
@@ -979,7 +1023,7 @@ static void batch_stage(BatchPlanImpl* p, int st, const void* in, void* out, hip
   // side streams next to the register launch on the caller's stream
   const int nside = std::min<int>((int)p->side.size(), (int)bs.launches.size());
   std::vector<hipStream_t> lanes;
-  if (nside > 0 && bs.launches.size() + (bs.nreg > 0 ? 1 : 0) >= 2) {
+  if (nside > 0 && bs.launches.size() + (bs.nreg > 0 ? 1 : 0) + (bs.nrows > 0 ? 1 : 0) >= 2) {
     hip_check(hipEventRecord(p->fork, s), "hipEventRecord");
     for (int i = 0; i < nside; ++i) {
       hip_check(hipStreamWaitEvent(p->side[i], p->fork, 0), "hipStreamWaitEvent");
@@ -1006,6 +1050,11 @@ static void batch_stage(BatchPlanImpl* p, int st, const void* in, void* out, hip
   if (bs.nreg > 0) {
     mark(true, s);
     hip_check(launch_reg(p->tf, p->d_progs, p->d_items + bs.reg0, bs.nreg, in, out, p->d_scr, s), "register kernel");
+    mark(false, s);
+  }
+  if (bs.nrows > 0) {
+    mark(true, s);
+    hip_check(launch_rows(p->tf, p->d_progs, p->d_items + bs.rows0, bs.nrows, in, out, p->d_scr, s), "row kernel");
     mark(false, s);
   }
   if (lanes[0] != s) {
@@ -1317,15 +1366,24 @@ int lfx_batch_launch_info(lfx_batch_plan plan, int32_t stage, int32_t max_launch
     if (!p) fail(LFX_EINVAL, "null plan");
     if (stage < 0 || stage >= (int)p->stages.size()) fail(LFX_EINVAL, "stage out of range");
     const BatchStage& bs = p->stages[stage];
-    const int n = (int)bs.launches.size() + (bs.nreg > 0 ? 1 : 0);
+    const int nl = (int)bs.launches.size();
+    const int n = nl + (bs.nreg > 0 ? 1 : 0) + (bs.nrows > 0 ? 1 : 0);
     for (int i = 0; i < n && i < max_launches && info; ++i) {
       lfx_launch_info& r = info[i];
-      const bool reg = i == (int)bs.launches.size();
-      r.kind = reg ? 1 : 0;
-      r.threads = reg ? 256 : (bs.launches[i].threads > 0 ? bs.launches[i].threads : block_threads());
-      r.nitems = reg ? bs.nreg : bs.launches[i].nitems;
-      r.alg_bytes = reg ? bs.reg_bytes : bs.launches[i].bytes;
-      r.lds_bytes = reg ? 0 : (int64_t)bs.launches[i].lds;
+      if (i < nl) {
+        r.kind = 0;
+        r.threads = bs.launches[i].threads > 0 ? bs.launches[i].threads : block_threads();
+        r.nitems = bs.launches[i].nitems;
+        r.alg_bytes = bs.launches[i].bytes;
+        r.lds_bytes = (int64_t)bs.launches[i].lds;
+      } else {
+        const bool reg = i == nl && bs.nreg > 0;
+        r.kind = reg ? 1 : 2;
+        r.threads = 256;
+        r.nitems = reg ? bs.nreg : bs.nrows;
+        r.alg_bytes = reg ? bs.reg_bytes : bs.rows_bytes;
+        r.lds_bytes = 0;
+      }
     }
     count = n;
   });
@@ -1339,7 +1397,7 @@ int lfx_batch_execute_stage_timed(lfx_batch_plan plan, int32_t stage, const void
     if (!p) fail(LFX_EINVAL, "null plan");
     if (stage < 0 || stage >= (int)p->stages.size()) fail(LFX_EINVAL, "stage out of range");
     const BatchStage& bs = p->stages[stage];
-    const int n = (int)bs.launches.size() + (bs.nreg > 0 ? 1 : 0);
+    const int n = (int)bs.launches.size() + (bs.nreg > 0 ? 1 : 0) + (bs.nrows > 0 ? 1 : 0);
     if (nevents < n || !ev_start || !ev_end) fail(LFX_EINVAL, "one start and one end event per launch are needed");
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev(n);
     for (int i = 0; i < n; ++i) {

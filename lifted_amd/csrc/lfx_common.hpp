@@ -133,7 +133,8 @@ struct BatchItem {
   int32_t arr[3];    // BatchArr of the three logical arrays
   int32_t pidx;      // register items: index into the plan's program table
   uint32_t wavelet;  // register items: wavelet id (selects the compiled step shapes)
-  int32_t pad;
+  int32_t rows;      // row items: lines in the item (1..4), one per 16-lane row; 0 otherwise
+  long long rstride[3];  // row items: element stride between consecutive lines of each array
 };
 
 }  // namespace lfx

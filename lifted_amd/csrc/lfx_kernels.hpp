@@ -33,6 +33,25 @@ inline hipError_t launch_reg(int tf, const void* progs, const BatchItem* items, 
   }
   return hipErrorInvalidValue;
 }
+// Row items (float32, Zero boundary): four equal short lines per wavefront.
+hipError_t launch_rows_0(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
+                         void* scr, hipStream_t st);
+hipError_t launch_rows_1(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
+                         void* scr, hipStream_t st);
+hipError_t launch_rows_2(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
+                         void* scr, hipStream_t st);
+hipError_t launch_rows_3(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
+                         void* scr, hipStream_t st);
+inline hipError_t launch_rows(int tf, const void* progs, const BatchItem* items, long long nitems, const void* in,
+                              void* out, void* scr, hipStream_t st) {
+  switch (tf) {
+    case 0: return launch_rows_0(progs, items, nitems, in, out, scr, st);
+    case 1: return launch_rows_1(progs, items, nitems, in, out, scr, st);
+    case 2: return launch_rows_2(progs, items, nitems, in, out, scr, st);
+    case 3: return launch_rows_3(progs, items, nitems, in, out, scr, st);
+  }
+  return hipErrorInvalidValue;
+}
 hipError_t launch_copy(int storage, const void* src, void* dst, const CopyMap& cm, long long total,
                        hipStream_t st);
 // Box copy between storage types (kF32 <-> kBF16 / kF16, or same type f32 /

@@ -39,23 +39,39 @@ constexpr int kUnits = (int)kRegUnits;        // the window in level-4 units
 static_assert(kLevels == 4 && kWindow == kLanes * kPerLane && (kPerLane == 16 || kPerLane == 32),
               "register tiles run four levels on 16 or 32 samples per lane");
 
+// Lane policies. ROWS = 1: the wavefront holds one window of a band. ROWS = 4
+// ("row mode"): it holds four equal lines of at most 512 samples, one per row
+// of 16 lanes, each row starting at band position 0; a lane's band position is
+// counted within its row, a read past a row's ends yields 0 (the Zero
+// boundary), and memory helpers address row r of an array at r * rs. The
+// trailing (rs, nr) arguments of the memory helpers (row stride, rows in use)
+// are ignored when ROWS = 1.
 #ifdef LFX_CPU_SIM
 // CPU simulation: a value is the 64 lanes of the wavefront. The elementwise
 // operations stay out of line: inlining a 64-lane loop into every unrolled
 // step makes the simulation too large to compile in reasonable time.
 #define LFX_SIM_NOINLINE __attribute__((noinline))
-struct LaneCpu {
+template <int ROWS>
+struct LaneCpuT {
+  static constexpr int kRows = ROWS;
+  static constexpr int kRowLanes = kLanes / ROWS;
   struct V {
     float v[kLanes];
   };
-  // value of x held by lane (lane + Q), the source lane clamped to the
-  // wavefront (as the GPU's chained one-lane shifts give)
+  // value of x held by lane (lane + Q): one window -- the source lane clamped
+  // to the wavefront (as the GPU's chained one-lane shifts give); rows -- 0
+  // when the source lies outside the lane's row
   template <int Q>
   LFX_SIM_NOINLINE static V from(const V& x) {
     V r;
     for (int l = 0; l < kLanes; ++l) {
-      const int src = l + Q < 0 ? 0 : (l + Q >= kLanes ? kLanes - 1 : l + Q);
-      r.v[l] = x.v[src];
+      const int src = l + Q;
+      if constexpr (ROWS == 1) {
+        r.v[l] = x.v[src < 0 ? 0 : (src >= kLanes ? kLanes - 1 : src)];
+      } else {
+        const bool in_row = src >= 0 && src < kLanes && src / kRowLanes == l / kRowLanes;
+        r.v[l] = in_row ? x.v[src] : 0.0f;
+      }
     }
     return r;
   }
@@ -86,7 +102,7 @@ struct LaneCpu {
     for (int l = 0; l < kLanes; ++l)
       if (pred(l)) dev::store_as<S, float>(p, idx(l), v.v[l]);
   }
-  // bit e of lane l: position p0 + E l + e lies in [0, n)
+  // bit e of lane l: band position p0 + E l' + e lies in [0, n) (l' = lane within its row)
   struct M {
     uint32_t m[kLanes];
   };
@@ -95,7 +111,7 @@ struct LaneCpu {
     for (int l = 0; l < kLanes; ++l) {
       r.m[l] = 0;
       for (int e = 0; e < E; ++e) {
-        const long long g = p0 + (long long)E * l + e;
+        const long long g = p0 + (long long)E * (l % kRowLanes) + e;
         if (g >= 0 && g < n) r.m[l] |= 1u << e;
       }
     }
@@ -112,15 +128,16 @@ struct LaneCpu {
     for (int l = 0; l < kLanes; ++l) r.v[l] = pred(l) ? dev::load_as<S, float>(p, idx(l)) : 0.0f;
     return r;
   }
-  // x[e] of lane l = p[pos0 + E l + e] where that position lies in [0, n), else 0.
-  // The model has no coalescing order: ld_raw yields the chunk values and
-  // xpose_in copies them.
+  // x[e] of lane l = p[pos0 + E l' + e] of its row where that position lies in
+  // [0, n), else 0. The model has no coalescing order: ld_raw yields the chunk
+  // values and xpose_in copies them.
   template <int E, class S>
-  static void ld_raw(const S* p, long long pos0, long long n, V (&t)[E]) {
+  static void ld_raw(const S* p, long long pos0, long long n, V (&t)[E], long long rs = 0, int nr = 1) {
     for (int e = 0; e < E; ++e)
       for (int l = 0; l < kLanes; ++l) {
-        const long long g = pos0 + (long long)E * l + e;
-        t[e].v[l] = (g >= 0 && g < n) ? dev::load_as<S, float>(p, g) : 0.0f;
+        const int r = l / kRowLanes;
+        const long long g = pos0 + (long long)E * (l % kRowLanes) + e;
+        t[e].v[l] = (r < nr && g >= 0 && g < n) ? dev::load_as<S, float>(p + r * rs, g) : 0.0f;
       }
   }
   template <int E>
@@ -128,9 +145,10 @@ struct LaneCpu {
     for (int e = 0; e < E; ++e) x[e] = t[e];
   }
   template <int E, class S>
-  static void ld_chunks(const S* p, long long pos0, long long n, V (&x)[E], float* lds) {
+  static void ld_chunks(const S* p, long long pos0, long long n, V (&x)[E], float* lds, long long rs = 0,
+                        int nr = 1) {
     V t[E];
-    ld_raw<E>(p, pos0, n, t);
+    ld_raw<E>(p, pos0, n, t, rs, nr);
     xpose_in<E>(t, x, lds);
   }
   // the N step coefficients of a program, read once per tile
@@ -139,31 +157,46 @@ struct LaneCpu {
     for (int i = 0; i < N; ++i) c[i] = p[i];
   }
   template <int E, class S>
-  static void st_chunks(S* p, long long pos0, const V (&v)[E], long long g0, long long lo, long long hi, float*) {
+  static void st_chunks(S* p, long long pos0, const V (&v)[E], long long g0, long long lo, long long hi, float*,
+                        long long rs = 0, int nr = 1) {
     for (int e = 0; e < E; ++e)
       for (int l = 0; l < kLanes; ++l) {
-        const long long g = g0 + (long long)E * l + e;
-        if (g >= lo && g < hi) dev::store_as<S, float>(p, pos0 + g, v[e].v[l]);
+        const int r = l / kRowLanes;
+        const long long g = g0 + (long long)E * (l % kRowLanes) + e;
+        if (r < nr && g >= lo && g < hi) dev::store_as<S, float>(p + r * rs, pos0 + g, v[e].v[l]);
       }
   }
 };
+using LaneCpu = LaneCpuT<1>;
+using RowCpu = LaneCpuT<4>;
 #else
-struct LaneGpu {
+template <int ROWS>
+struct LaneGpuT {
+  static constexpr int kRows = ROWS;
+  static constexpr int kRowLanes = kLanes / ROWS;
   using V = float;
   static __device__ __forceinline__ int lane() { return (int)(threadIdx.x & (kLanes - 1)); }
-  // One-lane whole-wave shifts (DPP wave_shl:1 / wave_shr:1): lane l gets the
-  // value of lane l + 1 (l - 1); the lane with no source keeps its own value.
+  // One-lane shifts by DPP. One window: wave_shl:1 / wave_shr:1, lane l gets
+  // the value of lane l + 1 (l - 1) and the lane with no source keeps its own
+  // value. Rows: row_shl:1 / row_shr:1 within each 16-lane row, the lane with
+  // no source in its row gets 0.
   static __device__ __forceinline__ V shl1(V v) {
     const int b = __builtin_bit_cast(int, v);
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, 0x130, 0xF, 0xF, false));
+    if constexpr (ROWS == 1)
+      return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, 0x130, 0xF, 0xF, false));
+    else
+      return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, b, 0x101, 0xF, 0xF, false));
   }
   static __device__ __forceinline__ V shr1(V v) {
     const int b = __builtin_bit_cast(int, v);
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, 0x138, 0xF, 0xF, false));
+    if constexpr (ROWS == 1)
+      return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, 0x138, 0xF, 0xF, false));
+    else
+      return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, b, 0x111, 0xF, 0xF, false));
   }
-  // value of v held by lane (lane + Q), the source lane clamped to the
-  // wavefront: chains of one-lane DPP shifts for |Q| <= 3 (a few cycles each,
-  // no LDS round trip), a clamped ds_bpermute beyond
+  // value of v held by lane (lane + Q): chains of one-lane DPP shifts for
+  // |Q| <= 3 (a few cycles each, no LDS round trip), a ds_bpermute beyond; one
+  // window clamps the source lane to the wavefront, rows give 0 outside the row
   template <int Q>
   static __device__ __forceinline__ V from(V v) {
     if constexpr (Q == 0) {
@@ -173,9 +206,13 @@ struct LaneGpu {
     } else if constexpr (Q < 0 && Q >= -3) {
       return from<Q + 1>(shr1(v));
     } else {
-      int src = lane() + Q;
+      const int l = lane();
+      int src = l + Q;
+      const bool in_row = src >= 0 && src < kLanes && src / kRowLanes == l / kRowLanes;
       src = src < 0 ? 0 : (src >= kLanes ? kLanes - 1 : src);
-      return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src << 2, __builtin_bit_cast(int, v)));
+      const float r = __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src << 2, __builtin_bit_cast(int, v)));
+      if constexpr (ROWS == 1) return r;
+      else return in_row ? r : 0.0f;
     }
   }
   static __device__ __forceinline__ V fma(float c, V x, V acc) { return __builtin_fmaf(c, x, acc); }
@@ -196,10 +233,10 @@ struct LaneGpu {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
-  // bit e: position p0 + E lane + e lies in [0, n)
+  // bit e: band position p0 + E l' + e lies in [0, n) (l' = lane within its row)
   using M = uint32_t;
   static __device__ __forceinline__ M band_mask(int E, long long p0, long long n) {
-    const long long g = p0 + (long long)E * lane();
+    const long long g = p0 + (long long)E * (lane() % kRowLanes);
     M r = 0;
     for (int e = 0; e < E; ++e) r |= (g + e >= 0 && g + e < n) ? (1u << e) : 0u;
     return r;
@@ -215,20 +252,32 @@ struct LaneGpu {
   // (row stride E + 1 keeps the chunk reads conflict-free). The bounds checks
   // run only for windows that overlap a band end (a wave-uniform test).
   // Split in two so that a tile can issue all its global loads first and
-  // transpose each band when a level needs it: ld_raw leaves t[j] =
-  // p[pos0 + 64 j + lane], xpose_in moves it to the chunk layout.
+  // transpose each band when a level needs it: ld_raw leaves t[j] = element
+  // u = 64 j + lane of the window, xpose_in moves it to the chunk layout. In
+  // row mode element u is position u % (16 E) of row u / (16 E).
   template <int E, class S>
-  static __device__ __forceinline__ void ld_raw(const S* p, long long pos0, long long n, V (&t)[E]) {
+  static __device__ __forceinline__ void ld_raw(const S* p, long long pos0, long long n, V (&t)[E], long long rs = 0,
+                                                int nr = 1) {
     const int l = lane();
-    if (pos0 >= 0 && pos0 + E * kLanes <= n) {
-      const S* q = p + pos0;
+    if constexpr (ROWS == 1) {
+      if (pos0 >= 0 && pos0 + E * kLanes <= n) {
+        const S* q = p + pos0;
 #pragma unroll
-      for (int j = 0; j < E; ++j) t[j] = dev::load_as<S, float>(q, j * kLanes + l);  // all loads in flight
+        for (int j = 0; j < E; ++j) t[j] = dev::load_as<S, float>(q, j * kLanes + l);  // all loads in flight
+      } else {
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+          const long long g = pos0 + j * kLanes + l;
+          t[j] = (g >= 0 && g < n) ? dev::load_as<S, float>(p, g) : 0.0f;
+        }
+      }
     } else {
+      constexpr int SEG = E * kRowLanes;
 #pragma unroll
       for (int j = 0; j < E; ++j) {
-        const long long g = pos0 + j * kLanes + l;
-        t[j] = (g >= 0 && g < n) ? dev::load_as<S, float>(p, g) : 0.0f;
+        const int u = j * kLanes + l, r = u / SEG;
+        const long long g = pos0 + u % SEG;
+        t[j] = (r < nr && g >= 0 && g < n) ? dev::load_as<S, float>(p + r * rs, g) : 0.0f;
       }
     }
   }
@@ -266,9 +315,10 @@ struct LaneGpu {
     }
   }
   template <int E, class S>
-  static __device__ __forceinline__ void ld_chunks(const S* p, long long pos0, long long n, V (&x)[E], float* lds) {
+  static __device__ __forceinline__ void ld_chunks(const S* p, long long pos0, long long n, V (&x)[E], float* lds,
+                                                   long long rs = 0, int nr = 1) {
     V t[E];
-    ld_raw<E>(p, pos0, n, t);
+    ld_raw<E>(p, pos0, n, t, rs, nr);
     xpose_in<E>(t, x, lds);
   }
   // The N step coefficients of a program, read once per tile through the
@@ -282,16 +332,30 @@ struct LaneGpu {
 #pragma unroll
     for (int i = 0; i < N; ++i) c[i] = q[i];
   }
-  // p[pos0 + g] = v for the owned globals g = g0 + E * lane + e in [lo, hi),
-  // transposed through LDS so that the global stores are coalesced.
+  // Store value u (coalesced order) of the window if its band position lies
+  // in [lo, hi). One window: position g0 + u, owned iff u in [t0, t1) (window
+  // offsets, 32-bit); rows: position g0 + u % (16 E) of row u / (16 E).
+  template <int E, class S>
+  static __device__ __forceinline__ void st_one(S* p, long long pos0, long long g0, long long lo, long long hi, int t0,
+                                                int t1, int u, float val, long long rs, int nr) {
+    if constexpr (ROWS == 1) {
+      if (u >= t0 && u < t1) dev::store_as<S, float>(p + pos0 + g0, u, val);
+    } else {
+      constexpr int SEG = E * kRowLanes;
+      const int r = u / SEG;
+      const long long g = g0 + u % SEG;
+      if (r < nr && g >= lo && g < hi) dev::store_as<S, float>(p + r * rs, pos0 + g, val);
+    }
+  }
+  // p[pos0 + g] = v for the owned band positions g = g0 + E * lane + e in
+  // [lo, hi), transposed through LDS so that the global stores are coalesced.
   template <int E, class S>
   static __device__ __forceinline__ void st_chunks(S* p, long long pos0, const V (&v)[E], long long g0, long long lo,
-                                                   long long hi, float* lds) {
+                                                   long long hi, float* lds, long long rs = 0, int nr = 1) {
     const int l = lane();
-    // owned window-relative range [t0, t1) of the E * 64 values
-    const int t0 = (int)(lo - g0 < 0 ? 0 : lo - g0);
-    const int t1 = (int)(hi - g0 > E * kLanes ? E * kLanes : hi - g0);
-    S* q = p + pos0 + g0;
+    // owned window-relative range [t0, t1) of the E * 64 values (one window)
+    const int t0 = (int)(lo - g0 < 0 ? 0 : (lo - g0 > E * kLanes ? E * kLanes : lo - g0));
+    const int t1 = (int)(hi - g0 > E * kLanes ? E * kLanes : (hi - g0 < 0 ? 0 : hi - g0));
     if constexpr (E <= 16) {
 #pragma unroll
       for (int e = 0; e < E; ++e) lds[l * (E + 1) + e] = v[e];
@@ -299,8 +363,7 @@ struct LaneGpu {
 #pragma unroll
       for (int j = 0; j < E; ++j) {
         const int u = j * kLanes + l;
-        const float val = lds[(u / E) * (E + 1) + (u % E)];
-        if (u >= t0 && u < t1) dev::store_as<S, float>(q, u, val);
+        st_one<E>(p, pos0, g0, lo, hi, t0, t1, u, lds[(u / E) * (E + 1) + (u % E)], rs, nr);
       }
       wave_sync();
     } else {
@@ -316,14 +379,15 @@ struct LaneGpu {
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
           const int u = (16 * h + j) * kLanes + l;
-          const float val = lds[(2 * j + (l >> 5)) * 33 + (l & 31)];
-          if (u >= t0 && u < t1) dev::store_as<S, float>(q, u, val);
+          st_one<E>(p, pos0, g0, lo, hi, t0, t1, u, lds[(2 * j + (l >> 5)) * 33 + (l & 31)], rs, nr);
         }
         wave_sync();
       }
     }
   }
 };
+using LaneGpu = LaneGpuT<1>;
+using RowGpu = LaneGpuT<4>;
 #endif
 
 template <int A, int B>
@@ -437,6 +501,14 @@ __device__ __forceinline__ void store_band(S* p, long long pos0, const typename 
   });
 }
 
+// Lines of a row-mode tile: element stride between consecutive lines in each
+// of the tile's three arrays (input, output, s band) and the number of lines
+// in use (1..4). Ignored by the one-window policies.
+struct RowArgs {
+  long long rs[3];
+  int n;
+};
+
 // Forward tile (Forward / InverseAdjoint) of K (1..4) levels: owned [a, b)
 // of the input band of length m0 (b == m0 for the last tile), window
 // [w0, w0 + kWindow) with w0 = a - H (a multiple of 16). Lane l holds the
@@ -447,17 +519,19 @@ __device__ __forceinline__ void store_band(S* p, long long pos0, const typename 
 // over the wavefront.
 template <class P, class W, int TF, class SI, class SO, class SK>
 __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long m0, int K, long long a, long long b,
-                                         long long H, const SI* in, SO* out, SK* sout, float* lds) {
+                                         long long H, const SI* in, SO* out, SK* sout, float* lds,
+                                         const RowArgs& ra = RowArgs{{0, 0, 0}, 1}) {
   using V = typename P::V;
   using M = typename P::M;
   constexpr int E1 = kPerLane / 2, E2 = kPerLane / 4, E3 = kPerLane / 8, E4 = kPerLane / 16;
+  static_assert(P::kRows == 1 || E4 > 1, "row mode stores level-4 bands through the chunk transposes");
   const long long w0 = a - H;
-  const bool masked = w0 < 0 || w0 + kWindow > m0;
+  const bool masked = P::kRows > 1 || w0 < 0 || w0 + kWindow > m0;
   constexpr int NC = wt::NCoef<typename W::list, TF>::value;
   float cf[NC > 0 ? NC : 1];
   P::template coefs<NC>(prog->coef, cf);
   V x[kPerLane];
-  P::template ld_chunks<kPerLane>(in, w0, m0, x, lds);
+  P::template ld_chunks<kPerLane>(in, w0, m0, x, lds, ra.rs[0], ra.n);
   V s1[E1], d1[E1], s2[E2], d2[E2], s3[E3], d3[E3], s4[E4], d4[E4];
   M ms{}, md{};
   long long m = m0, ns = m - m / 2, nd = m / 2;
@@ -470,9 +544,9 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long 
     md = P::band_mask(E1, w0 / 2, nd);
   }
   level<P, W, TF, E1>(s1, d1, cf, masked, ms, md);
-  P::template st_chunks<E1>(out, ns, d1, w0 / 2, a >> 1, hi_d(1), lds);
+  P::template st_chunks<E1>(out, ns, d1, w0 / 2, a >> 1, hi_d(1), lds, ra.rs[1], ra.n);
   if (K == 1) {
-    P::template st_chunks<E1>(sout, 0, s1, w0 / 2, a >> 1, hi_s(1), lds);
+    P::template st_chunks<E1>(sout, 0, s1, w0 / 2, a >> 1, hi_s(1), lds, ra.rs[2], ra.n);
     return;
   }
   m = ns;
@@ -484,9 +558,9 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long 
     md = P::band_mask(E2, w0 / 4, nd);
   }
   level<P, W, TF, E2>(s2, d2, cf, masked, ms, md);
-  P::template st_chunks<E2>(out, ns, d2, w0 / 4, a >> 2, hi_d(2), lds);
+  P::template st_chunks<E2>(out, ns, d2, w0 / 4, a >> 2, hi_d(2), lds, ra.rs[1], ra.n);
   if (K == 2) {
-    P::template st_chunks<E2>(sout, 0, s2, w0 / 4, a >> 2, hi_s(2), lds);
+    P::template st_chunks<E2>(sout, 0, s2, w0 / 4, a >> 2, hi_s(2), lds, ra.rs[2], ra.n);
     return;
   }
   m = ns;
@@ -498,9 +572,9 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long 
     md = P::band_mask(E3, w0 / 8, nd);
   }
   level<P, W, TF, E3>(s3, d3, cf, masked, ms, md);
-  P::template st_chunks<E3>(out, ns, d3, w0 / 8, a >> 3, hi_d(3), lds);
+  P::template st_chunks<E3>(out, ns, d3, w0 / 8, a >> 3, hi_d(3), lds, ra.rs[1], ra.n);
   if (K == 3) {
-    P::template st_chunks<E3>(sout, 0, s3, w0 / 8, a >> 3, hi_s(3), lds);
+    P::template st_chunks<E3>(sout, 0, s3, w0 / 8, a >> 3, hi_s(3), lds, ra.rs[2], ra.n);
     return;
   }
   m = ns;
@@ -516,8 +590,8 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long 
     store_band<P, 1>(out, ns, d4, w0 / 16, a >> 4, hi_d(4));
     store_band<P, 1>(sout, 0, s4, w0 / 16, a >> 4, hi_s(4));
   } else {
-    P::template st_chunks<E4>(out, ns, d4, w0 / 16, a >> 4, hi_d(4), lds);
-    P::template st_chunks<E4>(sout, 0, s4, w0 / 16, a >> 4, hi_s(4), lds);
+    P::template st_chunks<E4>(out, ns, d4, w0 / 16, a >> 4, hi_d(4), lds, ra.rs[1], ra.n);
+    P::template st_chunks<E4>(sout, 0, s4, w0 / 16, a >> 4, hi_s(4), lds, ra.rs[2], ra.n);
   }
 }
 
@@ -532,12 +606,14 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long 
 // to the chunk layout when its level runs.
 template <class P, class W, int TF, int K, class SI, class SK, class SO>
 __device__ __forceinline__ void rev_tile_k(const DProgram<float>* prog, long long m0, long long a, long long b,
-                                           long long Hk, const SI* in, const SK* sin, SO* out, float* lds) {
+                                           long long Hk, const SI* in, const SK* sin, SO* out, float* lds,
+                                           const RowArgs& ra) {
   using V = typename P::V;
   using M = typename P::M;
   constexpr int E1 = kPerLane / 2, E2 = kPerLane / 4, E3 = kPerLane / 8, E4 = kPerLane / 16;
+  static_assert(P::kRows == 1 || E4 > 1, "row mode loads level-4 bands through the chunk transposes");
   const long long g0 = (a >> 4) - Hk;
-  const bool masked = g0 < 0 || 16 * (g0 + kUnits) > m0;
+  const bool masked = P::kRows > 1 || g0 < 0 || 16 * (g0 + kUnits) > m0;
   const long long m1 = m0, m2 = m1 - m1 / 2, m3 = m2 - m2 / 2, m4 = m3 - m3 / 2;  // band split by level l
   const long long ns1 = m1 - m1 / 2, ns2 = m2 - m2 / 2, ns3 = m3 - m3 / 2, ns4 = m4 - m4 / 2;
   constexpr int NC = wt::NCoef<typename W::list, TF>::value;
@@ -551,19 +627,19 @@ __device__ __forceinline__ void rev_tile_k(const DProgram<float>* prog, long lon
       rs[0] = P::ld(sin, [&](int l) { return g0 + l; }, [&](int l) { return g0 + l >= 0 && g0 + l < ns4; });
       r4[0] = P::ld(in + ns4, [&](int l) { return g0 + l; }, [&](int l) { return g0 + l >= 0 && g0 + l < m4 / 2; });
     } else {
-      P::template ld_raw<E4>(sin, g0, ns4, rs);
-      P::template ld_raw<E4>(in + ns4, g0, m4 / 2, r4);
+      P::template ld_raw<E4>(sin, g0, ns4, rs, ra.rs[2], ra.n);
+      P::template ld_raw<E4>(in + ns4, g0, m4 / 2, r4, ra.rs[0], ra.n);
     }
   } else if constexpr (K == 3) {
-    P::template ld_raw<E3>(sin, 2 * g0, ns3, rs);
+    P::template ld_raw<E3>(sin, 2 * g0, ns3, rs, ra.rs[2], ra.n);
   } else if constexpr (K == 2) {
-    P::template ld_raw<E2>(sin, 4 * g0, ns2, rs);
+    P::template ld_raw<E2>(sin, 4 * g0, ns2, rs, ra.rs[2], ra.n);
   } else {
-    P::template ld_raw<E1>(sin, 8 * g0, ns1, rs);
+    P::template ld_raw<E1>(sin, 8 * g0, ns1, rs, ra.rs[2], ra.n);
   }
-  if constexpr (K >= 3) P::template ld_raw<E3>(in + ns3, 2 * g0, m3 / 2, r3);
-  if constexpr (K >= 2) P::template ld_raw<E2>(in + ns2, 4 * g0, m2 / 2, r2);
-  P::template ld_raw<E1>(in + ns1, 8 * g0, m1 / 2, r1);
+  if constexpr (K >= 3) P::template ld_raw<E3>(in + ns3, 2 * g0, m3 / 2, r3, ra.rs[0], ra.n);
+  if constexpr (K >= 2) P::template ld_raw<E2>(in + ns2, 4 * g0, m2 / 2, r2, ra.rs[0], ra.n);
+  P::template ld_raw<E1>(in + ns1, 8 * g0, m1 / 2, r1, ra.rs[0], ra.n);
   V s3[E3], d3[E3], s2[E2], d2[E2], s1[E1], d1[E1], x[kPerLane];
   M ms{}, md{};
   if constexpr (K == 4) {
@@ -613,17 +689,18 @@ __device__ __forceinline__ void rev_tile_k(const DProgram<float>* prog, long lon
   }
   level<P, W, TF, E1>(s1, d1, cf, masked, ms, md);
   merge<P, E1>(s1, d1, x);
-  P::template st_chunks<kPerLane>(out, 0, x, 16 * g0, a, b >= m0 ? m0 : b, lds);
+  P::template st_chunks<kPerLane>(out, 0, x, 16 * g0, a, b >= m0 ? m0 : b, lds, ra.rs[1], ra.n);
 }
 
 template <class P, class W, int TF, class SI, class SK, class SO>
 __device__ __forceinline__ void rev_tile(const DProgram<float>* prog, long long m0, int K, long long a, long long b,
-                                         long long Hk, const SI* in, const SK* sin, SO* out, float* lds) {
+                                         long long Hk, const SI* in, const SK* sin, SO* out, float* lds,
+                                         const RowArgs& ra = RowArgs{{0, 0, 0}, 1}) {
   switch (K) {
-    case 4: rev_tile_k<P, W, TF, 4>(prog, m0, a, b, Hk, in, sin, out, lds); return;
-    case 3: rev_tile_k<P, W, TF, 3>(prog, m0, a, b, Hk, in, sin, out, lds); return;
-    case 2: rev_tile_k<P, W, TF, 2>(prog, m0, a, b, Hk, in, sin, out, lds); return;
-    default: rev_tile_k<P, W, TF, 1>(prog, m0, a, b, Hk, in, sin, out, lds); return;
+    case 4: rev_tile_k<P, W, TF, 4>(prog, m0, a, b, Hk, in, sin, out, lds, ra); return;
+    case 3: rev_tile_k<P, W, TF, 3>(prog, m0, a, b, Hk, in, sin, out, lds, ra); return;
+    case 2: rev_tile_k<P, W, TF, 2>(prog, m0, a, b, Hk, in, sin, out, lds, ra); return;
+    default: rev_tile_k<P, W, TF, 1>(prog, m0, a, b, Hk, in, sin, out, lds, ra); return;
   }
 }
 

@@ -62,6 +62,64 @@ __global__ void __launch_bounds__(256) k_reg(const BatchItem* items, long long n
   }
 }
 
+// Row items: four equal lines of at most 512 samples per wavefront, one per
+// 16-lane row, whole bands from position 0 (Zero boundary).
+template <class W, int TF>
+__device__ __forceinline__ void row_item(const BatchItem& it, const DProgram<float>* prog, float* a0, float* a1,
+                                         float* a2, float* lds) {
+  const reg::RowArgs ra{{it.rstride[0], it.rstride[1], it.rstride[2]}, it.rows};
+  if constexpr (TF == kForward || TF == kInverseAdjoint)
+    reg::fwd_tile<reg::RowGpu, W, TF>(prog, it.m0, it.k, 0, it.m0, 0, (const float*)a0, a1, a2, lds, ra);
+  else
+    reg::rev_tile<reg::RowGpu, W, TF>(prog, it.m0, it.k, 0, it.m0, 0, (const float*)a0, (const float*)a2, a1, lds, ra);
+}
+
+template <int TF>
+__global__ void __launch_bounds__(256) k_reg_rows(const BatchItem* items, long long nitems,
+                                                  const DProgram<float>* progs, const float* uin, float* uout,
+                                                  float* scr) {
+  __shared__ float stage[4][kStageFloats];
+  const long long w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  if (w >= nitems) return;
+  float* lds = stage[threadIdx.x >> 6];
+  const BatchItem it = items[w];
+  const DProgram<float>* prog = progs + it.pidx;
+  auto pick = [&](int a) { return a == kArrIn ? const_cast<float*>(uin) : (a == kArrOut ? uout : scr); };
+  float* a0 = pick(it.arr[0]) + it.off[0];
+  float* a1 = pick(it.arr[1]) + it.off[1];
+  float* a2 = pick(it.arr[2]) + it.off[2];
+  switch (it.wavelet) {
+#define LFX_WAVELET(nm, idv, w)                           \
+  case idv:                                               \
+    row_item<wt::WL_##nm, TF>(it, prog, a0, a1, a2, lds); \
+    return;
+#define LFX_GENERIC(...)
+#define LFX_UNIT(...)
+#define LFX_REPEAT(...)
+#define LFX_SCALE(...)
+#define LFX_END(nm)
+#include "wavelet_table.inc"
+#undef LFX_WAVELET
+#undef LFX_GENERIC
+#undef LFX_UNIT
+#undef LFX_REPEAT
+#undef LFX_SCALE
+#undef LFX_END
+    default:
+      return;
+  }
+}
+
+template <int TF>
+hipError_t launch_rows_tf(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
+                          void* scr, hipStream_t st) {
+  if (nitems <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)((nitems + 3) / 4);
+  hipLaunchKernelGGL((k_reg_rows<TF>), dim3(blocks), dim3(256), 0, st, items, nitems, (const DProgram<float>*)progs,
+                     (const float*)in, (float*)out, (float*)scr);
+  return hipGetLastError();
+}
+
 template <int TF>
 hipError_t launch_reg_tf(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
                          void* scr, hipStream_t st) {

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../lifted_amd/csrc/lfx_bodies.hpp"
+#include "../../lifted_amd/csrc/lfx_reg.hpp"
 #include "../../lifted_amd/csrc/lfx_schedule.hpp"
 #include "../../lifted_amd/csrc/lfx_table.hpp"
 
@@ -29,6 +30,10 @@ bool sim_reg_tf0(uint32_t, const DProgram<float>&, const ItemSpec&, const float*
 bool sim_reg_tf1(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*);
 bool sim_reg_tf2(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*);
 bool sim_reg_tf3(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*);
+bool sim_rows_tf0(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*, const reg::RowArgs&);
+bool sim_rows_tf1(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*, const reg::RowArgs&);
+bool sim_rows_tf2(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*, const reg::RowArgs&);
+bool sim_rows_tf3(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*, const reg::RowArgs&);
 }  // namespace lfx
 
 template <class C>
@@ -142,6 +147,124 @@ static void run(const WaveletDefL& w, bool use_reg, int tf, int bc, long long n,
   }
 }
 
+// R equal lines (float32) through the same schedule; register items that hold
+// a whole band of at most 512 samples under the Zero boundary run as row items
+// (up to four lines per wavefront), like a plan of many equal lines.
+static void run_rows(const WaveletDefL& w, int tf, int bc, long long n, long long L, long long limit, long long tile,
+                     int k, int R, const std::vector<float>& in, std::vector<float>& out) {
+  const DProgram<float> prog = resolve<float>(w, tf);
+  const bool fwd = tf == kForward || tf == kInverseAdjoint;
+  const LineSchedule sc = schedule(prog, fwd, n, L, kF32, bc, tile, k, limit);
+  std::fprintf(stderr, "rows=%d passes=%zu m_line=%lld L_line=%lld\n", R, sc.passes.size(), sc.m_line, sc.L_line);
+  auto rows_call = [&](const ItemSpec& it, const float* a0, float* a1, float* a2, long long rs0, long long rs1,
+                       long long rs2, int nr) {
+    const reg::RowArgs ra{{rs0, rs1, rs2}, nr};
+    bool ok = false;
+    switch (tf) {
+      case 0: ok = sim_rows_tf0(w.id, prog, it, a0, a1, a2, ra); break;
+      case 1: ok = sim_rows_tf1(w.id, prog, it, a0, a1, a2, ra); break;
+      case 2: ok = sim_rows_tf2(w.id, prog, it, a0, a1, a2, ra); break;
+      case 3: ok = sim_rows_tf3(w.id, prog, it, a0, a1, a2, ra); break;
+    }
+    if (!ok) fail(LFX_EINVAL, "no row kernel for this wavelet");
+  };
+  auto row_ok = [&](const ItemSpec& it) {
+    return it.kind == kSpecReg && bc == kZero && it.a == 0 && it.b == it.m0 && it.m0 <= kRegWindow / 4;
+  };
+  auto line_run = [&](bool f, long long m, long long Ll, const float* src, const float* ssrc, float* dst) {
+    std::vector<float> lds(line_lds_bytes(m, 4) / 4);
+    if (f) line_fwd_body<float, float, float>(&prog, bc, m, (int)Ll, src, 1, dst, 1, lds.data(), lds.data() + m);
+    else line_rev_body<float, float, float, float>(&prog, bc, m, (int)Ll, src, 1, ssrc, 1, dst, 1, lds.data(),
+                                                   lds.data() + m);
+  };
+  if (fwd) {
+    std::vector<float> cur(in);
+    long long cs = n;  // row stride of cur
+    for (size_t i = 0; i < sc.passes.size(); ++i) {
+      const PassPlan& pp = sc.passes[i];
+      const bool last = (i + 1 == sc.passes.size()) && sc.L_line == 0;
+      const long long mk = band_len(pp.m0, pp.k);
+      std::vector<float> sk((size_t)R * mk);
+      const long long ss = last ? n : mk;
+      for (const ItemSpec& it : pass_items(prog, true, pp, true, kF32, bc)) {
+        if (row_ok(it)) {
+          std::fprintf(stderr, "  rows [0, %lld) k=%d\n", it.m0, it.k);
+          for (int r0 = 0; r0 < R; r0 += 4)
+            rows_call(it, cur.data() + r0 * cs, out.data() + r0 * n, (last ? out.data() + r0 * n : sk.data() + r0 * mk),
+                      cs, n, ss, std::min(4, R - r0));
+          continue;
+        }
+        for (int r = 0; r < R; ++r) {
+          float* skp = last ? out.data() + (size_t)r * n : sk.data() + (size_t)r * mk;
+          const float* cp = cur.data() + (size_t)r * cs;
+          float* op = out.data() + (size_t)r * n;
+          if (it.kind == kSpecReg) {
+            reg_dispatch<float>(tf, w.id, prog, it, cp, op, skp);
+            continue;
+          }
+          const TileArgs ta{pp.m0, it.b - it.a, it.H, 0, pp.k, 0};
+          std::vector<float> lds(tile_lds_bytes(ta, true, 4) / 4);
+          tile_fwd_body<float, float, float, float>(&prog, bc, pp.m0, pp.k, it.a, it.b, it.H, cp, 1, op, 1, skp, 1,
+                                                    lds.data(), lds.data() + tile_fwd_window(ta));
+        }
+      }
+      cur = sk;
+      cs = mk;
+    }
+    if (sc.L_line > 0)
+      for (int r = 0; r < R; ++r)
+        line_run(true, sc.m_line, sc.L_line, cur.data() + (size_t)r * cs, cur.data() + (size_t)r * cs,
+                 out.data() + (size_t)r * n);
+    return;
+  }
+  // reverse: s of the coarsest part, then passes coarse -> fine
+  std::vector<float> s;
+  long long sst = n;  // row stride of s
+  bool s_user = true;
+  if (sc.L_line > 0) {
+    s.resize((size_t)R * sc.m_line);
+    for (int r = 0; r < R; ++r)
+      line_run(false, sc.m_line, sc.L_line, in.data() + (size_t)r * n, in.data() + (size_t)r * n,
+               s.data() + (size_t)r * sc.m_line);
+    s_user = false;
+    sst = sc.m_line;
+  }
+  for (size_t j = sc.passes.size(); j-- > 0;) {
+    const PassPlan& pp = sc.passes[j];
+    std::vector<float> band(j == 0 ? 0 : (size_t)R * pp.m0);
+    float* dst = j == 0 ? out.data() : band.data();
+    const long long ds = j == 0 ? n : pp.m0;
+    const float* sp = s_user ? in.data() : s.data();
+    for (const ItemSpec& it : pass_items(prog, false, pp, true, kF32, bc)) {
+      if (row_ok(it)) {
+        std::fprintf(stderr, "  rows [0, %lld) k=%d\n", it.m0, it.k);
+        for (int r0 = 0; r0 < R; r0 += 4)
+          rows_call(it, in.data() + r0 * n, dst + r0 * ds, const_cast<float*>(sp) + r0 * sst, n, ds, sst,
+                    std::min(4, R - r0));
+        continue;
+      }
+      for (int r = 0; r < R; ++r) {
+        const float* ip = in.data() + (size_t)r * n;
+        const float* spr = sp + (size_t)r * sst;
+        float* dp = dst + (size_t)r * ds;
+        if (it.kind == kSpecReg) {
+          reg_dispatch<float>(tf, w.id, prog, it, ip, dp, const_cast<float*>(spr));
+          continue;
+        }
+        const TileArgs ta{pp.m0, it.b - it.a, it.H, 0, pp.k, 0};
+        std::vector<float> lds(tile_lds_bytes(ta, false, 4) / 4);
+        tile_rev_body<float, float, float, float>(&prog, bc, pp.m0, pp.k, it.a, it.b, it.H, ip, 1, spr, 1, dp, 1,
+                                                  lds.data(), lds.data() + tile_rev_window(ta));
+      }
+    }
+    if (j > 0) {
+      s = band;
+      s_user = false;
+      sst = pp.m0;
+    }
+  }
+}
+
 int main(int argc, char** argv) {
   if (argc < 10) {
     std::fprintf(stderr, "usage: lfx_sim wavelet tf bc dtype n L limit tile k [steps]\n");
@@ -162,6 +285,13 @@ int main(int argc, char** argv) {
       if (std::fread(in.data(), 8, n, stdin) != (size_t)n) return 3;
       run<double>(w, false, tf, bc, n, L, limit, tile, k, in, out);
       std::fwrite(out.data(), 8, n, stdout);
+    } else if (const char* rv = std::getenv("LFX_SIM_ROWS")) {
+      // R equal lines back to back on stdin (float32, table wavelets)
+      const int R = std::atoi(rv);
+      std::vector<float> in((size_t)R * n), out((size_t)R * n, 0.0f);
+      if (std::fread(in.data(), 4, in.size(), stdin) != in.size()) return 3;
+      run_rows(w, tf, bc, n, L, limit, tile, k, R, in, out);
+      std::fwrite(out.data(), 4, out.size(), stdout);
     } else {
       std::vector<float> in(n), out(n, 0.0f);
       if (std::fread(in.data(), 4, n, stdin) != (size_t)n) return 3;

@@ -221,3 +221,37 @@ def test_register_tiles_fewer_levels(sim, oracle, w):
             assert "reg [" in info, info
         cases.append(_case(sim, oracle, w, tf, 3, x, L, 600, 8192, 4, f"w={w} L={L} tf={tf}", check=has_reg))
     run_cases(cases)
+
+
+def run_sim_rows(sim, w, tf, x2d, L, limit, tile, k):
+    """R equal lines through the simulator's row mode (float32)."""
+    R, n = x2d.shape
+    args = [sim, str(w), str(tf), "0", "0", str(n), str(L), str(limit), str(tile), str(k)]
+    env = dict(os.environ, LFX_SIM_ROWS=str(R))
+    r = subprocess.run(args, input=np.ascontiguousarray(x2d).tobytes(), capture_output=True, env=env)
+    assert r.returncode == 0, r.stderr.decode()[-3000:]
+    return np.frombuffer(r.stdout, dtype=np.float32).reshape(R, n), r.stderr.decode()
+
+
+@pytest.mark.parametrize("w", SIM_WAVELETS)
+def test_row_mode_equal_short_lines(sim, oracle, w):
+    """Row mode (four equal lines per wavefront, one per 16-lane row, reads
+    past a row's ends are zeros): whole bands of at most 512 samples, Zero
+    boundary, groups of four lines and a partial group, bit-exact per line."""
+    g = np.random.default_rng(w + 17)
+    cases = []
+    for tf, n, R in itertools.product((0, 1, 2, 3), (512, 333, 100), (6,)):
+        L = min(6, oracle.max_level(w, n))
+        if L < 1:
+            continue
+        x = g.standard_normal((R, n)).astype(np.float32)
+
+        def case(tf=tf, n=n, x=x, L=L):
+            got, info = run_sim_rows(sim, w, tf, x, L, limit=32, tile=8192, k=4)
+            for r in range(x.shape[0]):
+                ref = oracle.lwt(x[r], w, tf, 0, level=[L])
+                np.testing.assert_array_equal(got[r], ref, err_msg=f"w={w} tf={tf} n={n} L={L} line {r}\n{info}")
+            if n >= 100 and w in (2, 100907):
+                assert "rows [0," in info, info
+        cases.append(case)
+    run_cases(cases)

@@ -224,3 +224,23 @@ def test_cpp_dropin_header_round_trips(gpu):
     r = subprocess.run([os.path.join(d, "build", "shim_main"), "all"], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("shape,axes,levels", [((37, 300), (1, 0), (3, 4)), ((64, 48, 20), (0, 2, 1), (4, 3, 2)),
+                                               ((64, 64, 64), (0, 1, 2), (4, 4, 4)), ((9, 512), (1,), (4,))])
+def test_nd_float32_short_lines(gpu, oracle, shape, axes, levels):
+    """float32 ND arrays whose lines run as batched plans of equal lines; with
+    the Zero boundary, bands of at most 512 samples run four lines per
+    wavefront (row items). Separable and level-by-level, all transforms,
+    against the oracle (4 ulp of max |coefficient|)."""
+    import lifted_amd as la
+    g = rng(sum(shape))
+    x = g.standard_normal(shape).astype(np.float32)
+    for (w, bc), tf, sep in itertools.product(((100907, 0), (10, 0), (10204, 0), (2, 3)), (F, I, FA, IA),
+                                              (True, False)):
+        lvl = list(levels) if sep else min(levels)
+        if any(not _ok_bands(bc, shape[a], max(levels)) for a in axes):
+            continue
+        ref = oracle.lwt(x, w, tf, bc, axes=list(axes), level=lvl)
+        got = la.lwt(_gpu(x, gpu), w, tf, mode=bc, axes=list(axes), level=lvl).cpu().numpy()
+        _check(got, ref, np.float32, f"shape={shape} axes={axes} w={w} bc={bc} tf={tf} sep={sep}")
