@@ -19,6 +19,8 @@ this host.
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-plain] [--no-traffic]
 
 Secondary workloads (one JSON line each; not the headline):
+    --workload c1   one float64 signal of 65536 samples, Daubechies2, 8 levels, forward
+                    then inverse (latency-bound), with the CPU restatement beside it
     --workload c2   4096^2 float32 sum of Gaussians, CDF 9/7, 5 levels, F / I / FA,
                     separable and level-by-level, fused and plain
     --workload c3   512^3 float32, CDF 9/7 levels (4, 4, 4): forward, soft-threshold
@@ -270,6 +272,39 @@ def _time_calls(fn, steps, warmup):
     return a.elapsed_time(b) / steps
 
 
+def workload_c1(steps, warmup, cpu):
+    """One float64 signal of 65536 N(0,1) samples, Daubechies2, 8 levels,
+    Zero boundary: forward then inverse (the reference's CPU-runnable case);
+    latency-bound on a GPU."""
+    import torch
+    import lifted_amd as la
+    x_np = np.random.default_rng(20261201).standard_normal(65536)
+    x = torch.tensor(x_np, device="cuda")
+    y = torch.empty_like(x)
+    z = torch.empty_like(x)
+
+    def fi():
+        la.lwt_forward(x, la.Wavelet.Daubechies2, 0, level=[8], out=y)
+        la.lwt_inverse(y, la.Wavelet.Daubechies2, 0, level=[8], out=z)
+
+    ms = _time_calls(fi, steps * 10, warmup)
+    res = {"workload": "c1: 1-D float64 N(0,1), n = 65536, Daubechies2, 8 levels, Zero BC, forward then inverse",
+           "ms_F_plus_I": round(ms, 4), "Gsamples_per_s": round(2 * 65536 / (ms * 1e-3) / 1e9, 3),
+           "reconstruction_max_abs_err": float((z - x).abs().max())}
+    if cpu:
+        import oracle as O
+        reps = 200
+        t = time.perf_counter()
+        for _ in range(reps):
+            yc = O.lwt(x_np, 2, O.FORWARD, O.ZERO, level=[8])
+            O.lwt(yc, 2, O.INVERSE, O.ZERO, level=[8])
+        dt = (time.perf_counter() - t) / reps
+        res["cpu_baseline"] = {"ms": round(dt * 1e3, 4), "cores": 1, "kind": "port",
+                               "sample": f"forward + inverse, mean of {reps} runs (a single line runs on one "
+                                         "thread in the reference)"}
+    return res
+
+
 def workload_c2(steps, warmup, cpu):
     """4096^2 float32, sum of 32 Gaussians + 0.01 N(0,1) noise; CDF 9/7, 5 levels."""
     import torch
@@ -397,8 +432,8 @@ def main():
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a device copy of the workload's input size (HBM counter calibration)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--workload", choices=["c4", "c2", "c3", "c5"], default="c4",
-                    help="c4 (default): the batched headline; c2 / c3 / c5: secondary workloads")
+    ap.add_argument("--workload", choices=["c4", "c1", "c2", "c3", "c5"], default="c4",
+                    help="c4 (default): the batched headline; c1 / c2 / c3 / c5: secondary workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); this build has no multi-GPU path: "
@@ -412,7 +447,8 @@ def main():
         import torch
         torch.cuda.set_device(0)
         steps = min(args.steps, 10)
-        fn = {"c2": lambda: workload_c2(steps, 2, not args.no_cpu), "c3": lambda: workload_c3(steps, 2),
+        fn = {"c1": lambda: workload_c1(steps, 2, not args.no_cpu),
+              "c2": lambda: workload_c2(steps, 2, not args.no_cpu), "c3": lambda: workload_c3(steps, 2),
               "c5": lambda: workload_c5(steps, 1)}[args.workload]
         print(json.dumps(fn()))
         return

@@ -244,3 +244,49 @@ def test_nd_float32_short_lines(gpu, oracle, shape, axes, levels):
         ref = oracle.lwt(x, w, tf, bc, axes=list(axes), level=lvl)
         got = la.lwt(_gpu(x, gpu), w, tf, mode=bc, axes=list(axes), level=lvl).cpu().numpy()
         _check(got, ref, np.float32, f"shape={shape} axes={axes} w={w} bc={bc} tf={tf} sep={sep}")
+
+
+def test_layouts_float32_batched_lines(gpu, oracle):
+    """float32 layouts through the batched-lines path: F-order, strided views
+    (in and out), a negative stride, in place, separable and level-by-level,
+    a size-1 dim, and 16-bit storage in F-order; all against the oracle."""
+    import torch
+    import lifted_amd as la
+    x = rng(12).standard_normal((40, 52, 3)).astype(np.float32)
+    xt = torch.from_numpy(x).to(gpu)
+    for (w, bc), tf in itertools.product(((100907, 0), (3, 3)), (F, I, FA, IA)):
+        for lvl, axes in (([2, 3], [0, 1]), (2, [1, 0]), ([1], [2])):
+            ref = oracle.lwt(x, w, tf, bc, axes=axes, level=lvl)
+            what = f"w={w} bc={bc} tf={tf} axes={axes} level={lvl}"
+            # Fortran-ordered in and out
+            xf = xt.permute(2, 1, 0).contiguous().permute(2, 1, 0)
+            got = la.lwt(xf, w, tf, mode=bc, axes=axes, level=lvl)
+            assert got.stride() == xf.stride()
+            _check(got.cpu().numpy(), ref, np.float32, "F-order " + what)
+            # strided view in, strided view out
+            big = torch.zeros((80, 104, 3), dtype=torch.float32, device=gpu)
+            big[::2, ::2] = xt
+            outb = torch.zeros((40, 156, 3), dtype=torch.float32, device=gpu)
+            got = la.lwt(big[::2, ::2], w, tf, mode=bc, axes=axes, level=lvl, out=outb[:, ::3])
+            _check(got.cpu().numpy(), ref, np.float32, "strided " + what)
+            # negative stride along the first axis (a numpy view: host path)
+            neg = np.ascontiguousarray(x[::-1])[::-1]
+            assert neg.strides[0] < 0
+            got = la.lwt(neg, w, tf, mode=bc, axes=axes, level=lvl)
+            _check(got, ref, np.float32, "negative stride " + what)
+            # in place
+            xi = xt.clone()
+            la.lwt(xi, w, tf, mode=bc, axes=axes, level=lvl, out=xi)
+            _check(xi.cpu().numpy(), ref, np.float32, "in place " + what)
+    # size-1 dimension and 16-bit storage in F-order
+    y = rng(13).standard_normal((1, 300, 5)).astype(np.float32)
+    for tf in (F, FA):
+        ref = oracle.lwt(y, 100907, tf, 0, axes=[1, 0], level=[3, 0])
+        got = la.lwt(torch.from_numpy(y).to(gpu), 100907, tf, axes=[1, 0], level=[3, 0]).cpu().numpy()
+        _check(got, ref, np.float32, f"size-1 dim tf={tf}")
+        h = torch.from_numpy(y).to(torch.bfloat16)
+        y16 = h.float().numpy()
+        ref16 = torch.from_numpy(oracle.lwt(y16, 100907, tf, 0, axes=[1], level=[3])).to(torch.bfloat16).float()
+        hf = h.to(gpu).permute(2, 1, 0).contiguous().permute(2, 1, 0)
+        got16 = la.lwt(hf, 100907, tf, axes=[1], level=[3]).float().cpu()
+        assert torch.equal(got16, ref16), f"bf16 F-order tf={tf}"
