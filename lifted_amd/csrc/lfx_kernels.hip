@@ -112,8 +112,27 @@ __global__ void __launch_bounds__(kThreads) k_tile_rev(const DProgram<C>* prog, 
 
 // Strided elementwise copy over an ND box (used by the level-by-level inverse,
 // which copies in -> out before transforming in place, drivers-inl.hpp:542-567).
+// Rows along the last dim of at least 256 elements: a workgroup per row
+// (grid-stride over rows), the row's offsets computed once; otherwise one
+// element per thread with the full index decomposition.
 template <class S>
 __global__ void k_copy_nd(const S* src, S* dst, CopyMap cm, long long total) {
+  const long long inner = cm.shape[cm.nd - 1];
+  if (inner >= 256) {
+    const long long rows = total / inner;
+    const long long si = cm.sst[cm.nd - 1], di = cm.dstr[cm.nd - 1];
+    for (long long row = blockIdx.x; row < rows; row += gridDim.x) {
+      long long r = row, so = 0, dof = 0;
+      for (int k = cm.nd - 2; k >= 0; --k) {
+        const long long q = r % cm.shape[k];
+        r /= cm.shape[k];
+        so += q * cm.sst[k];
+        dof += q * cm.dstr[k];
+      }
+      for (long long i = threadIdx.x; i < inner; i += blockDim.x) dst[dof + i * di] = src[so + i * si];
+    }
+    return;
+  }
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += (long long)gridDim.x * blockDim.x) {
     long long r = e, so = 0, dof = 0;
@@ -163,15 +182,33 @@ __global__ void __launch_bounds__(256) k_permute(const SI* __restrict__ src, SO*
   }
 }
 
-// fi == fo: both sides contiguous along dim i
+// fi == fo: both sides run along dim i. Lines of at least 256 elements: a
+// workgroup per line (grid-stride over lines), the batch offsets computed once
+// per line and the threads striding along it. Shorter lines: one element per
+// thread over the flattened (line, i) space.
 template <class SI, class SO, class C>
 __global__ void __launch_bounds__(256) k_convert(const SI* __restrict__ src, SO* __restrict__ dst, PermMap pm) {
-  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= pm.ni) return;
-  for (long long b = blockIdx.y; b < pm.nbatch; b += gridDim.y) {
-    long long os, od;
-    perm_batch(pm, b, os, od);
-    dev::store_as<SO, C>(dst, od + i * pm.so_i, dev::load_as<SI, C>(src, os + i * pm.si_i));
+  if (pm.ni >= 256) {
+    for (long long b = blockIdx.x; b < pm.nbatch; b += gridDim.x) {
+      long long os, od;
+      perm_batch(pm, b, os, od);
+      const SI* s = src + os;
+      SO* d = dst + od;
+      if (pm.si_i == 1 && pm.so_i == 1) {
+        for (long long i = threadIdx.x; i < pm.ni; i += 256) dev::store_as<SO, C>(d, i, dev::load_as<SI, C>(s, i));
+      } else {
+        for (long long i = threadIdx.x; i < pm.ni; i += 256)
+          dev::store_as<SO, C>(d, i * pm.so_i, dev::load_as<SI, C>(s, i * pm.si_i));
+      }
+    }
+  } else {
+    const long long total = pm.ni * pm.nbatch;
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
+      const long long b = e / pm.ni, i = e - b * pm.ni;
+      long long os, od;
+      perm_batch(pm, b, os, od);
+      dev::store_as<SO, C>(dst, od + i * pm.so_i, dev::load_as<SI, C>(src, os + i * pm.si_i));
+    }
   }
 }
 
@@ -179,8 +216,10 @@ template <class SI, class SO, class C>
 static hipError_t permute_t(const void* src, void* dst, const PermMap& pm, hipStream_t st) {
   const unsigned z = (unsigned)std::min<long long>(pm.nbatch, 65535);
   if (pm.same) {
-    hipLaunchKernelGGL((k_convert<SI, SO, C>), dim3((unsigned)((pm.ni + 255) / 256), z), dim3(256), 0, st,
-                       reinterpret_cast<const SI*>(src), reinterpret_cast<SO*>(dst), pm);
+    const long long work = pm.ni >= 256 ? pm.nbatch : (pm.ni * pm.nbatch + 255) / 256;
+    const unsigned blocks = (unsigned)std::min<long long>(work, 1 << 20);
+    hipLaunchKernelGGL((k_convert<SI, SO, C>), dim3(blocks), dim3(256), 0, st, reinterpret_cast<const SI*>(src),
+                       reinterpret_cast<SO*>(dst), pm);
   } else {
     hipLaunchKernelGGL((k_permute<SI, SO, C>), dim3((unsigned)((pm.ni + 63) / 64), (unsigned)((pm.no + 63) / 64), z),
                        dim3(256), 0, st, reinterpret_cast<const SI*>(src), reinterpret_cast<SO*>(dst), pm);
