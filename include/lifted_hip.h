@@ -174,6 +174,14 @@ typedef struct {
 int lfx_set_tuning_ex(const lfx_tuning* t);
 int lfx_get_tuning(lfx_tuning* t);
 
+/* Streaming-copy reference for measurements (no counterpart in the
+ * reference): copies `bytes` (rounded down to the pattern's unit) from src to
+ * dst, stream-ordered. pattern 0: one float4 per thread; pattern 1: one
+ * 2048-float window per wavefront, read and written as 32 coalesced rows of
+ * 64 floats (the register tiles' access pattern without the arithmetic).
+ * Buffers must be 16-byte aligned and must not overlap. */
+int lfx_stream_copy(const void* src_dev, void* dst_dev, int64_t bytes, int32_t pattern, void* stream);
+
 /* Message of the last error on this thread. */
 const char* lfx_last_error(void);
 

@@ -1419,6 +1419,15 @@ static void check_tuning(long long tile, int levels_per_pass) {
   if (tile > 0 && (tile < 64 || tile > (1 << 16))) fail(LFX_EINVAL, "tile must be in [64, 65536]");
 }
 
+int lfx_stream_copy(const void* src_dev, void* dst_dev, int64_t bytes, int32_t pattern, void* stream) {
+  return guarded([&] {
+    if (pattern < 0 || pattern > 1) fail(LFX_EINVAL, "pattern must be 0 or 1");
+    if (bytes < 0 || (bytes > 0 && (!src_dev || !dst_dev))) fail(LFX_EINVAL, "bad copy arguments");
+    if (((uintptr_t)src_dev | (uintptr_t)dst_dev) & 15) fail(LFX_EINVAL, "copy buffers must be 16-byte aligned");
+    hip_check(launch_stream_copy(src_dev, dst_dev, bytes, pattern, (hipStream_t)stream), "copy kernel");
+  });
+}
+
 int lfx_set_tuning(int64_t tile, int32_t levels_per_pass) {
   return guarded([&] {
     check_tuning(tile, levels_per_pass);

@@ -498,6 +498,42 @@ hipError_t launch_batch(int storage, bool fwd, int bc, const BatchItem* items, l
 #undef CALL_B
 }
 
+// Streaming-copy references for the benchmark (no transform): one float4 per
+// thread, and one 2048-float window per wavefront read as 32 coalesced rows of
+// 64 floats and written back the same way (the register tiles' access pattern
+// without the arithmetic).
+__global__ void __launch_bounds__(256) k_copy_f4(const float4* __restrict__ a, float4* __restrict__ b, long long n4) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n4) b[i] = a[i];
+}
+__global__ void __launch_bounds__(256) k_copy_win(const float* __restrict__ a, float* __restrict__ b, long long nwin) {
+  const long long w = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int l = (int)(threadIdx.x & 63);
+  if (w >= nwin) return;
+  const float* p = a + w * 2048;
+  float t[32];
+#pragma unroll
+  for (int j = 0; j < 32; ++j) t[j] = p[j * 64 + l];
+  float* q = b + w * 2048;
+#pragma unroll
+  for (int j = 0; j < 32; ++j) q[j * 64 + l] = t[j];
+}
+
+hipError_t launch_stream_copy(const void* src, void* dst, long long bytes, int pattern, hipStream_t st) {
+  if (pattern == 0) {
+    const long long n4 = bytes / 16;
+    if (n4 <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_f4, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, st, (const float4*)src,
+                       (float4*)dst, n4);
+  } else {
+    const long long nwin = bytes / (2048 * 4);
+    if (nwin <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_win, dim3((unsigned)((nwin + 3) / 4)), dim3(256), 0, st, (const float*)src, (float*)dst,
+                       nwin);
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_copy(int storage, const void* src, void* dst, const CopyMap& cm, long long total,
                        hipStream_t st) {
   if (total <= 0) return hipSuccess;

@@ -52,6 +52,9 @@ inline hipError_t launch_rows(int tf, const void* progs, const BatchItem* items,
   }
   return hipErrorInvalidValue;
 }
+// Streaming-copy references (pattern 0: float4 per thread; 1: 2048-float
+// window per wavefront); copies bytes rounded down to the pattern's unit.
+hipError_t launch_stream_copy(const void* src, void* dst, long long bytes, int pattern, hipStream_t st);
 hipError_t launch_copy(int storage, const void* src, void* dst, const CopyMap& cm, long long total,
                        hipStream_t st);
 // Box copy between storage types (kF32 <-> kBF16 / kF16, or same type f32 /

@@ -128,3 +128,23 @@ def test_autotuned_plans_match_oracle(gpu, oracle):
                 ref = oracle.lwt(xn[off:off + n], w, tf, 0, level=[L])
                 scale = max(np.abs(ref).max(), 1e-30)
                 assert np.abs(y[off:off + n] - ref).max() <= 4 * np.finfo(np.float32).eps * scale, (knobs, tf, w)
+
+
+def test_stream_copy_reference_is_exact(gpu):
+    """lfx_stream_copy (the benchmark's streaming-rate reference): both access
+    patterns copy the rounded-down byte count exactly and leave the rest."""
+    import ctypes
+    import torch
+    from lifted_amd import _lib
+    L = _lib.lib()
+    L.lfx_stream_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p]
+    n = 3 * 2048 * 5 + 700
+    x = torch.randn(n, device=gpu)
+    for pattern, unit in ((0, 4), (1, 2048)):
+        y = torch.full_like(x, -7.0)
+        _lib.check(L.lfx_stream_copy(x.data_ptr(), y.data_ptr(), n * 4, pattern, None))
+        torch.cuda.synchronize()
+        m = n // unit * unit
+        assert torch.equal(y[:m], x[:m]) and bool((y[m:] == -7.0).all()), pattern
+    with pytest.raises(_lib.LfxError):
+        _lib.check(L.lfx_stream_copy(x.data_ptr(), y.data_ptr(), 16, 2, None))
