@@ -2,19 +2,27 @@
 // levels in registers; LDS serves only to transpose global loads and stores
 // into coalesced ones (no workgroup barriers).
 //
-// Each of the 64 lanes holds 16 consecutive samples of the tile's window
-// (1024 samples). Level 1 splits them into 8 s and 8 d values per lane
-// (s[e] = x[2e], d[e] = x[2e+1]); level l + 1 splits s_l the same way, so after
-// four levels each lane holds one s_4 and one d_4. A lifting step
+// Each of the 64 lanes holds kPerLane (32) consecutive samples of the tile's
+// window (kWindow = 2048 samples). Level 1 splits them into 16 s and 16 d
+// values per lane (s[e] = x[2e], d[e] = x[2e+1]); level l + 1 splits s_l the
+// same way, so after four levels each lane holds two s_4 and two d_4. A
+// lifting step
 //     y[e] += sum_k c[k] * x[e + off + k]
 // reads x values past the lane's own chunk from neighbouring lanes (DPP shifts
 // by whole chunks). The step shapes (kind, band, offset, taps) are compile-time
 // (lfx_wavelet_types.hpp); the coefficients come from the runtime program.
 // The window's outer lanes compute values that depend on samples outside the
 // window; the halo is sized on the host (lfx_tiles.hpp rules) so that only
-// values owned by the tile -- all exact -- are stored. Tiles never touch a
-// band end, so no boundary map or adjoint fold is involved; the parts of a
-// band next to its ends run through the LDS tile bodies.
+// values owned by the tile -- all exact -- are stored. A window inside a band
+// applies no boundary map or adjoint fold; under the Zero boundary a window
+// may overlap a band end and runs masked (the band is zeroed outside its range
+// after every step, which is what skipping out-of-range terms computes).
+// Other boundaries keep the parts of a band next to its ends in the LDS tile
+// bodies.
+//
+// Row mode (lane policies with ROWS = 4) holds four equal lines of at most 512
+// samples, one per 16-lane row, each from band position 0; reads past a row's
+// ends yield zeros, so whole bands need no halo (Zero boundary).
 //
 // The lane policy P abstracts the wavefront: on the GPU a value is one float
 // per lane and cross-lane reads are lane shifts; in the CPU simulation a value is
