@@ -1,5 +1,7 @@
 """Batched plans (many packed 1-D signals of uneven length / wavelet / level
 count) against the oracle, per signal, fused and plain (one level per stage)."""
+import itertools
+
 import numpy as np
 import pytest
 
@@ -148,3 +150,51 @@ def test_stream_copy_reference_is_exact(gpu):
         assert torch.equal(y[:m], x[:m]) and bool((y[m:] == -7.0).all()), pattern
     with pytest.raises(_lib.LfxError):
         _lib.check(L.lfx_stream_copy(x.data_ptr(), y.data_ptr(), 16, 2, None))
+
+
+@pytest.mark.parametrize("storage", ["bfloat16", "float16"])
+def test_batch_16bit_storage(gpu, oracle, storage):
+    """16-bit storage, fp32 arithmetic: intermediate s bands stay fp32 in the
+    plan's scratch, so every output equals the fp32 oracle on the 16-bit input,
+    rounded once to the storage type (bit for bit)."""
+    import torch
+    from lifted_amd.batch import BatchPlan
+    tdt = getattr(torch, storage)
+    sigs, total = make_signals(31, 24, 100, 120000, oracle)
+    x = torch.from_numpy(np.random.default_rng(8).standard_normal(total).astype(np.float32)).to(tdt)
+    x32 = x.float().numpy()
+    for tf in (0, 1, 2, 3):
+        y = BatchPlan(sigs, tf, dtype=tdt).execute(x.to(gpu)).float().cpu()
+        for off, n, w, L in sigs:
+            ref = torch.from_numpy(oracle.lwt(x32[off:off + n], w, tf, 0, level=[L])).to(tdt).float()
+            assert torch.equal(y[off:off + n], ref), (storage, tf, off, n, w, L)
+
+
+def test_batch_custom_programs_periodic_and_max_level(gpu, oracle):
+    """Caller-defined step programs in a batch, the Periodic boundary on lines
+    that fit one workgroup (longer ones are refused), and levels = 0 (max
+    level), against the oracle."""
+    import torch
+    import lifted_amd as la
+    from helpers import MOCK_WAVELETS, oracle_steps
+    from lifted_amd.batch import BatchPlan
+    cw = la.CustomWavelet(MOCK_WAVELETS["MultiStep"], width=8)
+    st = oracle_steps(MOCK_WAVELETS["MultiStep"])
+    g = np.random.default_rng(9)
+    lens = [37, 300, 4000, 1000, 129]
+    sigs, off = [], 0
+    for i, n in enumerate(lens):
+        sigs.append((off, n, cw.id if i % 2 else 100907, 0 if i == 2 else 2))
+        off += n
+    x = g.standard_normal(off).astype(np.float32)
+    for bc, tf in itertools.product((2, 0, 4), (0, 2)):
+        y = BatchPlan(sigs, tf, mode=bc, dtype=torch.float32).execute(torch.from_numpy(x).to(gpu)).cpu().numpy()
+        for (o, n, w, L) in sigs:
+            if w == cw.id:
+                ref = oracle.lwt(x[o:o + n], 0, tf, bc, level=[L], steps=st, width=8)
+            else:
+                ref = oracle.lwt(x[o:o + n], w, tf, bc, level=[L if L else oracle.max_level(w, n)])
+            scale = max(np.abs(ref).max(), 1e-30)
+            assert np.abs(y[o:o + n] - ref).max() <= 4 * np.finfo(np.float32).eps * scale, (bc, tf, n, w, L)
+    with pytest.raises(la.LfxError):
+        BatchPlan([(0, 300000, 100907, 3)], 0, mode=2, dtype=torch.float32)
