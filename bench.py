@@ -155,6 +155,31 @@ def dominant_kernel(plans, launches):
             "ms_per_step": d["ms"]}
 
 
+def time_graph_step(plans, x, bufs, steps, warmup):
+    """ms per step of the plans captured once in a hipGraph and replayed."""
+    import torch
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            src = x
+            for p, b in zip(plans, bufs):
+                p.execute(src, b)
+                src = b
+    torch.cuda.current_stream().wait_stream(s)
+    for _ in range(warmup):
+        g.replay()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        g.replay()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / steps
+
+
 def measure_copies(x, reps=5):
     """GB/s (read + write) of copies of x: torch copy_, and lfx_stream_copy
     patterns 0 (float4 per thread) and 1 (2048-float window per wavefront)."""
@@ -429,6 +454,8 @@ def workload_c5(steps, warmup):
         y = la.lwt_forward(xs, 100907, 0, axes=axes, level=6)
         z = la.lwt_inverse(y, 100907, 0, axes=axes, level=6)
         e_coef = float((y.double() - ref).norm() / ref.norm())
+        e_coef_max = float((y.double() - ref).abs().max())
+        e_coef_max_rel = e_coef_max / float(ref.abs().max())
         e_rt = float((z.double() - xs.double()).norm() / xs.double().norm())
         e_rt64 = float((z.double() - x64).norm() / x64.norm())
         yo = torch.empty_like(xs)
@@ -440,7 +467,8 @@ def workload_c5(steps, warmup):
 
         ms = _time_calls(fi, steps, warmup)
         es = xs.element_size()
-        rows.append({"storage": name, "coef_rel_err_vs_fp64": e_coef, "roundtrip_rel_err": e_rt,
+        rows.append({"storage": name, "coef_rel_err_vs_fp64": e_coef, "coef_max_abs_err_vs_fp64": e_coef_max,
+                     "coef_max_err_rel_to_max_coef": e_coef_max_rel, "roundtrip_rel_err": e_rt,
                      "roundtrip_rel_err_vs_fp64_input": e_rt64, "ms_F_plus_I": round(ms, 3),
                      "GBps_compulsory": round(4 * xs.numel() * es / (ms * 1e-3) / 1e9, 1),
                      "Gsamples_per_s": round(2 * xs.numel() / (ms * 1e-3) / 1e9, 2)})
@@ -522,6 +550,10 @@ def main():
     value = 2 * total / (ms * 1e-3) / 1e9
     print(f"[bench] fused: {ms:.3f} ms/step, {value:.2f} Gsamples/s; dominant {dom['symbol']} "
           f"{dom['avg_launch_ms']:.3f} ms/launch, {achieved / 1e9:.0f} GB/s", file=sys.stderr, flush=True)
+    # the same step captured once as a hipGraph and replayed (no per-launch
+    # events inside: events recorded in a graph cannot be timed here)
+    graph_ms = time_graph_step(plans, x, [y, z], args.steps, args.warmup)
+    print(f"[bench] graph replay: {graph_ms:.3f} ms/step", file=sys.stderr, flush=True)
     # copies of the input size: the achievable streaming rate on this device
     # (torch's copy, and the library's reference copies: one float4 per thread,
     # one 2048-float window per wavefront like the register tiles)
@@ -537,6 +569,8 @@ def main():
         "tuning": {"chosen": best, "candidates": [{"knobs": k, "ms_per_step": round(t, 4)} for k, t in tried],
                    "seconds": round(t_tune, 1)},
         "copy_GBps": copy_gbs,
+        "graph_replay": {"ms_per_step": round(graph_ms, 4), "Gsamples_per_s": round(2 * total / (graph_ms * 1e-3) / 1e9, 3),
+                         "note": "forward + forward-adjoint plans captured once in a hipGraph, replayed K times"},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
                      "kernel": dom["symbol"].rstrip("("), "launches_per_step": dom["launches_per_step"],
