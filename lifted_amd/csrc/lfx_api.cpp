@@ -241,6 +241,7 @@ static thread_local Tuning g_tuning = env_tuning();
 
 static bool lines_fast(const Plan& p, const long long* shp, int ax, long long L, const void* src, const long long* sst,
                        void* dst, const long long* dstr, hipStream_t st);
+static bool sep_chain(const Plan& p, const std::vector<long long>& L, const void* in, void* out, hipStream_t st);
 
 template <class C>
 static void transform_lines_t(const Plan& p, const WaveletDefL& w, const void* prog, bool fwd, const long long* shp,
@@ -398,6 +399,7 @@ static void run_plan(const Plan& p, const void* in, void* out, hipStream_t st) {
       check_bands(p.bc, steps, len, L[ia]);
     }
     if (total == 0) return;
+    if (sep_chain(p, L, in, out, st)) return;
     for (size_t ia = 0; ia < p.axes.size(); ++ia) {
       const void* src = ia == 0 ? in : out;
       const long long* sst = ia == 0 ? p.sin : p.sout;
@@ -1206,6 +1208,94 @@ static bool lines_fast(const Plan& p, const long long* shp, int ax, long long L,
     const int fo = fast_dim(p.ndim, shp, dstr);
     hip_check(launch_permute(kF32, p.storage, ws2, dst, perm_map(p.ndim, shp, ws_str.data(), dstr, ax, fo), st),
               "permute kernel");
+  }
+  return true;
+}
+
+// Separable transforms of several axes through batched plans, chained in two
+// float32 workspaces: each axis transposes its input into the dense layout
+// with that axis contiguous (unless it already is) and runs its plan into the
+// other workspace; only the last axis writes the output (directly when its
+// lines are packed). One transpose and one plan per axis, where the axis-by-
+// axis path pays a transpose in and a transpose out (or an in-place copy) per
+// axis. Results are the same: every axis pass computes in float32 from the
+// previous axis's float32 values (16-bit storage is rounded once, at the end).
+static bool sep_chain(const Plan& p, const std::vector<long long>& L, const void* in, void* out, hipStream_t st) {
+  if (g_tuning.nd_batch == 0 || p.axes.size() < 2) return false;
+  if (p.storage == kF64 || p.bc == kPeriodic || (p.flags & LFX_FLAG_PLAIN) || p.wavelet >= kCustomBase ||
+      p.wavelet == 0)
+    return false;
+  long long total = 1;
+  for (int k = 0; k < p.ndim; ++k) total *= p.shape[k];
+  if (total <= 0) return false;
+  for (size_t ia = 0; ia < p.axes.size(); ++ia) {
+    const long long n = p.shape[p.axes[ia]];
+    if (L[ia] < 1 || n < 2 || total / n > (1ll << 24)) return false;
+  }
+  // dense layout with `ax` contiguous, the other dims in order
+  auto layout = [&](int ax) {
+    std::vector<long long> str(p.ndim);
+    long long acc = p.shape[ax];
+    str[ax] = 1;
+    for (int k = p.ndim - 1; k >= 0; --k) {
+      if (k == ax) continue;
+      str[k] = acc;
+      acc *= p.shape[k];
+    }
+    return str;
+  };
+  auto same = [&](const long long* a, const std::vector<long long>& b) {
+    for (int k = 0; k < p.ndim; ++k)
+      if (p.shape[k] > 1 && a[k] != b[k]) return false;
+    return true;
+  };
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_line_mu);
+  const size_t pad = ((size_t)total * sizeof(float) + 255) / 256 * 256;
+  WsLease lease;
+  lease_workspace(2 * pad, st, lease);
+  float* buf[2] = {(float*)lease.p, (float*)((char*)lease.p + pad)};
+  const void* cur = in;
+  int cur_buf = -1;  // -1: the user input
+  int cur_fast = -1;
+  std::vector<long long> cur_str(p.sin, p.sin + p.ndim);
+  for (size_t ia = 0; ia < p.axes.size(); ++ia) {
+    const int ax = p.axes[ia];
+    const bool last = ia + 1 == p.axes.size();
+    const long long n = p.shape[ax], nlines = total / n;
+    const std::vector<long long> wstr = layout(ax);
+    const bool direct = cur_buf < 0 ? (p.storage == kF32 && same(cur_str.data(), wstr)) : same(cur_str.data(), wstr);
+    const void* pin = cur;
+    int in_buf = cur_buf;
+    if (!direct) {
+      in_buf = cur_buf < 0 ? 0 : 1 - cur_buf;
+      const int fi = cur_buf < 0 ? fast_dim(p.ndim, p.shape, cur_str.data()) : cur_fast;
+      hip_check(launch_permute(cur_buf < 0 ? p.storage : kF32, kF32, cur, buf[in_buf],
+                               perm_map(p.ndim, p.shape, cur_str.data(), wstr.data(), fi, ax), st),
+                "permute kernel");
+      pin = buf[in_buf];
+    }
+    BatchPlanImpl* bp = line_plan(LinePlanKey{dev, nlines, n, L[ia], p.wavelet, p.tf, p.bc});
+    const int out_buf = in_buf < 0 ? 0 : 1 - in_buf;
+    if (last) {
+      // pin is a workspace here (the first axis is never the last)
+      if (p.storage == kF32 && same(p.sout, wstr)) {
+        batch_stage_all(bp, pin, out, st);
+      } else {
+        batch_stage_all(bp, pin, buf[out_buf], st);
+        const int fo = fast_dim(p.ndim, p.shape, p.sout);
+        hip_check(launch_permute(kF32, p.storage, buf[out_buf], out,
+                                 perm_map(p.ndim, p.shape, wstr.data(), p.sout, ax, fo), st),
+                  "permute kernel");
+      }
+    } else {
+      batch_stage_all(bp, pin, buf[out_buf], st);
+      cur = buf[out_buf];
+      cur_buf = out_buf;
+      cur_fast = ax;
+      cur_str = wstr;
+    }
   }
   return true;
 }
