@@ -241,7 +241,9 @@ static thread_local Tuning g_tuning = env_tuning();
 
 static bool lines_fast(const Plan& p, const long long* shp, int ax, long long L, const void* src, const long long* sst,
                        void* dst, const long long* dstr, hipStream_t st);
-static bool sep_chain(const Plan& p, const std::vector<long long>& L, const void* in, void* out, hipStream_t st);
+static bool sep_chain(const Plan& p, const long long* shape, const long long* sin, const long long* sout,
+                      const std::vector<int>& axes, const std::vector<long long>& L, const void* in, void* out,
+                      hipStream_t st);
 
 template <class C>
 static void transform_lines_t(const Plan& p, const WaveletDefL& w, const void* prog, bool fwd, const long long* shp,
@@ -399,7 +401,7 @@ static void run_plan(const Plan& p, const void* in, void* out, hipStream_t st) {
       check_bands(p.bc, steps, len, L[ia]);
     }
     if (total == 0) return;
-    if (sep_chain(p, L, in, out, st)) return;
+    if (sep_chain(p, p.shape, p.sin, p.sout, p.axes, L, in, out, st)) return;
     for (size_t ia = 0; ia < p.axes.size(); ++ia) {
       const void* src = ia == 0 ? in : out;
       const long long* sst = ia == 0 ? p.sin : p.sout;
@@ -442,6 +444,14 @@ static void run_plan(const Plan& p, const void* in, void* out, hipStream_t st) {
   }
   for (long long il = 0; il < L; ++il) {
     const std::vector<long long>& shp = fwd_stepping ? shapes[il] : shapes[L - 1 - il];
+    // one level on every axis of the box, chained through workspaces (the
+    // source is read in full by the first axis before the last axis writes
+    // `out`, so this is safe in place)
+    if (sep_chain(p, shp.data(), sst, p.sout, p.axes, std::vector<long long>(p.axes.size(), 1), src, out, st)) {
+      src = out;
+      sst = p.sout;
+      continue;
+    }
     for (int a : p.axes) {
       transform_lines(p, w, prog, fwd_stepping, shp.data(), a, 1, src, sst, out, p.sout, st);
       src = out;
@@ -1220,33 +1230,35 @@ static bool lines_fast(const Plan& p, const long long* shp, int ax, long long L,
 // axis path pays a transpose in and a transpose out (or an in-place copy) per
 // axis. Results are the same: every axis pass computes in float32 from the
 // previous axis's float32 values (16-bit storage is rounded once, at the end).
-static bool sep_chain(const Plan& p, const std::vector<long long>& L, const void* in, void* out, hipStream_t st) {
-  if (g_tuning.nd_batch == 0 || p.axes.size() < 2) return false;
+static bool sep_chain(const Plan& p, const long long* shape, const long long* sin, const long long* sout,
+                      const std::vector<int>& axes, const std::vector<long long>& L, const void* in, void* out,
+                      hipStream_t st) {
+  if (g_tuning.nd_batch == 0 || axes.size() < 2) return false;
   if (p.storage == kF64 || p.bc == kPeriodic || (p.flags & LFX_FLAG_PLAIN) || p.wavelet >= kCustomBase ||
       p.wavelet == 0)
     return false;
   long long total = 1;
-  for (int k = 0; k < p.ndim; ++k) total *= p.shape[k];
+  for (int k = 0; k < p.ndim; ++k) total *= shape[k];
   if (total <= 0) return false;
-  for (size_t ia = 0; ia < p.axes.size(); ++ia) {
-    const long long n = p.shape[p.axes[ia]];
+  for (size_t ia = 0; ia < axes.size(); ++ia) {
+    const long long n = shape[axes[ia]];
     if (L[ia] < 1 || n < 2 || total / n > (1ll << 24)) return false;
   }
   // dense layout with `ax` contiguous, the other dims in order
   auto layout = [&](int ax) {
     std::vector<long long> str(p.ndim);
-    long long acc = p.shape[ax];
+    long long acc = shape[ax];
     str[ax] = 1;
     for (int k = p.ndim - 1; k >= 0; --k) {
       if (k == ax) continue;
       str[k] = acc;
-      acc *= p.shape[k];
+      acc *= shape[k];
     }
     return str;
   };
   auto same = [&](const long long* a, const std::vector<long long>& b) {
     for (int k = 0; k < p.ndim; ++k)
-      if (p.shape[k] > 1 && a[k] != b[k]) return false;
+      if (shape[k] > 1 && a[k] != b[k]) return false;
     return true;
   };
   int dev = 0;
@@ -1259,20 +1271,20 @@ static bool sep_chain(const Plan& p, const std::vector<long long>& L, const void
   const void* cur = in;
   int cur_buf = -1;  // -1: the user input
   int cur_fast = -1;
-  std::vector<long long> cur_str(p.sin, p.sin + p.ndim);
-  for (size_t ia = 0; ia < p.axes.size(); ++ia) {
-    const int ax = p.axes[ia];
-    const bool last = ia + 1 == p.axes.size();
-    const long long n = p.shape[ax], nlines = total / n;
+  std::vector<long long> cur_str(sin, sin + p.ndim);
+  for (size_t ia = 0; ia < axes.size(); ++ia) {
+    const int ax = axes[ia];
+    const bool last = ia + 1 == axes.size();
+    const long long n = shape[ax], nlines = total / n;
     const std::vector<long long> wstr = layout(ax);
     const bool direct = cur_buf < 0 ? (p.storage == kF32 && same(cur_str.data(), wstr)) : same(cur_str.data(), wstr);
     const void* pin = cur;
     int in_buf = cur_buf;
     if (!direct) {
       in_buf = cur_buf < 0 ? 0 : 1 - cur_buf;
-      const int fi = cur_buf < 0 ? fast_dim(p.ndim, p.shape, cur_str.data()) : cur_fast;
+      const int fi = cur_buf < 0 ? fast_dim(p.ndim, shape, cur_str.data()) : cur_fast;
       hip_check(launch_permute(cur_buf < 0 ? p.storage : kF32, kF32, cur, buf[in_buf],
-                               perm_map(p.ndim, p.shape, cur_str.data(), wstr.data(), fi, ax), st),
+                               perm_map(p.ndim, shape, cur_str.data(), wstr.data(), fi, ax), st),
                 "permute kernel");
       pin = buf[in_buf];
     }
@@ -1280,13 +1292,13 @@ static bool sep_chain(const Plan& p, const std::vector<long long>& L, const void
     const int out_buf = in_buf < 0 ? 0 : 1 - in_buf;
     if (last) {
       // pin is a workspace here (the first axis is never the last)
-      if (p.storage == kF32 && same(p.sout, wstr)) {
+      if (p.storage == kF32 && same(sout, wstr)) {
         batch_stage_all(bp, pin, out, st);
       } else {
         batch_stage_all(bp, pin, buf[out_buf], st);
-        const int fo = fast_dim(p.ndim, p.shape, p.sout);
+        const int fo = fast_dim(p.ndim, shape, sout);
         hip_check(launch_permute(kF32, p.storage, buf[out_buf], out,
-                                 perm_map(p.ndim, p.shape, wstr.data(), p.sout, ax, fo), st),
+                                 perm_map(p.ndim, shape, wstr.data(), sout, ax, fo), st),
                   "permute kernel");
       }
     } else {
