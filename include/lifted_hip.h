@@ -178,7 +178,8 @@ int lfx_get_tuning(lfx_tuning* t);
  * reference): copies `bytes` (rounded down to the pattern's unit) from src to
  * dst, stream-ordered. pattern 0: one float4 per thread; pattern 1: one
  * 2048-float window per wavefront, read and written as 32 coalesced rows of
- * 64 floats (the register tiles' access pattern without the arithmetic).
+ * 64 floats, each XCD taking one contiguous range of windows (the register
+ * tiles' access pattern without the arithmetic).
  * Buffers must be 16-byte aligned and must not overlap. */
 int lfx_stream_copy(const void* src_dev, void* dst_dev, int64_t bytes, int32_t pattern, void* stream);
 

@@ -12,6 +12,7 @@
 
 #include "lfx_bodies.hpp"
 #include "lfx_kernels.hpp"
+#include "lfx_xcd.hpp"
 
 namespace lfx {
 
@@ -500,14 +501,14 @@ hipError_t launch_batch(int storage, bool fwd, int bc, const BatchItem* items, l
 
 // Streaming-copy references for the benchmark (no transform): one float4 per
 // thread, and one 2048-float window per wavefront read as 32 coalesced rows of
-// 64 floats and written back the same way (the register tiles' access pattern
-// without the arithmetic).
+// 64 floats and written back the same way, workgroups numbered contiguously
+// per XCD (the register tiles' access pattern without the arithmetic).
 __global__ void __launch_bounds__(256) k_copy_f4(const float4* __restrict__ a, float4* __restrict__ b, long long n4) {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   if (i < n4) b[i] = a[i];
 }
 __global__ void __launch_bounds__(256) k_copy_win(const float* __restrict__ a, float* __restrict__ b, long long nwin) {
-  const long long w = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long long w = (long long)xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   const int l = (int)(threadIdx.x & 63);
   if (w >= nwin) return;
   const float* p = a + w * 2048;

@@ -6,6 +6,7 @@
 
 #include "lfx_kernels.hpp"
 #include "lfx_reg.hpp"
+#include "lfx_xcd.hpp"
 
 namespace lfx {
 
@@ -29,7 +30,7 @@ __global__ void __launch_bounds__(256) k_reg(const BatchItem* items, long long n
   // per-wave transposition buffer: 64 rows of 17 floats (16 values per lane,
   // or 32 rows of 33 floats per round of the 32-per-lane transposes)
   __shared__ float stage[4][kStageFloats];
-  const long long w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  const long long w = __builtin_amdgcn_readfirstlane((int)(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6)));
   if (w >= nitems) return;
   float* lds = stage[threadIdx.x >> 6];
   const BatchItem it = items[w];
@@ -79,7 +80,7 @@ __global__ void __launch_bounds__(256) k_reg_rows(const BatchItem* items, long l
                                                   const DProgram<float>* progs, const float* uin, float* uout,
                                                   float* scr) {
   __shared__ float stage[4][kStageFloats];
-  const long long w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  const long long w = __builtin_amdgcn_readfirstlane((int)(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6)));
   if (w >= nitems) return;
   float* lds = stage[threadIdx.x >> 6];
   const BatchItem it = items[w];
