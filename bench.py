@@ -544,16 +544,26 @@ def main():
     plans = [fwd, adj]
     print(f"[bench] {len(sigs)} signals, {total} samples, plans in {t_plan:.1f}s, stages "
           f"{fwd.num_stages}+{adj.num_stages}", file=sys.stderr, flush=True)
-    ms, stage_ms, launches = time_plans(plans, x, [y, z], args.steps, args.warmup, per_launch=True)
+    # eager steps with HIP events around every launch (per-kernel roofline)
+    eager_ms, stage_ms, launches = time_plans(plans, x, [y, z], args.steps, args.warmup, per_launch=True)
     dom = dominant_kernel(plans, launches)
     achieved = dom["alg_bytes_per_launch"] / (dom["avg_launch_ms"] * 1e-3)
+    print(f"[bench] eager: {eager_ms:.3f} ms/step, {2 * total / (eager_ms * 1e-3) / 1e9:.2f} Gsamples/s; dominant "
+          f"{dom['symbol']} {dom['avg_launch_ms']:.3f} ms/launch, {achieved / 1e9:.0f} GB/s", file=sys.stderr,
+          flush=True)
+    eager_out = z.clone()
+    # the timed steps: the same two plans captured once as a hipGraph and
+    # replayed K times (the way a serving loop runs them); its output must
+    # equal the eager steps' bit for bit
+    z.zero_()
+    ms = time_graph_step(plans, x, [y, z], args.steps, args.warmup)
+    graph_exact = bool(torch.equal(z, eager_out))
+    del eager_out
+    if not graph_exact:
+        raise SystemExit("graph replay output differs from the eager steps")
     value = 2 * total / (ms * 1e-3) / 1e9
-    print(f"[bench] fused: {ms:.3f} ms/step, {value:.2f} Gsamples/s; dominant {dom['symbol']} "
-          f"{dom['avg_launch_ms']:.3f} ms/launch, {achieved / 1e9:.0f} GB/s", file=sys.stderr, flush=True)
-    # the same step captured once as a hipGraph and replayed (no per-launch
-    # events inside: events recorded in a graph cannot be timed here)
-    graph_ms = time_graph_step(plans, x, [y, z], args.steps, args.warmup)
-    print(f"[bench] graph replay: {graph_ms:.3f} ms/step", file=sys.stderr, flush=True)
+    print(f"[bench] graph replay: {ms:.3f} ms/step, {value:.2f} Gsamples/s (output equal to eager)",
+          file=sys.stderr, flush=True)
     # copies of the input size: the achievable streaming rate on this device
     # (torch's copy, and the library's reference copies: one float4 per thread,
     # one 2048-float window per wavefront like the register tiles)
@@ -569,8 +579,10 @@ def main():
         "tuning": {"chosen": best, "candidates": [{"knobs": k, "ms_per_step": round(t, 4)} for k, t in tried],
                    "seconds": round(t_tune, 1)},
         "copy_GBps": copy_gbs,
-        "graph_replay": {"ms_per_step": round(graph_ms, 4), "Gsamples_per_s": round(2 * total / (graph_ms * 1e-3) / 1e9, 3),
-                         "note": "forward + forward-adjoint plans captured once in a hipGraph, replayed K times"},
+        "step_mode": "forward + forward-adjoint plans captured once in a hipGraph, replayed K times "
+                     "(output checked equal to the eager steps)",
+        "eager": {"ms_per_step": round(eager_ms, 4), "Gsamples_per_s": round(2 * total / (eager_ms * 1e-3) / 1e9, 3),
+                  "note": "the same steps launched eagerly with HIP events around every launch (roofline source)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
                      "kernel": dom["symbol"].rstrip("("), "launches_per_step": dom["launches_per_step"],
@@ -587,7 +599,7 @@ def main():
         pa = BatchPlan(sigs, la.Transform.ForwardAdjoint, dtype=torch.float32, plain=True)
         pms, _, _ = time_plans([pf, pa], x, [y, z], args.plain_steps, 1)
         res["plain_ms_per_step"] = round(pms, 4)
-        res["vs_plain"] = round(pms / ms, 3)
+        res["vs_plain"] = round(pms / eager_ms, 3)
         res["plain_stages"] = pf.num_stages + pa.num_stages
         print(f"[bench] plain: {pms:.3f} ms/step ({pf.num_stages}+{pa.num_stages} launches)", file=sys.stderr,
               flush=True)

@@ -1481,7 +1481,7 @@ int lfx_batch_launch_info(lfx_batch_plan plan, int32_t stage, int32_t max_launch
       } else {
         const bool reg = i == nl && bs.nreg > 0;
         r.kind = reg ? 1 : 2;
-        r.threads = 256;
+        r.threads = 64 * kRegWaves;
         r.nitems = reg ? bs.nreg : bs.nrows;
         r.alg_bytes = reg ? bs.reg_bytes : bs.rows_bytes;
         r.lds_bytes = 0;

@@ -32,6 +32,9 @@ constexpr int kRegLevels = 4;
 constexpr int kRegPerLane = 32;
 constexpr long long kRegWindow = 64ll * kRegPerLane;
 constexpr long long kRegUnits = kRegWindow >> kRegLevels;
+// Wavefronts per workgroup of the register-tile launches (each wave runs its
+// own item; workgroups only group waves for the dispatcher).
+constexpr int kRegWaves = 1;
 
 // Base offsets of a line in three arrays: the line index is decomposed over
 // the non-transformed dims (last fastest); str[a][k] is array a's stride of

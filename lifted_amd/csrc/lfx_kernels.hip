@@ -507,8 +507,9 @@ __global__ void __launch_bounds__(256) k_copy_f4(const float4* __restrict__ a, f
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   if (i < n4) b[i] = a[i];
 }
-__global__ void __launch_bounds__(256) k_copy_win(const float* __restrict__ a, float* __restrict__ b, long long nwin) {
-  const long long w = (long long)xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+__global__ void __launch_bounds__(64 * kRegWaves) k_copy_win(const float* __restrict__ a, float* __restrict__ b,
+                                                             long long nwin) {
+  const long long w = (long long)xcd_block(blockIdx.x, gridDim.x) * kRegWaves + (threadIdx.x >> 6);
   const int l = (int)(threadIdx.x & 63);
   if (w >= nwin) return;
   const float* p = a + w * 2048;
@@ -529,8 +530,8 @@ hipError_t launch_stream_copy(const void* src, void* dst, long long bytes, int p
   } else {
     const long long nwin = bytes / (2048 * 4);
     if (nwin <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_copy_win, dim3((unsigned)((nwin + 3) / 4)), dim3(256), 0, st, (const float*)src, (float*)dst,
-                       nwin);
+    hipLaunchKernelGGL(k_copy_win, dim3((unsigned)((nwin + kRegWaves - 1) / kRegWaves)), dim3(64 * kRegWaves), 0, st,
+                       (const float*)src, (float*)dst, nwin);
   }
   return hipGetLastError();
 }

@@ -25,12 +25,12 @@ __device__ __forceinline__ void reg_item(const BatchItem& it, const DProgram<flo
 // selects the compiled step shapes (a wave-uniform branch), its program index
 // the coefficients in the plan's program table.
 template <int TF>
-__global__ void __launch_bounds__(256) k_reg(const BatchItem* items, long long nitems, const DProgram<float>* progs,
+__global__ void __launch_bounds__(64 * kRegWaves) k_reg(const BatchItem* items, long long nitems, const DProgram<float>* progs,
                                              const float* uin, float* uout, float* scr) {
   // per-wave transposition buffer: 64 rows of 17 floats (16 values per lane,
   // or 32 rows of 33 floats per round of the 32-per-lane transposes)
-  __shared__ float stage[4][kStageFloats];
-  const long long w = __builtin_amdgcn_readfirstlane((int)(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6)));
+  __shared__ float stage[kRegWaves][kStageFloats];
+  const long long w = __builtin_amdgcn_readfirstlane((int)(xcd_block(blockIdx.x, gridDim.x) * kRegWaves + (threadIdx.x >> 6)));
   if (w >= nitems) return;
   float* lds = stage[threadIdx.x >> 6];
   const BatchItem it = items[w];
@@ -76,11 +76,11 @@ __device__ __forceinline__ void row_item(const BatchItem& it, const DProgram<flo
 }
 
 template <int TF>
-__global__ void __launch_bounds__(256) k_reg_rows(const BatchItem* items, long long nitems,
+__global__ void __launch_bounds__(64 * kRegWaves) k_reg_rows(const BatchItem* items, long long nitems,
                                                   const DProgram<float>* progs, const float* uin, float* uout,
                                                   float* scr) {
-  __shared__ float stage[4][kStageFloats];
-  const long long w = __builtin_amdgcn_readfirstlane((int)(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6)));
+  __shared__ float stage[kRegWaves][kStageFloats];
+  const long long w = __builtin_amdgcn_readfirstlane((int)(xcd_block(blockIdx.x, gridDim.x) * kRegWaves + (threadIdx.x >> 6)));
   if (w >= nitems) return;
   float* lds = stage[threadIdx.x >> 6];
   const BatchItem it = items[w];
@@ -115,8 +115,8 @@ template <int TF>
 hipError_t launch_rows_tf(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
                           void* scr, hipStream_t st) {
   if (nitems <= 0) return hipSuccess;
-  const unsigned blocks = (unsigned)((nitems + 3) / 4);
-  hipLaunchKernelGGL((k_reg_rows<TF>), dim3(blocks), dim3(256), 0, st, items, nitems, (const DProgram<float>*)progs,
+  const unsigned blocks = (unsigned)((nitems + kRegWaves - 1) / kRegWaves);
+  hipLaunchKernelGGL((k_reg_rows<TF>), dim3(blocks), dim3(64 * kRegWaves), 0, st, items, nitems, (const DProgram<float>*)progs,
                      (const float*)in, (float*)out, (float*)scr);
   return hipGetLastError();
 }
@@ -125,8 +125,8 @@ template <int TF>
 hipError_t launch_reg_tf(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
                          void* scr, hipStream_t st) {
   if (nitems <= 0) return hipSuccess;
-  const unsigned blocks = (unsigned)((nitems + 3) / 4);
-  hipLaunchKernelGGL((k_reg<TF>), dim3(blocks), dim3(256), 0, st, items, nitems, (const DProgram<float>*)progs,
+  const unsigned blocks = (unsigned)((nitems + kRegWaves - 1) / kRegWaves);
+  hipLaunchKernelGGL((k_reg<TF>), dim3(blocks), dim3(64 * kRegWaves), 0, st, items, nitems, (const DProgram<float>*)progs,
                      (const float*)in, (float*)out, (float*)scr);
   return hipGetLastError();
 }
