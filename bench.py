@@ -554,16 +554,22 @@ def main():
     eager_out = z.clone()
     # the timed steps: the same two plans captured once as a hipGraph and
     # replayed K times (the way a serving loop runs them); its output must
-    # equal the eager steps' bit for bit
+    # equal the eager steps' bit for bit, else the eager steps are reported
     z.zero_()
-    ms = time_graph_step(plans, x, [y, z], args.steps, args.warmup)
-    graph_exact = bool(torch.equal(z, eager_out))
+    step_mode = ("forward + forward-adjoint plans captured once in a hipGraph, replayed K times "
+                 "(output checked equal to the eager steps)")
+    try:
+        ms = time_graph_step(plans, x, [y, z], args.steps, args.warmup)
+        if not torch.equal(z, eager_out):
+            raise RuntimeError("graph replay output differs from the eager steps")
+        print(f"[bench] graph replay: {ms:.3f} ms/step, {2 * total / (ms * 1e-3) / 1e9:.2f} Gsamples/s "
+              "(output equal to eager)", file=sys.stderr, flush=True)
+    except RuntimeError as e:
+        ms = eager_ms
+        step_mode = f"eager steps (hipGraph replay not used: {e})"
+        print(f"[bench] {step_mode}", file=sys.stderr, flush=True)
     del eager_out
-    if not graph_exact:
-        raise SystemExit("graph replay output differs from the eager steps")
     value = 2 * total / (ms * 1e-3) / 1e9
-    print(f"[bench] graph replay: {ms:.3f} ms/step, {value:.2f} Gsamples/s (output equal to eager)",
-          file=sys.stderr, flush=True)
     # copies of the input size: the achievable streaming rate on this device
     # (torch's copy, and the library's reference copies: one float4 per thread,
     # one 2048-float window per wavefront like the register tiles)
@@ -579,8 +585,7 @@ def main():
         "tuning": {"chosen": best, "candidates": [{"knobs": k, "ms_per_step": round(t, 4)} for k, t in tried],
                    "seconds": round(t_tune, 1)},
         "copy_GBps": copy_gbs,
-        "step_mode": "forward + forward-adjoint plans captured once in a hipGraph, replayed K times "
-                     "(output checked equal to the eager steps)",
+        "step_mode": step_mode,
         "eager": {"ms_per_step": round(eager_ms, 4), "Gsamples_per_s": round(2 * total / (eager_ms * 1e-3) / 1e9, 3),
                   "note": "the same steps launched eagerly with HIP events around every launch (roofline source)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
