@@ -1,6 +1,7 @@
-// Workgroup numbering for the eight XCDs of an MI355X.
+// Workgroup numbering for the eight XCDs of an MI355X (host-compilable, so the
+// CPU tests can check the renumbering).
 #pragma once
-#include <hip/hip_runtime.h>
+#include "lfx_common.hpp"
 
 namespace lfx {
 
@@ -10,7 +11,7 @@ namespace lfx {
 // a line then share that XCD's L2 (their halos hit there) and each XCD
 // streams one contiguous stretch of memory. A plain copy of 2048-float
 // windows per wavefront gains about 6 % from it on MI355X.
-__device__ __forceinline__ unsigned xcd_block(unsigned g, unsigned G) {
+LFX_HDI unsigned xcd_block(unsigned g, unsigned G) {
   const unsigned q = G >> 3, r = G & 7, x = g & 7, i = g >> 3;
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }

@@ -255,3 +255,11 @@ def test_row_mode_equal_short_lines(sim, oracle, w):
                 assert "rows [0," in info, info
         cases.append(case)
     run_cases(cases)
+
+
+def test_xcd_block_renumbering(sim):
+    """The register and window-copy launches renumber workgroups per XCD
+    (lfx_xcd.hpp): every grid size up to 5000 must map onto itself one to one,
+    with each XCD's workgroups numbered consecutively."""
+    r = subprocess.run([os.path.join(os.path.dirname(sim), "xcd_check")], capture_output=True)
+    assert r.returncode == 0 and r.stdout.decode().strip() == "ok", r.stdout.decode() + r.stderr.decode()
