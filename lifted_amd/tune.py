@@ -72,7 +72,7 @@ def default_space(dtype):
     import torch
     if dtype == torch.float64:
         return {"tile": [0, 2048, 8192], "levels_per_pass": [0, 4], "side_streams": [-1, 0]}
-    return {"levels_per_pass": [0, 3], "line_limit": [0, 1024], "side_streams": [-1, 0]}
+    return {"levels_per_pass": [0, 3], "line_limit": [0, 32, 1024], "side_streams": [-1, 0]}
 
 
 def _time_plans(plans, x, bufs, reps, warmup):
@@ -101,14 +101,17 @@ def _time_plans(plans, x, bufs, reps, warmup):
     return times[len(times) // 2]
 
 
-def tune_batch(signals, transforms, x, mode=0, space=None, reps=5, warmup=2, verbose=False):
+def tune_batch(signals, transforms, x, mode=0, space=None, reps=5, warmup=2, verbose=False, min_gain=0.01):
     """Search planning knobs for batched plans of `signals`.
 
     transforms: transform types run in sequence (x -> out_0 -> out_1 ...),
     e.g. (Forward, ForwardAdjoint). x: the packed input (device tensor, the
     plans' storage type). Returns (best knobs dict, [(knobs, ms), ...]) with
     the candidates in the order timed; the library defaults are always the
-    first candidate, so the result is never slower than them on this data."""
+    first candidate, so the result is never slower than them on this data.
+    Another candidate is chosen only when it beats the defaults by at least
+    min_gain (a fraction): closer timings are within run-to-run noise, and
+    keeping the defaults then keeps the chosen plans the same from run to run."""
     import torch
     from .batch import BatchPlan
     space = dict(space if space is not None else default_space(x.dtype))
@@ -128,5 +131,6 @@ def tune_batch(signals, transforms, x, mode=0, space=None, reps=5, warmup=2, ver
         results.append((knobs, ms))
         if verbose:
             print(f"[tune] {knobs}: {ms:.3f} ms (plans {time.perf_counter() - t0:.2f} s)", flush=True)
-    best = min(results, key=lambda r: r[1])[0]
+    fastest = min(results, key=lambda r: r[1])
+    best = fastest[0] if fastest[1] < results[0][1] * (1.0 - min_gain) else results[0][0]
     return best, results

@@ -39,6 +39,34 @@ def test_batch_matches_oracle(gpu, oracle, tf, plain):
             assert np.abs(got - ref).max() <= 4 * np.finfo(np.float32).eps * scale, (off, n, w, L, tf, bc)
 
 
+@pytest.mark.parametrize("line", [16, 64])
+def test_register_tiles_down_to_short_bands(gpu, oracle, line):
+    """Zero boundary, float32: with a whole-line threshold of 16 or 64 samples
+    the register tiles keep running passes of masked windows on short bands
+    (an extra stage) before the whole-line items; every transform type still
+    equals the oracle per signal, at each signal's maximum level."""
+    import torch
+    from lifted_amd.batch import BatchPlan
+    from lifted_amd.tune import tuning
+    g = np.random.default_rng(40 + line)
+    sigs, off = [], 0
+    for _ in range(36):
+        n = int(round(2 ** g.uniform(6, 18)))
+        w = int(g.choice([w for w in WAVELETS if w != 0]))
+        sigs.append((off, n, w, max(1, oracle.max_level(w, n))))
+        off += n
+    x = np.random.default_rng(2).standard_normal(off).astype(np.float32)
+    xd = torch.from_numpy(x).to(gpu)
+    for tf in (0, 1, 2, 3):
+        with tuning(line_limit=line):
+            plan = BatchPlan(sigs, tf, mode=0, dtype=torch.float32)
+        y = plan.execute(xd).cpu().numpy()
+        for o, n, w, L in sigs:
+            ref = oracle.lwt(x[o:o + n], w, tf, 0, level=[L])
+            scale = max(np.abs(ref).max(), 1e-30)
+            assert np.abs(y[o:o + n] - ref).max() <= 4 * np.finfo(np.float32).eps * scale, (line, tf, n, w, L)
+
+
 def test_batch_fp64_and_graph_replay(gpu, oracle):
     """fp64 batch; a captured graph of forward + forward-adjoint replays
     identically to eager execution (several replays, eager work in between)."""
