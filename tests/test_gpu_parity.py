@@ -246,6 +246,23 @@ def test_nd_float32_short_lines(gpu, oracle, shape, axes, levels):
         _check(got, ref, np.float32, f"shape={shape} axes={axes} w={w} bc={bc} tf={tf} sep={sep}")
 
 
+def test_nd_float32_whole_tile_transposes(gpu, oracle):
+    """Boxes whose transposed dims are multiples of 64 with 16-byte aligned
+    rows take the float4 transpose (k_permute_f4); C- and F-ordered inputs,
+    separable and level-by-level (whose coarser sub-boxes fall back to the
+    scalar transpose), all transforms, against the oracle."""
+    import torch
+    import lifted_amd as la
+    x = rng(21).standard_normal((128, 64, 192)).astype(np.float32)
+    xt = torch.from_numpy(x).to(gpu)
+    xf = xt.permute(2, 1, 0).contiguous().permute(2, 1, 0)
+    for tf, (axes, lvl) in itertools.product((F, I, FA, IA), (((0, 2, 1), [3, 2, 4]), ((2, 0), 2))):
+        ref = oracle.lwt(x, 100907, tf, 0, axes=list(axes), level=lvl)
+        for src, name in ((xt, "C"), (xf, "F")):
+            got = la.lwt(src, 100907, tf, axes=list(axes), level=lvl)
+            _check(got.cpu().numpy(), ref, np.float32, f"{name}-order tf={tf} axes={axes} level={lvl}")
+
+
 def test_layouts_float32_batched_lines(gpu, oracle):
     """float32 layouts through the batched-lines path: F-order, strided views
     (in and out), a negative stride, in place, separable and level-by-level,
