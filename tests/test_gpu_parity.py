@@ -247,10 +247,10 @@ def test_nd_float32_short_lines(gpu, oracle, shape, axes, levels):
 
 
 def test_nd_float32_whole_tile_transposes(gpu, oracle):
-    """Boxes whose transposed dims are multiples of 64 with 16-byte aligned
-    rows take the float4 transpose (k_permute_f4); C- and F-ordered inputs,
-    separable and level-by-level (whose coarser sub-boxes fall back to the
-    scalar transpose), all transforms, against the oracle."""
+    """Boxes whose transposed dims are multiples of 64 with aligned rows take
+    the vector transposes (k_permute_v4); C- and F-ordered inputs, separable
+    and level-by-level (whose coarser sub-boxes fall back to the scalar
+    transpose), all transforms, against the oracle; and 16-bit storage."""
     import torch
     import lifted_amd as la
     x = rng(21).standard_normal((128, 64, 192)).astype(np.float32)
@@ -261,6 +261,16 @@ def test_nd_float32_whole_tile_transposes(gpu, oracle):
         for src, name in ((xt, "C"), (xf, "F")):
             got = la.lwt(src, 100907, tf, axes=list(axes), level=lvl)
             _check(got.cpu().numpy(), ref, np.float32, f"{name}-order tf={tf} axes={axes} level={lvl}")
+    # 16-bit storage, separable over strided first and last axes: 16-bit ->
+    # float32 transposes in, float32 -> 16-bit out; the float32 oracle on the
+    # 16-bit input, rounded once to the storage type, bit for bit
+    for storage, tf in itertools.product(("bfloat16", "float16"), (F, FA)):
+        tdt = getattr(torch, storage)
+        h = xt.to(tdt)
+        ref = torch.from_numpy(oracle.lwt(h.float().cpu().numpy(), 100907, tf, 0, axes=[0, 2, 1],
+                                          level=[3, 2, 4])).to(tdt).float().numpy()
+        got = la.lwt(h, 100907, tf, axes=[0, 2, 1], level=[3, 2, 4]).float().cpu().numpy()
+        np.testing.assert_array_equal(got, ref, err_msg=f"{storage} tf={tf}")
 
 
 def test_layouts_float32_batched_lines(gpu, oracle):
