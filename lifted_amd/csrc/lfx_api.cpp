@@ -205,8 +205,44 @@ static std::vector<long long> dense_strides(int ndim, const long long* shp, int 
   return s;
 }
 
+// Identical strides on both sides over a box without gaps (any dim order,
+// negative strides allowed): the box is one memory range, copied flat (the
+// streaming float4 copy for its 16-byte aligned part). Returns false when the
+// box does not qualify.
+static bool flat_copy(size_t es, int ndim, const long long* shp, const void* src, const long long* sst, void* dst,
+                      const long long* dstr, hipStream_t st) {
+  std::vector<std::pair<long long, long long>> dims;  // (|stride|, size) of dims longer than 1
+  long long total = 1, lo = 0;
+  for (int k = 0; k < ndim; ++k) {
+    if (sst[k] != dstr[k]) return false;
+    total *= shp[k];
+    if (shp[k] == 1) continue;
+    dims.push_back({sst[k] < 0 ? -sst[k] : sst[k], shp[k]});
+    if (sst[k] < 0) lo += (shp[k] - 1) * sst[k];
+  }
+  if (total == 0) return true;
+  std::sort(dims.begin(), dims.end());
+  long long expect = 1;
+  for (const auto& d : dims) {
+    if (d.first != expect) return false;
+    expect *= d.second;
+  }
+  const char* s = static_cast<const char*>(src) + lo * (long long)es;
+  char* t = static_cast<char*>(dst) + lo * (long long)es;
+  const long long bytes = total * (long long)es;
+  long long head = 0;
+  if ((((uintptr_t)s | (uintptr_t)t) & 15) == 0) {
+    head = bytes & ~15ll;
+    hip_check(launch_stream_copy(s, t, head, 0, st), "copy kernel");
+  }
+  if (bytes > head) hip_check(hipMemcpyAsync(t + head, s + head, (size_t)(bytes - head), hipMemcpyDeviceToDevice, st),
+                              "hipMemcpyAsync");
+  return true;
+}
+
 static void copy_box(int storage, int ndim, const long long* shp, const void* src, const long long* sst, void* dst,
                      const long long* dstr, hipStream_t st) {
+  if (flat_copy(storage_bytes(storage), ndim, shp, src, sst, dst, dstr, st)) return;
   CopyMap cm{};
   cm.nd = ndim;
   long long total = 1;
