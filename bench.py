@@ -237,8 +237,9 @@ def _pmc_pass(counter, child_args, tmp):
     return agg
 
 
-def measure_traffic(total, symbol):
-    """HBM bytes per launch of `symbol` for one step of the workload, from
+def measure_traffic(total, symbol, tuning):
+    """HBM bytes per launch of `symbol` for one step of the workload (plans
+    built with the planning knobs `tuning`, as in the timed steps), from
     FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by the ratio of
     known bytes to the counter on a device copy of the input size."""
     if shutil.which("rocprofv3") is None:
@@ -251,7 +252,7 @@ def measure_traffic(total, symbol):
             ck = [k for k in copy if "copyBuffer" in k or "copy" in k.lower()]
             n, v = copy[ck[0]]
             factor = (total * 4 / 1024.0) / (v / n)
-            work = _pmc_pass(counter, ["--pmc-child"], tmp)
+            work = _pmc_pass(counter, ["--pmc-child", "--tuning-json", json.dumps(tuning)], tmp)
             wk = [k for k in work if k.startswith(symbol)]
             if not wk:
                 raise RuntimeError(f"{symbol} not in the {counter} pass")
@@ -293,8 +294,9 @@ def cpu_baseline(sigs, total, x_host, reps=3):
                             "sample": f"first {len(sub)} signals ({sub_n} samples), once, {dt1:.2f} s"}}
 
 
-def pmc_child():
-    """One step of the workload, untimed (run under rocprofv3 counter passes)."""
+def pmc_child(tuning):
+    """One step of the workload, untimed (run under rocprofv3 counter passes),
+    with the plans built from the planning knobs the timed steps used."""
     import torch
     import lifted_amd as la
     from lifted_amd.batch import BatchPlan
@@ -306,8 +308,8 @@ def pmc_child():
     x = torch.randn(total, generator=gen, device=dev, dtype=torch.float32)
     y = torch.empty_like(x)
     z = torch.empty_like(x)
-    fwd = BatchPlan(sigs, la.Transform.Forward, dtype=torch.float32)
-    adj = BatchPlan(sigs, la.Transform.ForwardAdjoint, dtype=torch.float32)
+    fwd = BatchPlan(sigs, la.Transform.Forward, dtype=torch.float32, tuning=tuning)
+    adj = BatchPlan(sigs, la.Transform.ForwardAdjoint, dtype=torch.float32, tuning=tuning)
     fwd.execute(x, y)
     adj.execute(y, z)
     torch.cuda.synchronize()
@@ -491,6 +493,7 @@ def main():
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a device copy of the workload's input size (HBM counter calibration)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--tuning-json", default="{}", help=argparse.SUPPRESS)
     ap.add_argument("--workload", choices=["c4", "c1", "c2", "c3", "c5"], default="c4",
                     help="c4 (default): the batched headline; c1 / c2 / c3 / c5: secondary workloads")
     args = ap.parse_args()
@@ -500,7 +503,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.pmc_child:
-        pmc_child()
+        pmc_child(json.loads(args.tuning_json))
         return
     if args.workload != "c4":
         import torch
@@ -533,7 +536,7 @@ def main():
     # planning knobs chosen by the autotuner on this data (defaults first)
     from lifted_amd.tune import tune_batch
     t_tune = time.perf_counter()
-    best, tried = tune_batch(sigs, (la.Transform.Forward, la.Transform.ForwardAdjoint), x, reps=3, warmup=1)
+    best, tried = tune_batch(sigs, (la.Transform.Forward, la.Transform.ForwardAdjoint), x, reps=5, warmup=2)
     t_tune = time.perf_counter() - t_tune
     print(f"[bench] autotune {t_tune:.1f}s: " + ", ".join(f"{k}={ms:.3f}ms" for k, ms in tried), file=sys.stderr,
           flush=True)
@@ -618,7 +621,7 @@ def main():
     if not args.no_traffic:
         del x, y, z
         torch.cuda.empty_cache()
-        traffic, note = measure_traffic(total, dom["symbol"])
+        traffic, note = measure_traffic(total, dom["symbol"], best)
         res["roofline"]["traffic"] = None if traffic is None else round(traffic)
         res["traffic_detail"] = note
         print(f"[bench] traffic: {traffic} ({note})", file=sys.stderr, flush=True)

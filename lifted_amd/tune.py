@@ -101,7 +101,7 @@ def _time_plans(plans, x, bufs, reps, warmup):
     return times[len(times) // 2]
 
 
-def tune_batch(signals, transforms, x, mode=0, space=None, reps=5, warmup=2, verbose=False, min_gain=0.01):
+def tune_batch(signals, transforms, x, mode=0, space=None, reps=5, warmup=2, verbose=False, min_gain=0.02):
     """Search planning knobs for batched plans of `signals`.
 
     transforms: transform types run in sequence (x -> out_0 -> out_1 ...),
