@@ -264,6 +264,35 @@ __global__ void __launch_bounds__(256) k_convert(const SI* __restrict__ src, SO*
   }
 }
 
+// fi == fo with unit stride along i on both sides, lines of at least 256
+// elements, a multiple of 4 long, each starting on a 4-element boundary:
+// a workgroup per line, 4 elements per thread and access.
+template <class SI, class SO>
+__global__ void __launch_bounds__(256) k_convert_v4(const SI* __restrict__ src, SO* __restrict__ dst, PermMap pm) {
+  const long long n4 = pm.ni / 4;
+  for (long long b = blockIdx.x; b < pm.nbatch; b += gridDim.x) {
+    long long os, od;
+    perm_batch(pm, b, os, od);
+    const Vec4<SI>* s = reinterpret_cast<const Vec4<SI>*>(src + os);
+    Vec4<SO>* d = reinterpret_cast<Vec4<SO>*>(dst + od);
+    for (long long i = threadIdx.x; i < n4; i += 256) {
+      const Vec4<SI> v = s[i];
+      Vec4<SO> w;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) dev::store_as<SO, float>(w.v, j, dev::load_as<SI, float>(v.v, j));
+      d[i] = w;
+    }
+  }
+}
+
+static bool convert_v4_ok(const PermMap& pm, const void* src, size_t es_in, const void* dst, size_t es_out) {
+  if (!pm.same || pm.si_i != 1 || pm.so_i != 1 || pm.ni < 256 || pm.ni % 4) return false;
+  if (((uintptr_t)src & (4 * es_in - 1)) || ((uintptr_t)dst & (4 * es_out - 1))) return false;
+  for (int k = 0; k < pm.nb; ++k)
+    if (pm.bss[k] % 4 || pm.bds[k] % 4) return false;
+  return true;
+}
+
 template <class SI, class SO, class C>
 static hipError_t permute_t(const void* src, void* dst, const PermMap& pm, hipStream_t st) {
   const unsigned z = (unsigned)std::min<long long>(pm.nbatch, 65535);
@@ -282,8 +311,21 @@ static hipError_t permute_t(const void* src, void* dst, const PermMap& pm, hipSt
 hipError_t launch_permute(int sin, int sout, const void* src, void* dst, const PermMap& pm, hipStream_t st) {
   if (pm.ni <= 0 || pm.no <= 0 || pm.nbatch <= 0) return hipSuccess;
   if (sin == kF64 && sout == kF64) return permute_t<double, double, double>(src, dst, pm, st);
-  if (sin != kF64 && sout != kF64 && (sin == kF32 || sout == kF32) &&
-      permute_v4_ok(pm, src, storage_bytes(sin), dst, storage_bytes(sout))) {
+  const bool v4 = sin != kF64 && sout != kF64 && (sin == kF32 || sout == kF32);
+  if (v4 && convert_v4_ok(pm, src, storage_bytes(sin), dst, storage_bytes(sout))) {
+    const dim3 grid((unsigned)std::min<long long>(pm.nbatch, 1 << 20));
+#define LFX_V4(SI_, SO_)                                                                                    \
+  hipLaunchKernelGGL((k_convert_v4<SI_, SO_>), grid, dim3(256), 0, st, reinterpret_cast<const SI_*>(src), \
+                     reinterpret_cast<SO_*>(dst), pm);                                                     \
+  return hipGetLastError();
+    if (sin == kF32 && sout == kF32) { LFX_V4(float, float) }
+    if (sin == kBF16) { LFX_V4(bf16_t, float) }
+    if (sin == kF16) { LFX_V4(f16_t, float) }
+    if (sout == kBF16) { LFX_V4(float, bf16_t) }
+    if (sout == kF16) { LFX_V4(float, f16_t) }
+#undef LFX_V4
+  }
+  if (v4 && permute_v4_ok(pm, src, storage_bytes(sin), dst, storage_bytes(sout))) {
     const dim3 grid((unsigned)(pm.ni / 64), (unsigned)(pm.no / 64), (unsigned)std::min<long long>(pm.nbatch, 65535));
 #define LFX_V4(SI_, SO_)                                                                                    \
   hipLaunchKernelGGL((k_permute_v4<SI_, SO_>), grid, dim3(256), 0, st, reinterpret_cast<const SI_*>(src), \
