@@ -273,6 +273,48 @@ def test_nd_float32_whole_tile_transposes(gpu, oracle):
         np.testing.assert_array_equal(got, ref, err_msg=f"{storage} tf={tf}")
 
 
+def test_level_by_level_inverse_copies_in_to_out(gpu, oracle):
+    """Level-by-level Inverse / ForwardAdjoint with a separate output of the
+    input's strides first copy the input to the output. Gap-free boxes copy as
+    one flat range: C and F order, a size-1 dim, byte counts that are not a
+    multiple of 16 (the tail is copied separately), and a base that is not
+    16-byte aligned (no streaming part); all against the oracle."""
+    import torch
+    import lifted_amd as la
+    cases = [((3, 37, 5), torch.float32), ((40, 1, 52), torch.float32), ((3, 37, 5), torch.bfloat16),
+             ((64, 48), torch.float16), ((33, 20), torch.float64)]
+    for shape, dt in cases:
+        x = torch.from_numpy(rng(len(shape) + shape[0]).standard_normal(shape).astype(np.float32)).to(dt)
+        xs = x.float().numpy() if dt != torch.float64 else x.numpy()
+        axes = [0, len(shape) - 1]
+        for tf, order in itertools.product((I, FA), ("C", "F", "offset")):
+            if order == "C":
+                src = x.to(gpu)
+                out = torch.empty_like(src)
+            elif order == "F":
+                src = x.to(gpu).permute(*reversed(range(len(shape)))).contiguous().permute(
+                    *reversed(range(len(shape))))
+                out = torch.empty_like(src)
+                assert out.stride() == src.stride()
+            else:
+                # a view one element into a larger buffer: base off the 16-byte grid
+                buf = torch.empty(x.numel() + 1, dtype=dt, device=gpu)
+                src = buf[1:].view(shape)
+                src.copy_(x.to(gpu))
+                obuf = torch.empty(x.numel() + 1, dtype=dt, device=gpu)
+                out = obuf[1:].view(shape)
+            ref = oracle.lwt(xs.astype(np.float64) if dt == torch.float64 else xs, 100907, tf, 0, axes=axes, level=2)
+            got = la.lwt(src, 100907, tf, axes=axes, level=2, out=out)
+            what = f"shape={shape} dtype={dt} tf={tf} order={order}"
+            if dt in (torch.float32, torch.float64):
+                _check(got.cpu().numpy(), ref, np.float64 if dt == torch.float64 else np.float32, what)
+            else:
+                # 16-bit storage is rounded once per level: compare within the storage type's precision
+                g = got.float().cpu().numpy()
+                tol = 8 * float(torch.finfo(dt).eps) * float(np.abs(ref).max())
+                assert np.abs(g - ref).max() <= tol, what
+
+
 def test_layouts_float32_batched_lines(gpu, oracle):
     """float32 layouts through the batched-lines path: F-order, strided views
     (in and out), a negative stride, in place, separable and level-by-level,
