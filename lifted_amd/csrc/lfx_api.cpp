@@ -295,8 +295,9 @@ static void transform_lines_t(const Plan& p, const WaveletDefL& w, const void* p
   // LFX_FLAG_PLAIN: one tiled pass per level (one launch per level, the band
   // round-trips through memory between levels); Periodic keeps whole lines
   const bool plain = (p.flags & LFX_FLAG_PLAIN) != 0 && p.bc != kPeriodic;
-  const LineSchedule sc = plain ? schedule(hp, fwd, n, L, p.storage, p.bc, g_tuning.tile, 1, 1)
-                                : schedule(hp, fwd, n, L, p.storage, p.bc, g_tuning.tile, g_tuning.k);
+  const long long tile = tile_for_lines(g_tuning.tile, p.storage, nlines, n);
+  const LineSchedule sc = plain ? schedule(hp, fwd, n, L, p.storage, p.bc, tile, 1, 1)
+                                : schedule(hp, fwd, n, L, p.storage, p.bc, tile, g_tuning.k);
   const bool alias = src == dst;
   if (sc.passes.empty()) {
     LineLaunch ll{};

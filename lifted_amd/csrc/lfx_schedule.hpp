@@ -64,6 +64,19 @@ long long find_halo(const DProgram<C>& hp, bool fwd, long long m0, int k, long l
   return -1;
 }
 
+// Default LDS tile length (samples at a pass's finest level) by compute type.
+inline long long default_tile(int storage) { return compute_bytes(storage) == 8 ? 4096 : 8192; }
+
+// The default tile, halved (down to 512) while a pass over `nlines` lines of
+// n samples would launch fewer than 512 tiles: a few long lines then still
+// spread over the chip's 256 CUs. A caller's tile (tile > 0) is kept.
+inline long long tile_for_lines(long long tile, int storage, long long nlines, long long n) {
+  if (tile > 0) return tile;
+  long long T = default_tile(storage);
+  while (T > 512 && nlines * ((n + T - 1) / T) < 512) T /= 2;
+  return T;
+}
+
 template <class C>
 LineSchedule schedule(const DProgram<C>& hp, bool fwd, long long n, long long L, int storage, int bc,
                       long long tile_hint, int k_hint, long long limit_override = 0) {
@@ -74,7 +87,7 @@ LineSchedule schedule(const DProgram<C>& hp, bool fwd, long long n, long long L,
     fail(LFX_EUNSUPPORTED, "Periodic boundary on lines longer than " + std::to_string(limit) +
                                " samples is not supported (its wrap reads the far end of the band)");
   long long m = n, lv = 0;
-  const long long T0 = tile_hint > 0 ? tile_hint : (compute_bytes(storage) == 8 ? 4096 : 8192);
+  const long long T0 = tile_hint > 0 ? tile_hint : default_tile(storage);
   while (lv < L && m > limit) {
     const int k0 = (int)std::min<long long>(L - lv, k_hint > 0 ? k_hint : 6);
     int k = k0;
