@@ -101,7 +101,8 @@ def _time_plans(plans, x, bufs, reps, warmup):
     return times[len(times) // 2]
 
 
-def tune_batch(signals, transforms, x, mode=0, space=None, reps=5, warmup=2, verbose=False, min_gain=0.02):
+def tune_batch(signals, transforms, x, mode=0, space=None, reps=5, warmup=2, verbose=False, min_gain=0.02,
+               rounds=2):
     """Search planning knobs for batched plans of `signals`.
 
     transforms: transform types run in sequence (x -> out_0 -> out_1 ...),
@@ -109,28 +110,34 @@ def tune_batch(signals, transforms, x, mode=0, space=None, reps=5, warmup=2, ver
     plans' storage type). Returns (best knobs dict, [(knobs, ms), ...]) with
     the candidates in the order timed; the library defaults are always the
     first candidate, so the result is never slower than them on this data.
-    Another candidate is chosen only when it beats the defaults by at least
-    min_gain (a fraction): closer timings are within run-to-run noise, and
-    keeping the defaults then keeps the chosen plans the same from run to run."""
+    The candidates are timed in `rounds` round-robin rounds (plans rebuilt
+    each round) and each keeps its fastest round, so that the first timings
+    of a run (clocks, caches still settling) do not count against the
+    candidate timed first. Another candidate is chosen only when it beats
+    the defaults by at least min_gain (a fraction): closer timings are
+    within run-to-run noise, and keeping the defaults then keeps the chosen
+    plans the same from run to run."""
     import torch
     from .batch import BatchPlan
     space = dict(space if space is not None else default_space(x.dtype))
     keys = list(space)
-    results = []
+    cands = [dict(zip(keys, values)) for values in itertools.product(*(space[k] for k in keys))]
+    best_ms = [float("inf")] * len(cands)
     bufs = [torch.empty_like(x) for _ in transforms]
-    for values in itertools.product(*(space[k] for k in keys)):
-        knobs = dict(zip(keys, values))
-        t0 = time.perf_counter()
-        with tuning(**knobs):
-            plans = [BatchPlan(signals, tf, mode=mode, dtype=x.dtype) for tf in transforms]
-        try:
-            ms = _time_plans(plans, x, bufs, reps, warmup)
-        finally:
-            for p in plans:
-                p.close()
-        results.append((knobs, ms))
-        if verbose:
-            print(f"[tune] {knobs}: {ms:.3f} ms (plans {time.perf_counter() - t0:.2f} s)", flush=True)
+    for rnd in range(max(1, int(rounds))):
+        for i, knobs in enumerate(cands):
+            t0 = time.perf_counter()
+            with tuning(**knobs):
+                plans = [BatchPlan(signals, tf, mode=mode, dtype=x.dtype) for tf in transforms]
+            try:
+                ms = _time_plans(plans, x, bufs, reps, warmup)
+            finally:
+                for p in plans:
+                    p.close()
+            best_ms[i] = min(best_ms[i], ms)
+            if verbose:
+                print(f"[tune] round {rnd}: {knobs}: {ms:.3f} ms (plans {time.perf_counter() - t0:.2f} s)", flush=True)
+    results = list(zip(cands, best_ms))
     fastest = min(results, key=lambda r: r[1])
     best = fastest[0] if fastest[1] < results[0][1] * (1.0 - min_gain) else results[0][0]
     return best, results
