@@ -267,31 +267,70 @@ def measure_traffic(total, symbol, tuning):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
-def cpu_baseline(sigs, total, x_host, reps=3):
-    import oracle as O
-    threads = min(os.cpu_count() or 1, 16)
-    y = np.empty_like(x_host)
-    z = np.empty_like(x_host)
+def cpu_model():
+    """CPU model name of this host (/proc/cpuinfo), or the platform string."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
+
+
+def cpu_cores():
+    """Cores this process may run on: its CPU affinity, capped at the 16-core
+    host share of one GPU (the GPU hosts give each GPU's jobs 16 cores,
+    whatever nproc reports)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16)), os.cpu_count() or 1
+
+
+def _median_timed(fn, reps=5, warmup=1):
+    for _ in range(warmup):
+        fn()
     times = []
     for _ in range(reps):
         t = time.perf_counter()
-        O.batch(x_host, sigs, O.FORWARD, O.ZERO, out=y, n_threads=threads)
-        O.batch(y, sigs, O.FORWARD_ADJOINT, O.ZERO, out=z, n_threads=threads)
+        fn()
         times.append(time.perf_counter() - t)
-    dt = float(np.median(times))
-    # one core on the first eighth of the signals
-    sub = sigs[: len(sigs) // 8]
+    return float(np.median(times))
+
+
+def cpu_baseline(sigs, total, x_host, reps=5):
+    """The CPU restatement of the reference timed on this host: 1 warm-up and
+    the median of `reps` runs, on all the cores this process may use (a
+    dynamic parallel-for over signals: the reference runs one line on one
+    thread) and on one core (a subset of the signals)."""
+    import oracle as O
+    threads, nproc = cpu_cores()
+    y = np.empty_like(x_host)
+    z = np.empty_like(x_host)
+
+    def run(sub, th):
+        O.batch(x_host, sub, O.FORWARD, O.ZERO, out=y, n_threads=th)
+        O.batch(y, sub, O.FORWARD_ADJOINT, O.ZERO, out=z, n_threads=th)
+
+    dt = _median_timed(lambda: run(sigs, threads), reps)
+    # one core on the first sixteenth of the signals (about 1 s per run)
+    sub = sigs[: len(sigs) // 16]
     sub_n = sum(s[1] for s in sub)
-    t = time.perf_counter()
-    O.batch(x_host, sub, O.FORWARD, O.ZERO, out=y, n_threads=1)
-    O.batch(y, sub, O.FORWARD_ADJOINT, O.ZERO, out=z, n_threads=1)
-    dt1 = time.perf_counter() - t
+    dt1 = _median_timed(lambda: run(sub, 1), reps)
+    model = cpu_model()
     return {"value": round(2 * total / dt / 1e9, 4), "unit": "Gsamples/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "nproc": nproc,
             "sample": f"the full workload ({len(sigs)} signals, {total} samples), forward then forward-adjoint, "
-                      f"median of {reps} runs ({dt:.2f} s each), CPU restatement of the reference algorithm "
-                      f"(its Highway build is unavailable offline), dynamic parallel-for over signals",
+                      f"1 warm-up then the median of {reps} runs ({dt:.2f} s each) on {threads} threads (all cores "
+                      f"of this process's share; nproc {nproc}), {model}; CPU restatement of the reference "
+                      f"algorithm (its Highway build is unavailable offline), dynamic parallel-for over signals",
             "single_core": {"value": round(2 * sub_n / dt1 / 1e9, 4), "cores": 1,
-                            "sample": f"first {len(sub)} signals ({sub_n} samples), once, {dt1:.2f} s"}}
+                            "sample": f"first {len(sub)} signals ({sub_n} samples), 1 warm-up then the median of "
+                                      f"{reps} runs ({dt1:.2f} s each)"}}
 
 
 def pmc_child(tuning):
@@ -603,16 +642,36 @@ def main():
                       "ms": round(r["ms"], 4)} for r in launches],
     }
     if not args.no_plain:
+        # plain = one launch per level: the same register-tile kernels run one
+        # level per pass (the band round-trips through HBM between levels),
+        # short bands one level per LDS launch; timed like the fused step
+        # (hipGraph replay) and eagerly. The LDS-only plain plans (the generic
+        # step interpreter, one level per launch) are a secondary figure.
         pf = BatchPlan(sigs, la.Transform.Forward, dtype=torch.float32, plain=True)
         pa = BatchPlan(sigs, la.Transform.ForwardAdjoint, dtype=torch.float32, plain=True)
-        pms, _, _ = time_plans([pf, pa], x, [y, z], args.plain_steps, 1)
+        pms_eager, _, _ = time_plans([pf, pa], x, [y, z], args.plain_steps, 1)
+        try:
+            pms = time_graph_step([pf, pa], x, [y, z], args.plain_steps, 1)
+        except RuntimeError:
+            pms = pms_eager
         res["plain_ms_per_step"] = round(pms, 4)
-        res["vs_plain"] = round(pms / eager_ms, 3)
+        res["plain_eager_ms_per_step"] = round(pms_eager, 4)
+        res["vs_plain"] = round(pms / ms, 3)
         res["plain_stages"] = pf.num_stages + pa.num_stages
-        print(f"[bench] plain: {pms:.3f} ms/step ({pf.num_stages}+{pa.num_stages} launches)", file=sys.stderr,
-              flush=True)
+        res["plain_note"] = ("one launch per level with the same register-tile kernels (levels_per_pass = 1), "
+                             "short bands one level per LDS launch; hipGraph replay like the fused step")
+        print(f"[bench] plain (register tiles, one level per launch): {pms:.3f} ms/step graph, {pms_eager:.3f} "
+              f"eager ({pf.num_stages}+{pa.num_stages} stages)", file=sys.stderr, flush=True)
         pf.close()
         pa.close()
+        qf = BatchPlan(sigs, la.Transform.Forward, dtype=torch.float32, plain=True, lds_only=True)
+        qa = BatchPlan(sigs, la.Transform.ForwardAdjoint, dtype=torch.float32, plain=True, lds_only=True)
+        qms, _, _ = time_plans([qf, qa], x, [y, z], args.plain_steps, 1)
+        res["plain_lds_ms_per_step"] = round(qms, 4)
+        res["vs_plain_lds"] = round(qms / ms, 3)
+        print(f"[bench] plain LDS-only: {qms:.3f} ms/step eager", file=sys.stderr, flush=True)
+        qf.close()
+        qa.close()
     fwd.close()
     adj.close()
     if not args.no_cpu:

@@ -35,8 +35,10 @@ extern "C" {
 #define LFX_BF16 2
 #define LFX_F16 3
 
-/* lfx_desc.flags */
-#define LFX_FLAG_PLAIN 1u /* one launch per level (no multi-level fusion) */
+/* lfx_desc.flags and lfx_batch_plan_create flags */
+#define LFX_FLAG_PLAIN 1u    /* one launch per level (no multi-level fusion); float32 batched
+                                plans still run each level as register tiles */
+#define LFX_FLAG_LDS_ONLY 2u /* batched plans: LDS kernels only (no register tiles) */
 
 typedef struct {
   uint32_t wavelet;         /* lifted::Wavelet value, or an id from lfx_program_create */

@@ -11,6 +11,7 @@ LIB_PATH = os.path.join(HERE, "liblifted_hip.so")
 
 LFX_F32, LFX_F64, LFX_BF16, LFX_F16 = 0, 1, 2, 3
 LFX_FLAG_PLAIN = 1
+LFX_FLAG_LDS_ONLY = 2
 
 
 class LfxError(RuntimeError):

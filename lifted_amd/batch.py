@@ -52,13 +52,15 @@ class BatchPlan:
     tuning: planning knobs from lifted_amd.tune.tune_batch (None: defaults).
     """
 
-    def __init__(self, signals, transform, mode=0, dtype=None, plain=False, tuning=None):
-        """tuning: planning knobs (see lifted_amd.tune) used for this plan only."""
+    def __init__(self, signals, transform, mode=0, dtype=None, plain=False, tuning=None, lds_only=False):
+        """plain: one launch per level (no multi-level fusion; float32 plans
+        still run each level as register tiles). lds_only: LDS kernels only.
+        tuning: planning knobs (see lifted_amd.tune) used for this plan only."""
         import torch
         if tuning:
             from . import tune
             with tune.tuning(**tuning):
-                self.__init__(signals, transform, mode, dtype, plain)
+                self.__init__(signals, transform, mode, dtype, plain, None, lds_only)
             return
         L = _bind()
         sigs = list(signals)
@@ -68,8 +70,9 @@ class BatchPlan:
         self.dtype = dtype or torch.float32
         storage = _storage_torch(self.dtype)
         h = ctypes.c_void_p()
-        _lib.check(L.lfx_batch_plan_create(arr, len(sigs), int(transform), int(mode), storage,
-                                           LFX_FLAG_PLAIN if plain else 0, ctypes.byref(h)))
+        flags = (LFX_FLAG_PLAIN if plain else 0) | (_lib.LFX_FLAG_LDS_ONLY if lds_only else 0)
+        _lib.check(L.lfx_batch_plan_create(arr, len(sigs), int(transform), int(mode), storage, flags,
+                                           ctypes.byref(h)))
         self._h = h
         self.transform = int(transform)
         self.signals = sigs

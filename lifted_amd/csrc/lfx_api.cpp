@@ -124,13 +124,30 @@ static Plan make_plan(const lfx_desc* d) {
 }
 
 // ---------------------------------------------------------------------------
-// workspace: one grow-only device buffer per device. Work that uses it is
-// serialized across streams with an event (not while a stream is capturing).
+// workspace: one grow-only device buffer per device, used by one call at a
+// time. A lease holds the workspace mutex from the moment it is handed out
+// until the call has enqueued all its work and recorded the workspace's event
+// on its stream; the next lease makes its stream wait on that event. So two
+// host threads on two streams never share the buffer, and the buffer is only
+// replaced (grown) by the thread that holds it, after a device
+// synchronization. Under stream capture a lease neither waits on nor records
+// the event (capture cannot); a buffer that a captured graph may use is never
+// freed (kept until the process ends), and it is not grown during a capture.
+
+static std::mutex g_ws_mu;  // taken after g_line_mu when both are held
+
+static bool stream_capturing(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  hip_check(hipStreamIsCapturing(st, &cs), "hipStreamIsCapturing");
+  return cs != hipStreamCaptureStatusNone;
+}
 
 struct Workspace {
   void* p = nullptr;
   size_t bytes = 0;
   hipEvent_t ev = nullptr;
+  bool captured = false;           // p is referenced by a captured graph
+  std::vector<void*> retired;      // replaced buffers a captured graph may still use
 };
 static std::map<int, Workspace> g_ws;
 
@@ -138,22 +155,37 @@ struct WsLease {
   void* p = nullptr;
   hipStream_t st = nullptr;
   hipEvent_t ev = nullptr;
+  std::unique_lock<std::mutex> lk;
+  WsLease() = default;
+  WsLease(const WsLease&) = delete;
+  WsLease& operator=(const WsLease&) = delete;
   ~WsLease() {
     if (ev) (void)hipEventRecord(ev, st);
-  }
+  }  // the mutex is released after the event is recorded (member destroyed last)
 };
 
 static void lease_workspace(size_t bytes, hipStream_t st, WsLease& lease) {
   int dev = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
-  std::lock_guard<std::mutex> lk(g_mu);
-  Workspace& w = g_ws[dev];
+  const bool capturing = stream_capturing(st);
+  lease.lk = std::unique_lock<std::mutex>(g_ws_mu);
+  Workspace* wp = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    wp = &g_ws[dev];
+  }
+  Workspace& w = *wp;
   if (w.bytes < bytes) {
+    if (capturing)
+      fail(LFX_EUNSUPPORTED, "the transform needs a larger workspace (" + std::to_string(bytes) +
+                                 " bytes) while the stream is capturing: run it once outside capture first");
     if (w.p) {
       hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
-      (void)hipFree(w.p);
+      if (w.captured) w.retired.push_back(w.p);
+      else (void)hipFree(w.p);
       w.p = nullptr;
       w.bytes = 0;
+      w.captured = false;
     }
     const size_t want = std::max(bytes, (size_t)1 << 20);
     if (hipMalloc(&w.p, want) != hipSuccess) {
@@ -162,12 +194,15 @@ static void lease_workspace(size_t bytes, hipStream_t st, WsLease& lease) {
     }
     w.bytes = want;
   }
-  if (!w.ev) hip_check(hipEventCreateWithFlags(&w.ev, hipEventDisableTiming), "hipEventCreate");
-  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  hip_check(hipStreamIsCapturing(st, &cs), "hipStreamIsCapturing");
-  if (cs == hipStreamCaptureStatusNone) {
+  if (!w.ev) {
+    if (capturing) fail(LFX_EUNSUPPORTED, "run the transform once outside stream capture first");
+    hip_check(hipEventCreateWithFlags(&w.ev, hipEventDisableTiming), "hipEventCreate");
+  }
+  if (!capturing) {
     hip_check(hipStreamWaitEvent(st, w.ev, 0), "hipStreamWaitEvent");
     lease.ev = w.ev;
+  } else {
+    w.captured = true;
   }
   lease.p = w.p;
   lease.st = st;
@@ -698,7 +733,11 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
   plan->bc = bc;
   plan->storage = storage;
   plan->fwd = tf == kForward || tf == kInverseAdjoint;
+  // plain: one level per group (one launch per level, the band round-tripping
+  // through memory between levels); register tiles still run each level
+  // unless LFX_FLAG_LDS_ONLY
   const bool plain = (flags & LFX_FLAG_PLAIN) != 0;
+  const bool lds_only = (flags & LFX_FLAG_LDS_ONLY) != 0;
   const size_t es = storage_bytes(storage), cs = compute_bytes(storage);
   std::vector<std::vector<BatchItem>> per_stage;
   std::vector<std::vector<double>> cost;
@@ -735,8 +774,8 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
     if (L == 0 && sg.length > 0) fail(LFX_EINVAL, "batched signals need at least one level");
     levels[i] = L;
     if (sg.length == 0 || L == 0) continue;
-    const bool reg = storage == kF32 && !plain && sg.wavelet < kCustomBase;
-    const bool rows = reg && nrep >= 4 && bc == kZero;
+    const bool reg = storage == kF32 && !lds_only && sg.wavelet < kCustomBase;
+    const bool rows = reg && !plain && nrep >= 4 && bc == kZero;
     groups[i] = cs == 8 ? signal_groups<double>(w, tf, storage, bc, sg.length, L, plain, reg)
                         : signal_groups<float>(w, tf, storage, bc, sg.length, L, plain, reg, rows);
     Gmax = std::max(Gmax, (int)groups[i].size());
@@ -748,7 +787,7 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
     plan->total += sg.length;
     if (sg.length == 0 || L == 0) continue;
     const void* prog = device_program(sg.wavelet, tf, storage);
-    const bool reg = storage == kF32 && !plain && sg.wavelet < kCustomBase;
+    const bool reg = storage == kF32 && !lds_only && sg.wavelet < kCustomBase;
     const std::vector<Group>& gs = groups[i];
     const DProgram<float> hpf = resolve<float>(w, tf);
     const DProgram<double> hpd = resolve<double>(w, tf);
@@ -1122,42 +1161,68 @@ static void batch_stage(BatchPlanImpl* p, int st, const void* in, void* out, hip
 // back. Plans are cached per (device, lines, length, wavelet, levels,
 // transform, boundary).
 
+// The key holds the calling thread's planning knobs too: a plan built under
+// one tuning is not reused by a call made under another.
 struct LinePlanKey {
   int dev;
   long long nlines, n, L;
   uint32_t wavelet;
   int tf, bc;
+  long long tile, line, small;
+  int k, streams;
+  static LinePlanKey make(int dev, long long nlines, long long n, long long L, uint32_t wavelet, int tf, int bc,
+                          const Tuning& t) {
+    return LinePlanKey{dev, nlines, n, L, wavelet, tf, bc, t.tile, t.line, t.small, t.k, t.streams};
+  }
   bool operator<(const LinePlanKey& o) const {
-    return std::tie(dev, nlines, n, L, wavelet, tf, bc) < std::tie(o.dev, o.nlines, o.n, o.L, o.wavelet, o.tf, o.bc);
+    return std::tie(dev, nlines, n, L, wavelet, tf, bc, tile, line, small, k, streams) <
+           std::tie(o.dev, o.nlines, o.n, o.L, o.wavelet, o.tf, o.bc, o.tile, o.line, o.small, o.k, o.streams);
   }
 };
-static std::map<LinePlanKey, std::pair<BatchPlanImpl*, unsigned long long>> g_line_plans;
+struct CachedPlan {
+  BatchPlanImpl* plan;
+  unsigned long long tick;
+  bool captured;  // used during a stream capture: a graph may replay it
+};
+static std::map<LinePlanKey, CachedPlan> g_line_plans;
+static std::vector<BatchPlanImpl*> g_retired_plans;  // evicted after a capture used them: kept
 static unsigned long long g_line_tick = 0;
 static constexpr size_t kLinePlanCache = 32;
 // Guards the line-plan cache and each cached plan's launches (a plan's fork /
 // join events must not interleave between host threads). Distinct from g_mu:
 // batch_create takes g_mu (device programs), so holding g_mu here would
-// self-deadlock.
+// self-deadlock. Lock order: g_line_mu, then g_ws_mu.
 static std::mutex g_line_mu;
 
-// Caller holds g_line_mu.
-static BatchPlanImpl* line_plan(const LinePlanKey& k) {
+// Caller holds g_line_mu. Creating or evicting a plan allocates, frees and
+// synchronizes, none of which a capturing stream allows: under capture only
+// cached plans are used.
+static BatchPlanImpl* line_plan(const LinePlanKey& k, hipStream_t st) {
+  const bool capturing = stream_capturing(st);
   auto it = g_line_plans.find(k);
   if (it != g_line_plans.end()) {
-    it->second.second = ++g_line_tick;
-    return it->second.first;
+    it->second.tick = ++g_line_tick;
+    it->second.captured = it->second.captured || capturing;
+    return it->second.plan;
   }
+  if (capturing)
+    fail(LFX_EUNSUPPORTED, "no cached plan for this transform while the stream is capturing: run it once outside "
+                           "capture first");
   if (g_line_plans.size() >= kLinePlanCache) {
     auto old = g_line_plans.begin();
     for (auto j = g_line_plans.begin(); j != g_line_plans.end(); ++j)
-      if (j->second.second < old->second.second) old = j;
-    (void)hipDeviceSynchronize();  // the evicted plan may still be in flight on a caller's stream
-    free_plan(old->second.first);
+      if (j->second.tick < old->second.tick) old = j;
+    if (old->second.captured) {
+      g_retired_plans.push_back(old->second.plan);
+    } else {
+      (void)hipDeviceSynchronize();  // the evicted plan may still be in flight on a caller's stream
+      free_plan(old->second.plan);
+    }
     g_line_plans.erase(old);
   }
   lfx_signal sg{0, k.n, k.wavelet, (int32_t)k.L};
   BatchPlanImpl* bp = batch_create(&sg, 1, k.tf, k.bc, kF32, 0, k.nlines, k.n);
-  g_line_plans[k] = {bp, ++g_line_tick};
+  g_line_plans[k] = CachedPlan{bp, ++g_line_tick, false};
   return bp;
 }
 
@@ -1219,7 +1284,7 @@ static bool lines_fast(const Plan& p, const long long* shp, int ax, long long L,
   int dev = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
   std::lock_guard<std::mutex> lk(g_line_mu);
-  BatchPlanImpl* bp = line_plan(LinePlanKey{dev, nlines, n, L, p.wavelet, p.tf, p.bc});
+  BatchPlanImpl* bp = line_plan(LinePlanKey::make(dev, nlines, n, L, p.wavelet, p.tf, p.bc, g_tuning), st);
   // A float32 side whose lines are already packed along the axis (its strides
   // equal the dense layout's on every dim longer than 1) is used directly:
   // the plan reads the input there, or writes the output there, saving that
@@ -1325,7 +1390,7 @@ static bool sep_chain(const Plan& p, const long long* shape, const long long* si
                 "permute kernel");
       pin = buf[in_buf];
     }
-    BatchPlanImpl* bp = line_plan(LinePlanKey{dev, nlines, n, L[ia], p.wavelet, p.tf, p.bc});
+    BatchPlanImpl* bp = line_plan(LinePlanKey::make(dev, nlines, n, L[ia], p.wavelet, p.tf, p.bc, g_tuning), st);
     const int out_buf = in_buf < 0 ? 0 : 1 - in_buf;
     if (last) {
       // pin is a workspace here (the first axis is never the last)

@@ -47,6 +47,14 @@ __device__ __forceinline__ double fmac<double>(double a, double b, double c) { r
 // Boundary map of the reference (src/lifted-common.hpp:719-779) on one band of
 // length n: returns the mapped index, or -1 when the term is skipped (Zero BC).
 // Periodic keeps the reference's unsigned modulo for negative indices.
+//
+// The reference iterates its Symmetric / Reflect mirrors until the index lands
+// in the band. The iteration is the even extension of the band, so it has a
+// closed form: Symmetric (half-sample mirrors at -1/2 and n - 1/2) has period
+// 2n, Reflect (whole-sample mirrors at 0 and n - 1) period 2n - 2. The closed
+// form has no loop, so a degenerate band (n = 0, or n = 1 under Reflect, where
+// the reference never terminates and the host refuses the call) cannot spin a
+// wave: it yields -1 (skip) or 0 instead.
 __device__ __forceinline__ long long bc_map(int bc, long long m, long long n) {
   if (m >= 0 && m < n) return m;
   switch (bc) {
@@ -55,16 +63,21 @@ __device__ __forceinline__ long long bc_map(int bc, long long m, long long n) {
     case kConstant:
       return m < 0 ? 0 : n - 1;
     case kPeriodic:
-      return (long long)((unsigned long long)m % (unsigned long long)n);
+      return n > 0 ? (long long)((unsigned long long)m % (unsigned long long)n) : -1;
     case kSymmetric: {
-      long long io = m;
-      while (io >= n || io < 0) io = io < 0 ? -(io + 1) : 2 * (n - 1) - (io - 1);
-      return io;
+      if (n <= 0) return -1;
+      const long long p = 2 * n;
+      long long r = m % p;
+      if (r < 0) r += p;
+      return r < n ? r : p - 1 - r;
     }
     default: {
-      long long io = m;
-      while (io >= n || io < 0) io = io < 0 ? -io : 2 * (n - 1) - io;
-      return io;
+      if (n <= 0) return -1;
+      if (n == 1) return 0;
+      const long long p = 2 * n - 2;
+      long long r = m % p;
+      if (r < 0) r += p;
+      return r < n ? r : p - r;
     }
   }
 }

@@ -5,12 +5,16 @@
 //   - <Forward x, y> == <x, ForwardAdjoint y>
 //   - that invalid arguments throw std::invalid_argument.
 // Mode "errors" runs only the argument checks that fail before any device
-// work; mode "all" runs everything and prints one line of results.
+// work; mode "all" runs everything and prints one line of results; mode
+// "dump DIR" runs one call of each overload on seeded inputs and writes the
+// inputs and outputs as raw little-endian arrays (x_<case>.bin,
+// y_<case>.bin) for the test suite to compare with the oracle.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <random>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "lifted.hpp"
@@ -93,7 +97,71 @@ static int errors() {
   return ok;
 }
 
+template <class T>
+static void write_raw(const std::string& path, const std::vector<T>& v) {
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f || std::fwrite(v.data(), sizeof(T), v.size(), f) != v.size()) throw std::runtime_error("cannot write " + path);
+  std::fclose(f);
+}
+
+template <class T>
+static std::vector<T> seeded(size_t n, unsigned seed) {
+  std::mt19937_64 g(seed);
+  std::normal_distribution<double> nd;
+  std::vector<T> x(n);
+  for (auto& v : x) v = (T)nd(g);
+  return x;
+}
+
+// One call of each lifted::lwt overload (src/lifted.hpp:13-39) on a 33 x 40 x 27
+// box, axes {0, 2, 1}: level-by-level float / double (scalar level 2) and
+// separable float / double (levels {2, 3, 1}); the double separable call reads
+// a Fortran-ordered input and writes a C-ordered output.
+static int dump(const std::string& dir) {
+  const size_v shape{33, 40, 27};
+  const stride_v c_st{40 * 27, 27, 1};
+  const stride_v f_st{1, 33, 33 * 40};
+  const size_v axes{0, 2, 1};
+  const size_v lv{2, 3, 1};
+  const size_t n = 33 * 40 * 27;
+  {
+    auto x = seeded<float>(n, 11);
+    std::vector<float> y(n);
+    lifted::lwt(Wavelet::CDF9_7, Transform::Forward, BoundaryCondition::Symmetric, shape, c_st, c_st, axes, (size_t)2,
+                x.data(), y.data());
+    write_raw(dir + "/x_lbl_f32.bin", x);
+    write_raw(dir + "/y_lbl_f32.bin", y);
+  }
+  {
+    auto x = seeded<double>(n, 12);
+    std::vector<double> y(n);
+    lifted::lwt(Wavelet::Daubechies4, Transform::Inverse, BoundaryCondition::Zero, shape, c_st, c_st, axes, (size_t)2,
+                x.data(), y.data());
+    write_raw(dir + "/x_lbl_f64.bin", x);
+    write_raw(dir + "/y_lbl_f64.bin", y);
+  }
+  {
+    auto x = seeded<float>(n, 13);
+    std::vector<float> y(n);
+    lifted::lwt(Wavelet::Bior4_4, Transform::ForwardAdjoint, BoundaryCondition::Constant, shape, c_st, c_st, axes, lv,
+                x.data(), y.data());
+    write_raw(dir + "/x_sep_f32.bin", x);
+    write_raw(dir + "/y_sep_f32.bin", y);
+  }
+  {
+    auto x = seeded<double>(n, 14);  // element (i, j, k) at i + 33 j + 1320 k
+    std::vector<double> y(n);
+    lifted::lwt(Wavelet::Symlet5, Transform::InverseAdjoint, BoundaryCondition::Reflect, shape, f_st, c_st, axes, lv,
+                x.data(), y.data());
+    write_raw(dir + "/x_sep_f64.bin", x);
+    write_raw(dir + "/y_sep_f64.bin", y);
+  }
+  std::printf("dumped 4 cases to %s\n", dir.c_str());
+  return 0;
+}
+
 int main(int argc, char** argv) {
+  if (argc > 2 && std::strcmp(argv[1], "dump") == 0) return dump(argv[2]);
   const bool all = argc > 1 && std::strcmp(argv[1], "all") == 0;
   const int ok = errors();
   std::printf("errors_thrown %d/3 max_level_db2_65536 %zu\n", ok, lifted::max_level(Wavelet::Daubechies2, 65536));

@@ -34,6 +34,66 @@ def max_ulp_diff(a, b):
     return float(np.max(np.abs(a.astype(np.float64) - b.astype(np.float64))) / float(ulp))
 
 
+# Every parity comparison of the session: (test id, what, bit-exact, worst
+# per-band error in units of the band's own ulp). tests/conftest.py prints a
+# summary of it at the end of the run (also under -q).
+PARITY = []
+
+
+def band_slices(shape, axes, level):
+    """Index tuples of every coefficient band of a transform's output: the
+    separable layout for a sequence `level`, the level-by-level sub-boxes for
+    an int (lifted_amd.layout)."""
+    from lifted_amd import layout
+    ndim = len(shape)
+    axes = list(range(ndim)) if axes is None else ([axes] if isinstance(axes, int) else list(axes))
+    if isinstance(level, (int, np.integer)):
+        approx, details = layout.level_by_level_bands(shape, axes, int(level))
+        return [approx] + [idx for lvl in details for idx in lvl.values()]
+    return list(layout.separable_bands(shape, axes, [int(v) for v in level]).values())
+
+
+def band_ulp_error(got, ref, bands=None):
+    """Worst error over the bands, each in units of the last place of that
+    band's own max |ref| (so a small detail band is not hidden behind a large
+    approximation band); bands=None: the whole array is one band."""
+    got = np.asarray(got)
+    ref = np.asarray(ref)
+    worst = 0.0
+    for idx in (bands if bands is not None else [Ellipsis]):
+        g = got[idx].astype(np.float64)
+        r = ref[idx]
+        if r.size == 0:
+            continue
+        scale = float(np.max(np.abs(r)))
+        diff = float(np.max(np.abs(g - r.astype(np.float64))))
+        if diff == 0.0:
+            continue
+        if scale == 0.0:
+            return np.inf
+        worst = max(worst, diff / float(np.spacing(np.asarray(scale, dtype=r.dtype))))
+    return worst
+
+
+def assert_parity(got, ref, what, bands=None, exact=True, tol_ulp=4.0):
+    """GPU result against the oracle. exact=True (the GPU and the oracle run
+    the same operations in the same order): bit for bit. exact=False: at most
+    tol_ulp units of the last place of each band's own max |ref|. The measured
+    worst per-band error is recorded for the session summary either way."""
+    got = np.asarray(got)
+    ref = np.asarray(ref)
+    assert got.shape == ref.shape, f"{what}: shape {got.shape} != {ref.shape}"
+    same = bool(np.array_equal(got, ref))
+    err = 0.0 if same else band_ulp_error(got, ref, bands)
+    import os
+    PARITY.append((os.environ.get("PYTEST_CURRENT_TEST", "?").split(" ")[0], what, same, err))
+    if exact:
+        assert same, f"{what}: not bit-exact, worst {err:.3g} ulp of a band's max |ref|"
+    else:
+        assert err <= tol_ulp, f"{what}: {err:.3g} ulp of a band's max |ref| (bar {tol_ulp})"
+    return err
+
+
 # Mock wavelets of the reference's step tests (tests/test_helpers.h:102-360),
 # as step programs: kind, target, offset, taps / sign / n / value.
 def _g(t, off, *taps):
@@ -103,3 +163,49 @@ def c_strides(shape):
     for i in range(len(shape) - 2, -1, -1):
         st[i] = st[i + 1] * shape[i + 1]
     return st
+
+
+def round_to(x, storage):
+    """float32 values rounded to a 16-bit storage type ("bfloat16" /
+    "float16", round to nearest even) and widened back to float32."""
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(getattr(torch, storage)).float().numpy()
+
+
+def oracle_level_by_level_rounded(oracle, x, wavelet, transform, bc, axes, level, storage):
+    """What the level-by-level path computes for 16-bit storage with several
+    axes: every level reads its sub-box [0, n_(l-1)) of the transformed axes
+    from the storage type, runs one level along each axis in order in float32,
+    and rounds the sub-box once to the storage type (Forward / InverseAdjoint
+    walk the levels fine -> coarse, Inverse / ForwardAdjoint coarse -> fine,
+    src/lifted-drivers-inl.hpp:488-610). x holds values of the storage type
+    (as float32); returns float32 values of the storage type."""
+    x = np.array(x, dtype=np.float32, copy=True)
+    boxes, box = [], list(x.shape)
+    for _ in range(int(level)):
+        boxes.append(tuple(box))
+        for a in axes:
+            box[a] = box[a] - box[a] // 2
+    order = range(level) if transform in (0, 3) else reversed(range(level))
+    for l in order:
+        idx = tuple(slice(0, b) for b in boxes[l])
+        sub = np.ascontiguousarray(x[idx])
+        x[idx] = round_to(oracle.lwt(sub, wavelet, transform, bc, axes=list(axes), level=1), storage)
+    return x
+
+
+def field_1f2(shape, seed):
+    """Seeded 1/f^2 random fields over the last two dims (white noise, Fourier
+    amplitudes times 1/|k|^2 with k = 0 zeroed), each scaled to zero mean and
+    unit variance; float32."""
+    g = np.random.default_rng(seed)
+    ny, nx = shape[-2], shape[-1]
+    ky = np.fft.fftfreq(ny)[:, None]
+    kx = np.fft.rfftfreq(nx)[None, :]
+    k2 = ky * ky + kx * kx
+    filt = np.where(k2 > 0, 1.0 / np.where(k2 > 0, k2, 1.0), 0.0)
+    out = np.empty(shape, np.float32)
+    for idx in np.ndindex(*shape[:-2]):
+        f = np.fft.irfft2(np.fft.rfft2(g.standard_normal((ny, nx))) * filt, s=(ny, nx))
+        out[idx] = ((f - f.mean()) / f.std()).astype(np.float32)
+    return out

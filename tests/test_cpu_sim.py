@@ -263,3 +263,14 @@ def test_xcd_block_renumbering(sim):
     with each XCD's workgroups numbered consecutively."""
     r = subprocess.run([os.path.join(os.path.dirname(sim), "xcd_check")], capture_output=True)
     assert r.returncode == 0 and r.stdout.decode().strip() == "ok", r.stdout.decode() + r.stderr.decode()
+
+
+def test_boundary_maps_closed_form(sim):
+    """The kernels' Symmetric / Reflect maps are closed forms (periods 2n and
+    2n - 2, no loop). They must equal the reference's iterated mirrors
+    (src/lifted-common.hpp:750-777) at every index |i| <= 4n + 3 for n >= 1
+    (Symmetric) and n >= 2 (Reflect), and return at once on degenerate bands."""
+    r = subprocess.run([os.path.join(os.path.dirname(sim), "bcmap_check")], capture_output=True, timeout=60)
+    out = r.stdout.decode()
+    assert r.returncode == 0 and out.startswith("ok"), out + r.stderr.decode()
+    print(out.strip())

@@ -5,7 +5,12 @@ import itertools
 import numpy as np
 import pytest
 
-from helpers import WAVELETS
+from helpers import WAVELETS, assert_parity
+
+
+def line_bands(n, L):
+    from lifted_amd.layout import line_bands as lb
+    return lb(n, L)
 
 pytestmark = pytest.mark.gpu
 
@@ -22,21 +27,26 @@ def make_signals(seed, nsig, lo, hi, oracle):
     return sigs, off
 
 
-@pytest.mark.parametrize("plain", [False, True])
+@pytest.mark.parametrize("variant", ["fused", "plain", "plain_lds"])
 @pytest.mark.parametrize("tf", [0, 1, 2, 3])
-def test_batch_matches_oracle(gpu, oracle, tf, plain):
+def test_batch_matches_oracle(gpu, oracle, tf, variant):
+    """Fused plans, plain plans (one launch per level, register tiles) and
+    plain LDS-only plans against the oracle per signal, bit for bit."""
     import torch
     from lifted_amd.batch import BatchPlan
+    plain = variant != "fused"
     sigs, total = make_signals(10 + tf, 40, 16, 300000, oracle)
     for bc in (0, 3):
         x = np.random.default_rng(1).standard_normal(total).astype(np.float32)
-        plan = BatchPlan(sigs, tf, mode=bc, dtype=torch.float32, plain=plain)
+        plan = BatchPlan(sigs, tf, mode=bc, dtype=torch.float32, plain=plain, lds_only=variant == "plain_lds")
+        if plain:
+            assert plan.num_stages >= max(L for _, _, _, L in sigs)
         y = plan.execute(torch.from_numpy(x).to(gpu)).cpu().numpy()
         for off, n, w, L in sigs:
             ref = oracle.lwt(x[off:off + n], w, tf, bc, level=[L])
             got = y[off:off + n]
-            scale = max(np.abs(ref).max(), 1e-30)
-            assert np.abs(got - ref).max() <= 4 * np.finfo(np.float32).eps * scale, (off, n, w, L, tf, bc)
+            assert_parity(got, ref, f"off={off} n={n} w={w} L={L} tf={tf} bc={bc} {variant}",
+                          bands=list(line_bands(n, L).values()))
 
 
 @pytest.mark.parametrize("line", [16, 64])
@@ -63,8 +73,8 @@ def test_register_tiles_down_to_short_bands(gpu, oracle, line):
         y = plan.execute(xd).cpu().numpy()
         for o, n, w, L in sigs:
             ref = oracle.lwt(x[o:o + n], w, tf, 0, level=[L])
-            scale = max(np.abs(ref).max(), 1e-30)
-            assert np.abs(y[o:o + n] - ref).max() <= 4 * np.finfo(np.float32).eps * scale, (line, tf, n, w, L)
+            assert_parity(y[o:o + n], ref, f"line={line} tf={tf} n={n} w={w} L={L}",
+                          bands=list(line_bands(n, L).values()))
 
 
 def test_batch_fp64_and_graph_replay(gpu, oracle):
@@ -84,7 +94,7 @@ def test_batch_fp64_and_graph_replay(gpu, oracle):
     xn = x.cpu().numpy()
     for off, n, w, L in sigs:
         r = oracle.lwt(xn[off:off + n], w, 0, 0, level=[L])
-        assert np.abs(ye.cpu().numpy()[off:off + n] - r).max() <= 1e-12 * np.abs(r).max()
+        assert_parity(ye.cpu().numpy()[off:off + n], r, f"fp64 n={n} w={w} L={L}")
     s = torch.cuda.Stream()
     s.wait_stream(torch.cuda.current_stream())
     g = torch.cuda.CUDAGraph()
@@ -156,8 +166,7 @@ def test_autotuned_plans_match_oracle(gpu, oracle):
             y = BatchPlan(sigs, tf, dtype=torch.float32, tuning=knobs).execute(x).cpu().numpy()
             for off, n, w, L in sigs:
                 ref = oracle.lwt(xn[off:off + n], w, tf, 0, level=[L])
-                scale = max(np.abs(ref).max(), 1e-30)
-                assert np.abs(y[off:off + n] - ref).max() <= 4 * np.finfo(np.float32).eps * scale, (knobs, tf, w)
+                assert_parity(y[off:off + n], ref, f"{knobs} tf={tf} n={n} w={w} L={L}")
 
 
 def test_stream_copy_reference_is_exact(gpu):
@@ -222,7 +231,6 @@ def test_batch_custom_programs_periodic_and_max_level(gpu, oracle):
                 ref = oracle.lwt(x[o:o + n], 0, tf, bc, level=[L], steps=st, width=8)
             else:
                 ref = oracle.lwt(x[o:o + n], w, tf, bc, level=[L if L else oracle.max_level(w, n)])
-            scale = max(np.abs(ref).max(), 1e-30)
-            assert np.abs(y[o:o + n] - ref).max() <= 4 * np.finfo(np.float32).eps * scale, (bc, tf, n, w, L)
+            assert_parity(y[o:o + n], ref, f"bc={bc} tf={tf} n={n} w={w} L={L}")
     with pytest.raises(la.LfxError):
         BatchPlan([(0, 300000, 100907, 3)], 0, mode=2, dtype=torch.float32)

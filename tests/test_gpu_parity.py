@@ -1,10 +1,12 @@
 """GPU parity: the HIP path (through the C ABI) against the oracle on the same
 seeded inputs.
 
-Bar: float64 within 1e-12 relative to max|ref|, float32 within 4 ulp of
-max|ref|. Both paths perform the same operations in the same order with the
-same fused multiply-adds, so the expected difference is zero; the measured
-maximum is asserted against those tolerances and printed.
+Both paths perform the same operations in the same order with the same fused
+multiply-adds, so every comparison here is bit for bit (the contract's bar,
+4 ulp in float32 and 1e-12 relative in float64, is looser). A failure reports
+the worst error in units of the last place of each coefficient band's own
+max |ref| (bands from lifted_amd.layout), and the session summary prints the
+measured worst case of every test file.
 """
 import itertools
 import os
@@ -12,20 +14,21 @@ import os
 import numpy as np
 import pytest
 
-from helpers import (BCS, MOCK_WAVELETS, WAVELETS, WIDTHS, fill_sin, max_ulp_diff, oracle_steps, rng)
+from helpers import (BCS, MOCK_WAVELETS, WAVELETS, WIDTHS, assert_parity, band_slices, fill_sin,
+                     oracle_level_by_level_rounded, oracle_steps, rng)
 
 pytestmark = pytest.mark.gpu
 F, I, FA, IA = 0, 1, 2, 3
 
 
-def _check(got, ref, dtype, what):
-    if dtype == np.float64:
-        scale = max(float(np.abs(ref).max()), 1e-300)
-        err = float(np.abs(got - ref).max()) / scale
-        assert err <= 1e-12, f"{what}: rel err {err}"
-    else:
-        u = max_ulp_diff(got, ref)
-        assert u <= 4.0, f"{what}: {u} ulp"
+def _check(got, ref, dtype, what, axes=None, level=None):
+    """Bit for bit; on failure the message gives the worst per-band error
+    (bands of the transform given by axes / level, else the whole array)."""
+    got = np.asarray(got)
+    ref = np.asarray(ref)
+    assert got.dtype == ref.dtype == dtype, f"{what}: dtype {got.dtype} / {ref.dtype}"
+    bands = band_slices(ref.shape, axes, level) if level is not None else None
+    assert_parity(got, ref, what, bands=bands)
 
 
 def _gpu(x_np, dev):
@@ -57,7 +60,7 @@ def test_1d_all_wavelets_bcs_transforms(gpu, oracle, w, dtype):
         x = g.standard_normal(n).astype(dtype)
         ref = oracle.lwt(x, w, tf, bc, level=[L])
         got = la.lwt(_gpu(x, gpu), w, tf, mode=bc, level=[L]).cpu().numpy()
-        _check(got, ref, dtype, f"w={w} n={n} L={L} bc={bc} tf={tf}")
+        _check(got, ref, dtype, f"w={w} n={n} L={L} bc={bc} tf={tf}", axes=[0], level=[L])
 
 
 @pytest.mark.parametrize("name", list(MOCK_WAVELETS))
@@ -93,7 +96,8 @@ def test_nd_shapes_of_reference_tests(gpu, oracle, shape, axes):
         lvl = [level] * len(axes) if sep else level
         ref = oracle.lwt(x, w, tf, bc, axes=list(axes), level=lvl)
         got = la.lwt(_gpu(x, gpu), w, tf, mode=bc, axes=list(axes), level=lvl).cpu().numpy()
-        _check(got, ref, np.float64, f"shape={shape} axes={axes} w={w} L={level} tf={tf} sep={sep}")
+        _check(got, ref, np.float64, f"shape={shape} axes={axes} w={w} L={level} tf={tf} sep={sep}", axes=list(axes),
+               level=lvl)
 
 
 def test_layouts_fortran_strided_inplace(gpu, oracle):
@@ -197,7 +201,7 @@ def test_long_lines_tiled(gpu, oracle, tf):
         x = g.standard_normal(n).astype(dtype)
         ref = oracle.lwt(x, w, tf, bc, level=[L])
         got = la.lwt(_gpu(x, gpu), w, tf, mode=bc, level=[L]).cpu().numpy()
-        _check(got, ref, dtype, f"w={w} bc={bc} n={n} L={L} tf={tf}")
+        _check(got, ref, dtype, f"w={w} bc={bc} n={n} L={L} tf={tf}", axes=[0], level=[L])
 
 
 def test_long_lines_2d_strided_axis(gpu, oracle):
@@ -226,6 +230,31 @@ def test_cpp_dropin_header_round_trips(gpu):
     assert r.returncode == 0, r.stdout + r.stderr
 
 
+@pytest.mark.gpu
+def test_cpp_dropin_overloads_match_oracle(gpu, oracle, tmp_path):
+    """One call of each of the four lifted::lwt overloads of include/lifted.hpp
+    (lfx_lwt_f32 / _f64 / _sep_f32 / _sep_f64 underneath) on seeded host arrays,
+    written out by the shim and compared here with the oracle bit for bit:
+    level-by-level float CDF 9/7 Symmetric Forward, level-by-level double
+    Daubechies4 Zero Inverse, separable float Bior4_4 Constant ForwardAdjoint,
+    separable double Symlet5 Reflect InverseAdjoint from a Fortran-ordered
+    input into a C-ordered output."""
+    import subprocess
+    d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp_shim")
+    subprocess.run(["make", "-s", "-C", d], check=True)
+    r = subprocess.run([os.path.join(d, "build", "shim_main"), "dump", str(tmp_path)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    shape, axes = (33, 40, 27), [0, 2, 1]
+    cases = [("lbl_f32", np.float32, 100907, F, 3, 2, "C"), ("lbl_f64", np.float64, 4, I, 0, 2, "C"),
+             ("sep_f32", np.float32, 10404, FA, 1, [2, 3, 1], "C"), ("sep_f64", np.float64, 105, IA, 4, [2, 3, 1], "F")]
+    for name, dt, w, tf, bc, lvl, order in cases:
+        x = np.fromfile(tmp_path / f"x_{name}.bin", dtype=dt).reshape(shape, order=order)
+        got = np.fromfile(tmp_path / f"y_{name}.bin", dtype=dt).reshape(shape)
+        ref = oracle.lwt(x, w, tf, bc, axes=axes, level=lvl, out=np.empty(shape, dt))
+        _check(got, ref, dt, f"lifted::lwt {name}", axes=axes, level=lvl)
+
+
 @pytest.mark.parametrize("shape,axes,levels", [((37, 300), (1, 0), (3, 4)), ((64, 48, 20), (0, 2, 1), (4, 3, 2)),
                                                ((64, 64, 64), (0, 1, 2), (4, 4, 4)), ((9, 512), (1,), (4,))])
 def test_nd_float32_short_lines(gpu, oracle, shape, axes, levels):
@@ -243,7 +272,8 @@ def test_nd_float32_short_lines(gpu, oracle, shape, axes, levels):
             continue
         ref = oracle.lwt(x, w, tf, bc, axes=list(axes), level=lvl)
         got = la.lwt(_gpu(x, gpu), w, tf, mode=bc, axes=list(axes), level=lvl).cpu().numpy()
-        _check(got, ref, np.float32, f"shape={shape} axes={axes} w={w} bc={bc} tf={tf} sep={sep}")
+        _check(got, ref, np.float32, f"shape={shape} axes={axes} w={w} bc={bc} tf={tf} sep={sep}", axes=list(axes),
+               level=lvl)
 
 
 def test_nd_float32_whole_tile_transposes(gpu, oracle):
@@ -309,10 +339,10 @@ def test_level_by_level_inverse_copies_in_to_out(gpu, oracle):
             if dt in (torch.float32, torch.float64):
                 _check(got.cpu().numpy(), ref, np.float64 if dt == torch.float64 else np.float32, what)
             else:
-                # 16-bit storage is rounded once per level: compare within the storage type's precision
-                g = got.float().cpu().numpy()
-                tol = 8 * float(torch.finfo(dt).eps) * float(np.abs(ref).max())
-                assert np.abs(g - ref).max() <= tol, what
+                # 16-bit storage is rounded once per level: the oracle one level
+                # at a time, rounded to the storage type after each level
+                ref16 = oracle_level_by_level_rounded(oracle, xs, 100907, tf, 0, axes, 2, str(dt).split(".")[1])
+                _check(got.float().cpu().numpy(), ref16, np.float32, what, axes=axes, level=2)
 
 
 def test_layouts_float32_batched_lines(gpu, oracle):
@@ -359,3 +389,41 @@ def test_layouts_float32_batched_lines(gpu, oracle):
         hf = h.to(gpu).permute(2, 1, 0).contiguous().permute(2, 1, 0)
         got16 = la.lwt(hf, 100907, tf, axes=[1], level=[3]).float().cpu()
         assert torch.equal(got16, ref16), f"bf16 F-order tf={tf}"
+
+
+def test_two_threads_two_streams_share_the_workspace(gpu, oracle):
+    """Two host threads transform float64 arrays of different sizes on two
+    streams at once, repeatedly. Long lines take the tiled passes, whose band
+    scratch lives in the library's one workspace per device (the second size
+    needs a larger one, so it is grown while the other thread may be using
+    it). Every result equals the oracle bit for bit."""
+    import concurrent.futures
+    import torch
+    import lifted_amd as la
+    cases = [((3, 70001), 100907, 3, [6]), ((2, 150001), 3, 1, [9])]
+    data = []
+    for k, (shape, w, bc, lvl) in enumerate(cases):
+        x = rng(300 + k).standard_normal(shape)
+        data.append((x, w, bc, lvl, oracle.lwt(x, w, F, bc, axes=[1], level=lvl),
+                     oracle.lwt(x, w, FA, bc, axes=[1], level=lvl)))
+
+    def worker(k):
+        x, w, bc, lvl, ref_f, ref_a = data[k]
+        s = torch.cuda.Stream()
+        outs = []
+        with torch.cuda.stream(s):
+            xd = torch.from_numpy(x).to(gpu, non_blocking=False)
+            for _ in range(6):
+                yf = la.lwt(xd, w, F, mode=bc, axes=[1], level=lvl)
+                ya = la.lwt(xd, w, FA, mode=bc, axes=[1], level=lvl)
+                outs.append((yf, ya))
+        s.synchronize()
+        return [(yf.cpu().numpy(), ya.cpu().numpy()) for yf, ya in outs]
+
+    with concurrent.futures.ThreadPoolExecutor(2) as ex:
+        results = list(ex.map(worker, [0, 1]))
+    for k, res in enumerate(results):
+        _, _, _, lvl, ref_f, ref_a = data[k]
+        for i, (gf, ga) in enumerate(res):
+            _check(gf, ref_f, np.float64, f"thread {k} call {i} forward", axes=[1], level=lvl)
+            _check(ga, ref_a, np.float64, f"thread {k} call {i} adjoint", axes=[1], level=lvl)

@@ -112,6 +112,105 @@ def test_db2_interior_filter_taps(oracle):
     np.testing.assert_allclose(got, sorted(np.abs(taps)), atol=1e-15)
 
 
+def _interior_rows(oracle, w, n=64, i=16):
+    """Rows of the level-1 forward matrix for s[i] and d[i] (Zero boundary,
+    interior i), centred on input samples 2i and 2i + 1."""
+    W = np.stack([oracle.lwt(np.eye(n)[j], w, F, oracle.ZERO, level=[1]) for j in range(n)], axis=1)
+    return W[i], W[n // 2 + i]
+
+
+def test_cdf97_rows_are_the_published_analysis_filters(oracle):
+    """CDF 9/7 level 1 equals the biorthogonal 9/7 analysis pair of
+    Cohen-Daubechies-Feauveau as tabulated for JPEG 2000 (low-pass 9 taps,
+    high-pass 7 taps, low-pass DC gain 1), with the reference's scale step
+    normalising them to sqrt(2) h and g / sqrt(2): s[i] = sqrt(2) sum_k h_k
+    x[2i + k], d[i] = sum_k g_k x[2i + 1 + k] / sqrt(2)."""
+    h = [0.02674875741080976, -0.01686411844287495, -0.07822326652898785, 0.2668641184428723,
+         0.6029490182363579, 0.2668641184428723, -0.07822326652898785, -0.01686411844287495, 0.02674875741080976]
+    g = [0.09127176311424948, -0.05754352622849957, -0.5912717631142470, 1.115087052456994,
+         -0.5912717631142470, -0.05754352622849957, 0.09127176311424948]
+    srow, drow = _interior_rows(oracle, 100907)
+    i = 16
+    exp_s = np.zeros(64)
+    exp_s[2 * i - 4:2 * i + 5] = np.sqrt(2) * np.array(h)
+    exp_d = np.zeros(64)
+    exp_d[2 * i + 1 - 3:2 * i + 1 + 4] = np.array(g) / np.sqrt(2)
+    np.testing.assert_allclose(srow, exp_s, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(drow, exp_d, rtol=0, atol=1e-12)
+    # the pair's defining moments: DC passes the low-pass (gain sqrt 2), the
+    # high-pass kills constants and linear ramps (two vanishing moments ... 4)
+    k = np.arange(-3, 4)
+    assert abs(exp_s.sum() - np.sqrt(2)) < 1e-12
+    for p in range(4):
+        assert abs(float(np.sum(np.array(g) * k ** p))) < 1e-12, p
+
+
+def test_cdf53_rows_are_the_legall_pair(oracle):
+    """CDF 5/3 level 1: the LeGall 5/3 pair in the reference's arrangement,
+    which updates s from d first (src/lifted-wavelets.hpp:1078-1090), so the
+    low-pass is the 3-tap sqrt(2) [1/4, 1/2, 1/4] and the high-pass the 5-tap
+    sqrt(2) [-1/8, -1/4, 3/4, -1/4, -1/8] -- exact rationals times sqrt 2."""
+    srow, drow = _interior_rows(oracle, 100503)
+    i = 16
+    exp_s = np.zeros(64)
+    exp_s[2 * i - 1:2 * i + 2] = np.sqrt(2) * np.array([0.25, 0.5, 0.25])
+    exp_d = np.zeros(64)
+    exp_d[2 * i + 1 - 2:2 * i + 1 + 3] = np.sqrt(2) * np.array([-0.125, -0.25, 0.75, -0.25, -0.125])
+    np.testing.assert_allclose(srow, exp_s, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(drow, exp_d, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("bc", [oracle_bc for oracle_bc in (1, 2, 3, 4)])
+def test_constant_input_has_zero_details(oracle, bc):
+    """Every table wavelet with at least one vanishing moment maps a constant
+    line to exactly zero detail bands under the boundaries that extend a
+    constant as a constant (Constant, Periodic, Symmetric, Reflect), at every
+    level, for even and odd lengths; the approximation band is the constant
+    times sqrt(2)^L (orthonormal scaling). Zero is excluded: it truncates the
+    constant at the ends. Three table entries of the reference do not
+    annihilate constants, and the oracle keeps them verbatim: Daubechies8 (its
+    coefficients are not an orthogonal filter, pinned by
+    test_reference_daubechies8_table_is_not_orthogonal), Bior3_1 (two
+    s-updates instead of an s- and a d-update, src/lifted-wavelets.hpp:869-870)
+    and Bior5_5; for them the level-1 detail band of a constant is checked to
+    be a nonzero constant."""
+    from lifted_amd import layout
+    quirky = (8, 10301, 10505)
+    for w in quirky:
+        r = oracle.lwt(np.full(96, 2.5), w, F, bc, level=[1])
+        d = r[layout.line_bands(96, 1)[("d", 1)]]
+        assert np.abs(d).min() > 0.1 and np.ptp(d) < 1e-12, (w, "constant, nonzero details")
+    for w in [w for w in WAVELETS if w != 0 and w not in quirky]:
+        for n in (96, 101):
+            L = min(3, max(1, oracle.max_level(w, n)))
+            r = oracle.lwt(np.full(n, 2.5), w, F, bc, level=[L])
+            bands = layout.line_bands(n, L)
+            for l in range(1, L + 1):
+                d = r[bands[("d", l)]]
+                assert np.abs(d).max() <= 1e-12, (w, bc, n, l, np.abs(d).max())
+            s = r[bands["s"]]
+            assert np.abs(np.abs(s) - 2.5 * np.sqrt(2) ** L).max() <= 1e-11 * 2 ** L, (w, bc, n)
+
+
+def test_batch_entry_equals_per_signal_lwt(oracle):
+    """oracle.batch (the checker of the batched GPU workload) equals oracle.lwt
+    run signal by signal, bit for bit: packed signals of uneven length, every
+    wavelet, levels up to the maximum, all transforms, Zero and Symmetric
+    boundaries, float32 and float64, one and several threads."""
+    g = np.random.default_rng(77)
+    sigs, off = [], 0
+    for w in [w for w in WAVELETS if w != 0]:
+        n = int(g.integers(40, 3000))
+        sigs.append((off, n, w, int(g.integers(1, max(1, oracle.max_level(w, n)) + 1))))
+        off += n
+    for dt, tf, bc, th in itertools.product((np.float32, np.float64), (F, I, FA, IA), (0, 3), (1, 4)):
+        x = g.standard_normal(off).astype(dt)
+        got = oracle.batch(x, sigs, tf, bc, n_threads=th)
+        for o, n, w, L in sigs:
+            ref = oracle.lwt(x[o:o + n], w, tf, bc, level=[L])
+            np.testing.assert_array_equal(got[o:o + n], ref, err_msg=f"{dt} tf={tf} bc={bc} w={w}")
+
+
 # ---------------- brute force on tiny n ----------------
 
 def _mat(oracle, w, tf, bc, n, level=1):
