@@ -61,6 +61,21 @@ typedef struct {
  * lifting_transform / seperable_lifting_transform (src/lifted-drivers-inl.hpp:630-656). */
 int lfx_lwt(const lfx_desc* desc, const void* in_dev, void* out_dev, void* stream);
 
+/* lfx_lwt without any device allocation. No counterpart in the reference: it
+ * is for callers that cannot allocate (code under hipGraph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Same descriptor, arrays and
+ * results as lfx_lwt, restricted to table wavelets (and, for 16-bit storage,
+ * to one transformed axis); it runs the LDS line and
+ * tile kernels with their band scratch in a fixed device array of at most
+ * 64 MiB (the library holds one per GPU until the process ends; the step
+ * programs of every table wavelet sit in a fixed table beside it). A call
+ * whose scratch would exceed that -- long lines (tiled passes), or an
+ * in-place call of them -- fails with LFX_EINVAL. Nothing in the scratch
+ * needs to start at zero. Every call on one GPU shares that GPU's scratch:
+ * calls that overlap on one GPU (two streams, or two host threads) race and
+ * must be serialized by the caller. */
+int lfx_lwt_noalloc(const lfx_desc* desc, const void* in_dev, void* out_dev, void* stream);
+
 /* Same on host arrays: copies to the current device, transforms, copies back,
  * blocking -- the calling convention of lifted::lwt. */
 int lfx_lwt_host(const lfx_desc* desc, const void* in_host, void* out_host);

@@ -25,6 +25,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 
 SOURCES = [
     ("lfx_kernels.hip", True),
+    ("lfx_noalloc.hip", True),
     ("lfx_reg_tf0.hip", True),
     ("lfx_reg_tf1.hip", True),
     ("lfx_reg_tf2.hip", True),

@@ -33,6 +33,7 @@
 #include "lfx_error.hpp"
 #include "lfx_table.hpp"
 #include "lfx_schedule.hpp"
+#include "lfx_noalloc.hpp"
 
 namespace lfx {
 
@@ -310,6 +311,17 @@ static Tuning env_tuning() {
 }
 static thread_local Tuning g_tuning = env_tuning();
 
+// Fixed memory of a no-allocation call (lfx_lwt_noalloc): its scratch and
+// program tables. Such a call runs the LDS line and tile kernels only (the
+// batched-plan paths need allocations) and fails when its scratch would not
+// fit.
+struct RunCtx {
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  const void* progs_f32 = nullptr;
+  const void* progs_f64 = nullptr;
+};
+
 static bool lines_fast(const Plan& p, const long long* shp, int ax, long long L, const void* src, const long long* sst,
                        void* dst, const long long* dstr, hipStream_t st);
 static bool sep_chain(const Plan& p, const long long* shape, const long long* sin, const long long* sout,
@@ -319,13 +331,13 @@ static bool sep_chain(const Plan& p, const long long* shape, const long long* si
 template <class C>
 static void transform_lines_t(const Plan& p, const WaveletDefL& w, const void* prog, bool fwd, const long long* shp,
                               int ax, long long L, const void* src, const long long* sst, void* dst,
-                              const long long* dstr, hipStream_t st) {
+                              const long long* dstr, hipStream_t st, const RunCtx* ctx) {
   const long long n = shp[ax];
   long long nlines = 1;
   for (int k = 0; k < p.ndim; ++k)
     if (k != ax) nlines *= shp[k];
   if (nlines == 0 || n == 0 || L == 0) return;
-  if (lines_fast(p, shp, ax, L, src, sst, dst, dstr, st)) return;
+  if (!ctx && lines_fast(p, shp, ax, L, src, sst, dst, dstr, st)) return;
   const DProgram<C> hp = resolve<C>(w, p.tf);
   // LFX_FLAG_PLAIN: one tiled pass per level (one launch per level, the band
   // round-trips through memory between levels); Periodic keeps whole lines
@@ -352,8 +364,17 @@ static void transform_lines_t(const Plan& p, const WaveletDefL& w, const void* p
   const size_t scr_bytes = (size_t)(nlines * mmax) * cs;
   const size_t scr_pad = (scr_bytes + 255) / 256 * 256;
   WsLease lease;
-  lease_workspace(copy_pad + 2 * scr_pad, st, lease);
-  char* ws = (char*)lease.p;
+  char* ws = nullptr;
+  if (ctx) {
+    if (copy_pad + 2 * scr_pad > ctx->ws_bytes)
+      fail(LFX_EINVAL, "this transform needs " + std::to_string(copy_pad + 2 * scr_pad) +
+                           " bytes of scratch; the no-allocation entry point holds at most " +
+                           std::to_string(ctx->ws_bytes));
+    ws = (char*)ctx->ws;
+  } else {
+    lease_workspace(copy_pad + 2 * scr_pad, st, lease);
+    ws = (char*)lease.p;
+  }
   void* scr[2] = {ws + copy_pad, ws + copy_pad + scr_pad};
   const void* in = src;
   std::vector<long long> in_str(sst, sst + p.ndim);
@@ -442,12 +463,25 @@ static void transform_lines_t(const Plan& p, const WaveletDefL& w, const void* p
 // One 1-D transform of L levels (L == 1 for a single level) of all lines of `shp` along `ax`.
 static void transform_lines(const Plan& p, const WaveletDefL& w, const void* prog, bool fwd, const long long* shp,
                             int ax, long long L, const void* src, const long long* sst, void* dst,
-                            const long long* dstr, hipStream_t st) {
-  if (compute_bytes(p.storage) == 8) transform_lines_t<double>(p, w, prog, fwd, shp, ax, L, src, sst, dst, dstr, st);
-  else transform_lines_t<float>(p, w, prog, fwd, shp, ax, L, src, sst, dst, dstr, st);
+                            const long long* dstr, hipStream_t st, const RunCtx* ctx) {
+  if (compute_bytes(p.storage) == 8)
+    transform_lines_t<double>(p, w, prog, fwd, shp, ax, L, src, sst, dst, dstr, st, ctx);
+  else
+    transform_lines_t<float>(p, w, prog, fwd, shp, ax, L, src, sst, dst, dstr, st, ctx);
 }
 
-static void run_plan(const Plan& p, const void* in, void* out, hipStream_t st) {
+// Device program of a no-allocation call: its slot in the fixed tables.
+static const void* fixed_program(const RunCtx& ctx, uint32_t wavelet, int tf, int storage) {
+  const int i = wavelet < kCustomBase ? table_index(wavelet) : -1;
+  if (i < 0)
+    fail(LFX_EUNSUPPORTED, "the no-allocation entry point takes table wavelets only (a caller-defined program "
+                           "needs a device allocation)");
+  if (storage == kF64) return (const DProgram<double>*)ctx.progs_f64 + 4 * i + tf;
+  return (const DProgram<float>*)ctx.progs_f32 + 4 * i + tf;
+}
+
+// ctx: null for lfx_lwt / lfx_lwt_host; the fixed memory of a no-allocation call.
+static void run_plan(const Plan& p, const void* in, void* out, hipStream_t st, const RunCtx* ctx = nullptr) {
   const WaveletDefL w = find_wavelet(p.wavelet);
   const bool steps = has_lift_steps(w);
   const bool fwd_stepping = p.tf == kForward || p.tf == kInverseAdjoint;
@@ -458,7 +492,7 @@ static void run_plan(const Plan& p, const void* in, void* out, hipStream_t st) {
       if (p.shape[k] > 1 && p.sin[k] != p.sout[k])
         fail(LFX_EINVAL, "in-place transforms need identical input and output strides");
   }
-  const void* prog = device_program(p.wavelet, p.tf, p.storage);
+  const void* prog = ctx ? fixed_program(*ctx, p.wavelet, p.tf, p.storage) : device_program(p.wavelet, p.tf, p.storage);
 
   if (p.separable) {
     // resolve levels and validate before launching anything
@@ -473,7 +507,7 @@ static void run_plan(const Plan& p, const void* in, void* out, hipStream_t st) {
       check_bands(p.bc, steps, len, L[ia]);
     }
     if (total == 0) return;
-    if (sep_chain(p, p.shape, p.sin, p.sout, p.axes, L, in, out, st)) return;
+    if (!ctx && sep_chain(p, p.shape, p.sin, p.sout, p.axes, L, in, out, st)) return;
     for (size_t ia = 0; ia < p.axes.size(); ++ia) {
       const void* src = ia == 0 ? in : out;
       const long long* sst = ia == 0 ? p.sin : p.sout;
@@ -482,7 +516,7 @@ static void run_plan(const Plan& p, const void* in, void* out, hipStream_t st) {
         if (ia == 0 && in != out) copy_box(p.storage, p.ndim, p.shape, in, p.sin, out, p.sout, st);
         continue;
       }
-      transform_lines(p, w, prog, fwd_stepping, p.shape, p.axes[ia], L[ia], src, sst, out, p.sout, st);
+      transform_lines(p, w, prog, fwd_stepping, p.shape, p.axes[ia], L[ia], src, sst, out, p.sout, st, ctx);
     }
     return;
   }
@@ -519,13 +553,13 @@ static void run_plan(const Plan& p, const void* in, void* out, hipStream_t st) {
     // one level on every axis of the box, chained through workspaces (the
     // source is read in full by the first axis before the last axis writes
     // `out`, so this is safe in place)
-    if (sep_chain(p, shp.data(), sst, p.sout, p.axes, std::vector<long long>(p.axes.size(), 1), src, out, st)) {
+    if (!ctx && sep_chain(p, shp.data(), sst, p.sout, p.axes, std::vector<long long>(p.axes.size(), 1), src, out, st)) {
       src = out;
       sst = p.sout;
       continue;
     }
     for (int a : p.axes) {
-      transform_lines(p, w, prog, fwd_stepping, shp.data(), a, 1, src, sst, out, p.sout, st);
+      transform_lines(p, w, prog, fwd_stepping, shp.data(), a, 1, src, sst, out, p.sout, st, ctx);
       src = out;
       sst = p.sout;
     }
@@ -1426,6 +1460,18 @@ int lfx_lwt(const lfx_desc* desc, const void* in_dev, void* out_dev, void* strea
   return guarded([&] {
     Plan p = make_plan(desc);
     run_plan(p, in_dev, out_dev, (hipStream_t)stream);
+  });
+}
+
+int lfx_lwt_noalloc(const lfx_desc* desc, const void* in_dev, void* out_dev, void* stream) {
+  return guarded([&] {
+    Plan p = make_plan(desc);
+    if ((p.storage == kBF16 || p.storage == kF16) && p.axes.size() > 1)
+      fail(LFX_EUNSUPPORTED, "16-bit storage over several axes runs through float32 workspaces, which the "
+                             "no-allocation entry point does not hold");
+    const NoAllocDevice& d = noalloc_device();
+    const RunCtx ctx{d.scratch, kNoAllocScratchBytes, d.progs_f32, d.progs_f64};
+    run_plan(p, in_dev, out_dev, (hipStream_t)stream, &ctx);
   });
 }
 

@@ -77,6 +77,26 @@ WaveletDefL find_wavelet(uint32_t id) {
   fail(LFX_EINVAL, "unknown wavelet id " + std::to_string(id));
 }
 
+int table_index(uint32_t id) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  const auto& t = table();
+  for (size_t i = 0; i < t.size(); ++i)
+    if (t[i].id == id) return (int)i;
+  return -1;
+}
+
+int table_size() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return (int)table().size();
+}
+
+WaveletDefL table_entry(int i) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  const auto& t = table();
+  if (i < 0 || i >= (int)t.size()) fail(LFX_EINVAL, "wavelet table index out of range");
+  return t[i];
+}
+
 uint32_t register_custom(WaveletDefL w) {
   std::lock_guard<std::mutex> lk(g_mu);
   w.id = kCustomBase + (uint32_t)g_custom.size();

@@ -16,6 +16,11 @@ constexpr uint32_t kCustomBase = 0x80000000u;
 // program (ids from kCustomBase); throws Error for unknown ids.
 WaveletDefL find_wavelet(uint32_t id);
 uint32_t register_custom(WaveletDefL w);
+// Position of a table wavelet in the table (-1: not a table wavelet), the
+// table's size, and its i-th entry.
+int table_index(uint32_t id);
+int table_size();
+WaveletDefL table_entry(int i);
 
 // Resolve a wavelet for one transform type (see lfx_program.hpp).
 template <class C>
