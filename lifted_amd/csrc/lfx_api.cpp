@@ -346,6 +346,33 @@ static void transform_lines_t(const Plan& p, const WaveletDefL& w, const void* p
   const LineSchedule sc = plain ? schedule(hp, fwd, n, L, p.storage, p.bc, tile, 1, 1)
                                 : schedule(hp, fwd, n, L, p.storage, p.bc, tile, g_tuning.k);
   const bool alias = src == dst;
+  if (sc.global_line) {
+    // one workgroup per line with its bands in global scratch (in place is
+    // safe: a line is read in full before any of it is written), lines in
+    // chunks whose scratch fits the budget
+    const size_t per_line = (size_t)gline_elems(n) * sizeof(C);
+    const size_t budget = ctx ? ctx->ws_bytes : ((size_t)256 << 20);
+    const long long lpl = std::min<long long>(nlines, std::max<long long>(1, (long long)(budget / per_line)));
+    const size_t need = (size_t)lpl * per_line;
+    WsLease lease;
+    void* g = nullptr;
+    if (ctx) {
+      if (need > ctx->ws_bytes)
+        fail(LFX_EINVAL, "this transform needs " + std::to_string(need) +
+                             " bytes of scratch; the no-allocation entry point holds at most " +
+                             std::to_string(ctx->ws_bytes));
+      g = ctx->ws;
+    } else {
+      lease_workspace(need, st, lease);
+      g = lease.p;
+    }
+    LineLaunch ll{};
+    ll.storage = p.storage; ll.fwd = fwd; ll.prog = prog; ll.bc = p.bc; ll.n = n; ll.L = (int)L;
+    ll.a0 = src; ll.a1 = dst; ll.a2 = src;
+    ll.lm = line_map3(p.ndim, shp, ax, sst, sst[ax], dstr, dstr[ax], sst, sst[ax], ll.nlines);
+    hip_check(launch_gline(ll, g, lpl, st), "global line kernel");
+    return;
+  }
   if (sc.passes.empty()) {
     LineLaunch ll{};
     ll.storage = p.storage; ll.fwd = fwd; ll.prog = prog; ll.bc = p.bc; ll.n = n; ll.L = (int)L;
@@ -673,6 +700,9 @@ static int lwt_overload(uint32_t wavelet, uint32_t transform, uint32_t bc, size_
 // (forward: fine -> coarse; reverse: coarse -> fine); a stage is one launch
 // whose items are tiles or whole lines of any signal, longest first.
 
+// LDS class marker of global-line items (they need no dynamic LDS)
+static constexpr size_t kGlobalLineClass = ~(size_t)0;
+
 struct BatchLaunch {
   long long item0 = 0, nitems = 0;
   size_t lds = 0;
@@ -713,6 +743,7 @@ struct Group {
   long long m0;  // band length entering the group (fine side)
   int k;         // levels
   long long T, H, ntiles;
+  bool global = false;  // whole-line group with its bands in the plan's scratch
 };
 
 template <class C>
@@ -735,11 +766,11 @@ static std::vector<Group> signal_groups(const WaveletDefL& w, int tf, int storag
     if (plain) {
       long long m = sc.m_line;
       for (long long l = 0; l < sc.L_line; ++l) {
-        g.push_back(Group{false, m, 1, 0, 0, 1});
+        g.push_back(Group{false, m, 1, 0, 0, 1, sc.global_line && m > line_limit(storage)});
         m = band_len(m, 1);
       }
     } else {
-      g.push_back(Group{false, sc.m_line, (int)sc.L_line, 0, 0, 1});
+      g.push_back(Group{false, sc.m_line, (int)sc.L_line, 0, 0, 1, sc.global_line});
     }
   }
   return g;  // fine -> coarse
@@ -875,7 +906,7 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
       item_bytes[stage].push_back(b);
       item_lds[stage].push_back(l);
       cost[stage].push_back(c);
-      lds[stage] = std::max(lds[stage], l);
+      if (l != kGlobalLineClass) lds[stage] = std::max(lds[stage], l);
       bytes[stage] += b;
     };
     auto arr_bytes = [&](int arr) { return arr == kArrScr ? (double)cs : (double)es; };
@@ -921,7 +952,12 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
           it.arr[0] = in_arr; it.off[0] = in_off;
           it.arr[1] = kArrOut; it.off[1] = sg.offset;
           it.arr[2] = in_arr; it.off[2] = in_off;
-          add_item(g + (Gmax - G), it, (double)gr.m0, line_lds_bytes(gr.m0, cs),
+          if (gr.global) {
+            it.kind = kItemGLine;
+            it.a = scr_elems;
+            scr_elems += gline_elems(gr.m0);
+          }
+          add_item(g + (Gmax - G), it, (double)gr.m0, gr.global ? kGlobalLineClass : line_lds_bytes(gr.m0, cs),
                    gr.m0 * (arr_bytes(in_arr) + (double)es), sg.wavelet);
         }
         in_arr = s_arr;
@@ -963,8 +999,13 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
           it.arr[0] = kArrIn; it.off[0] = sg.offset;
           it.arr[1] = o_arr; it.off[1] = o_off;
           it.arr[2] = s_arr; it.off[2] = s_off;
-          add_item(gi, it, (double)gr.m0, line_lds_bytes(gr.m0, cs), gr.m0 * ((double)es + arr_bytes(o_arr)),
-                   sg.wavelet);
+          if (gr.global) {
+            it.kind = kItemGLine;
+            it.a = scr_elems;
+            scr_elems += gline_elems(gr.m0);
+          }
+          add_item(gi, it, (double)gr.m0, gr.global ? kGlobalLineClass : line_lds_bytes(gr.m0, cs),
+                   gr.m0 * ((double)es + arr_bytes(o_arr)), sg.wavelet);
         }
         s_arr = o_arr;
         s_off = o_off;
@@ -975,6 +1016,7 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
     // replicate: user-array offsets shift by rep_stride, scratch by scr_elems
     auto shift = [&](BatchItem it, long long r) {
       for (int a = 0; a < 3; ++a) it.off[a] += r * (it.arr[a] == kArrScr ? scr_elems : rep_stride);
+      if (it.kind == kItemGLine) it.a += r * scr_elems;
       return it;
     };
     for (size_t st = 0; st < per_stage.size(); ++st) {
@@ -1038,22 +1080,24 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
   for (size_t st = 0; st < per_stage.size(); ++st) {
     BatchStage bs;
     bs.alg_bytes = bytes[st];
-    for (int c = 3; c >= 0; --c) {
+    // class 4: global-line items (bands in scratch, no dynamic LDS, 1024 threads)
+    for (int c = 4; c >= 0; --c) {
       std::vector<size_t> idx;
       for (size_t j = 0; j < per_stage[st].size(); ++j) {
         const size_t l = item_lds[st][j];
-        const int cls = l <= kClasses[0] ? 0 : (l <= kClasses[1] ? 1 : (l <= kClasses[2] ? 2 : 3));
+        const int cls = l == kGlobalLineClass ? 4
+                        : (l <= kClasses[0] ? 0 : (l <= kClasses[1] ? 1 : (l <= kClasses[2] ? 2 : 3)));
         if (cls == c) idx.push_back(j);
       }
       if (idx.empty()) continue;
       std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return cost[st][a] > cost[st][b]; });
       BatchLaunch bl;
-      bl.threads = c == 0 ? 64 : 0;
+      bl.threads = c == 0 ? 64 : (c == 4 ? kThreads : 0);
       bl.item0 = (long long)all.size();
       bl.nitems = (long long)idx.size();
       for (size_t j : idx) {
         all.push_back(per_stage[st][j]);
-        bl.lds = std::max(bl.lds, item_lds[st][j]);
+        if (c != 4) bl.lds = std::max(bl.lds, item_lds[st][j]);
         bl.bytes += item_bytes[st][j];
       }
       bs.nitems += bl.nitems;

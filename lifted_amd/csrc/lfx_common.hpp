@@ -70,6 +70,9 @@ struct PermMap {
 
 // LDS bytes of the whole-line kernels for a line of n samples
 inline size_t line_lds_bytes(long long n, size_t csize) { return (size_t)(n + (n - n / 2)) * csize; }
+// Elements of global scratch per line of the global whole-line kernels (the
+// same two buffers, rounded up to 64 elements)
+inline long long gline_elems(long long n) { return (n + (n - n / 2) + 63) / 64 * 64; }
 
 // Whole-line launch: arrays 0 (input), 1 (output), 2 (reverse: s_L source).
 struct LineLaunch {
@@ -124,7 +127,10 @@ struct TileLaunch {
 // three logical arrays (as in LineLaunch / TileLaunch) lives in one of the
 // user input, the user output or the plan's scratch, at an element offset.
 enum BatchArr : int32_t { kArrIn = 0, kArrOut = 1, kArrScr = 2 };
-enum BatchKind : int32_t { kItemLine = 0, kItemTile = 1, kItemReg = 2 };
+// kItemGLine: a whole line whose bands live in the plan's scratch (global
+// memory) instead of LDS, at element offset BatchItem::a (Periodic lines
+// longer than the LDS limit).
+enum BatchKind : int32_t { kItemLine = 0, kItemTile = 1, kItemReg = 2, kItemGLine = 3 };
 struct BatchItem {
   const void* prog;  // device program
   long long m0;      // band length

@@ -79,6 +79,34 @@ __global__ void __launch_bounds__(kThreads) k_line_rev(const DProgram<C>* prog, 
                                lm.ax[1], P, P + cap0);
 }
 
+// Whole lines whose two band buffers live in global scratch (gstride
+// elements per workgroup) instead of LDS: the same bodies, one workgroup per
+// line, its threads synchronized by workgroup barriers (which also order the
+// workgroup's global-memory accesses). For lines longer than the LDS limit
+// under the Periodic boundary, whose wrap reads the far end of every band.
+template <class C, class S>
+__global__ void __launch_bounds__(kThreads) k_gline_fwd(const DProgram<C>* prog, int bc, long long n, int L,
+                                                        const S* src, S* dst, LineMap3 lm, C* gscr,
+                                                        long long gstride, long long line0) {
+  long long off[3];
+  line_bases(lm, line0 + blockIdx.x, off);
+  C* P = gscr + (long long)blockIdx.x * gstride;
+  prog = stage_program<C>(prog);
+  line_fwd_body<C, S, S>(prog, bc, n, L, src + off[0], lm.ax[0], dst + off[1], lm.ax[1], P, P + n);
+}
+
+template <class C, class S>
+__global__ void __launch_bounds__(kThreads) k_gline_rev(const DProgram<C>* prog, int bc, long long n, int L,
+                                                        const S* src, S* dst, LineMap3 lm, C* gscr,
+                                                        long long gstride, long long line0) {
+  long long off[3];
+  line_bases(lm, line0 + blockIdx.x, off);
+  C* P = gscr + (long long)blockIdx.x * gstride;
+  prog = stage_program<C>(prog);
+  line_rev_body<C, S, S, S>(prog, bc, n, L, src + off[0], lm.ax[0], src + off[2], lm.ax[2], dst + off[1], lm.ax[1],
+                            P, P + n);
+}
+
 // forward tiles: array 0 = input band, 1 = output (d regions), 2 = s_k out
 template <class C, class SI, class SO, class SK>
 __global__ void __launch_bounds__(kThreads) k_tile_fwd(const DProgram<C>* prog, int bc, TileArgs ta, const SI* src,
@@ -467,6 +495,29 @@ static hipError_t tile_rev_t(const TileLaunch& tl, hipStream_t st) {
   });
 }
 
+template <int S>
+static hipError_t gline_t(const LineLaunch& ll, void* gscr, long long lines_per_launch, hipStream_t st) {
+  using C = typename StoreT<S>::C;
+  using T = typename StoreT<S>::T;
+  const long long gstride = gline_elems(ll.n);
+  for (long long b0 = 0; b0 < ll.nlines; b0 += lines_per_launch) {
+    const unsigned cnt = (unsigned)std::min(lines_per_launch, ll.nlines - b0);
+    if (ll.fwd)
+      hipLaunchKernelGGL((k_gline_fwd<C, T>), dim3(cnt), dim3(kThreads), 0, st,
+                         reinterpret_cast<const DProgram<C>*>(ll.prog), ll.bc, ll.n, ll.L,
+                         reinterpret_cast<const T*>(ll.a0), reinterpret_cast<T*>(ll.a1), ll.lm,
+                         reinterpret_cast<C*>(gscr), gstride, b0);
+    else
+      hipLaunchKernelGGL((k_gline_rev<C, T>), dim3(cnt), dim3(kThreads), 0, st,
+                         reinterpret_cast<const DProgram<C>*>(ll.prog), ll.bc, ll.n, ll.L,
+                         reinterpret_cast<const T*>(ll.a0), reinterpret_cast<T*>(ll.a1), ll.lm,
+                         reinterpret_cast<C*>(gscr), gstride, b0);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 #define LFX_DISPATCH_STORAGE(S, CALL) \
   switch (S) {                        \
     case kF32: return CALL(kF32);     \
@@ -495,6 +546,14 @@ hipError_t launch_line(const LineLaunch& ll, hipStream_t st) {
               : hipErrorInvalidValue
   LFX_DISPATCH_STORAGE(ll.storage, CALL_R)
 #undef CALL_R
+}
+
+hipError_t launch_gline(const LineLaunch& ll, void* gscr, long long lines_per_launch, hipStream_t st) {
+  if (ll.nlines <= 0) return hipSuccess;
+  if (lines_per_launch <= 0 || ll.scr[0] || ll.scr[1] || ll.scr[2]) return hipErrorInvalidValue;
+#define CALL_G(S) gline_t<S>(ll, gscr, lines_per_launch, st)
+  LFX_DISPATCH_STORAGE(ll.storage, CALL_G)
+#undef CALL_G
 }
 
 hipError_t launch_tile(const TileLaunch& tl, hipStream_t st) {
@@ -540,6 +599,20 @@ __device__ __forceinline__ void batch_item(const BatchItem& it, int bc, const Ar
   auto user_in = [&](int i) -> const S* { return (it.arr[i] == kArrIn ? a.in : a.out) + it.off[i]; };
   auto user_out = [&](int i) -> S* { return a.out + it.off[i]; };
   auto scr = [&](int i) -> C* { return a.scr + it.off[i]; };
+  if (it.kind == kItemGLine) {
+    // bands in the plan's scratch (a separate branch, so that the LDS items
+    // keep LDS addressing)
+    C* G = a.scr + it.a;
+    C* GQ = G + it.m0;
+    if (FWD) {
+      if (s0) line_fwd_body<C, C, S>(prog, bc, it.m0, it.k, scr(0), 1, user_out(1), 1, G, GQ);
+      else line_fwd_body<C, S, S>(prog, bc, it.m0, it.k, user_in(0), 1, user_out(1), 1, G, GQ);
+    } else {
+      if (s1) line_rev_body<C, S, S, C>(prog, bc, it.m0, it.k, user_in(0), 1, user_in(2), 1, scr(1), 1, G, GQ);
+      else line_rev_body<C, S, S, S>(prog, bc, it.m0, it.k, user_in(0), 1, user_in(2), 1, user_out(1), 1, G, GQ);
+    }
+    return;
+  }
   if (it.kind == kItemLine) {
     C* Q = P + it.m0;
     if (FWD) {

@@ -8,6 +8,10 @@ namespace lfx {
 
 hipError_t launch_line(const LineLaunch& ll, hipStream_t st);
 hipError_t launch_tile(const TileLaunch& tl, hipStream_t st);
+// Whole lines with their bands in global scratch (gscr: gline_elems(n)
+// compute-type elements per line for lines_per_launch lines; launches run in
+// turn over chunks of lines_per_launch lines). User arrays only (no scr flags).
+hipError_t launch_gline(const LineLaunch& ll, void* gscr, long long lines_per_launch, hipStream_t st);
 // One batched stage: `nitems` items (sorted by decreasing cost), arrays
 // `in` / `out` in storage type, `scr` in compute type.
 // threads: workgroup size (0: the default of block_threads())

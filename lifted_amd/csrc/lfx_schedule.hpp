@@ -32,6 +32,10 @@ struct LineSchedule {
   std::vector<PassPlan> passes;
   long long m_line;  // band length of the whole-line part
   long long L_line;  // its levels (0: none)
+  // the whole-line part keeps its bands in global scratch instead of LDS
+  // (Periodic lines longer than the LDS limit: the wrap reads the far end of
+  // every band, which no tile holds)
+  bool global_line = false;
 };
 
 inline std::vector<long long> probe_tiles(long long ntiles) {
@@ -83,9 +87,10 @@ LineSchedule schedule(const DProgram<C>& hp, bool fwd, long long n, long long L,
   LineSchedule s{{}, n, L};
   const long long limit = limit_override > 0 ? std::min(limit_override, line_limit(storage)) : line_limit(storage);
   if (n <= limit || L == 0) return s;
-  if (bc == kPeriodic)
-    fail(LFX_EUNSUPPORTED, "Periodic boundary on lines longer than " + std::to_string(limit) +
-                               " samples is not supported (its wrap reads the far end of the band)");
+  if (bc == kPeriodic) {
+    s.global_line = n > line_limit(storage);
+    return s;
+  }
   long long m = n, lv = 0;
   const long long T0 = tile_hint > 0 ? tile_hint : default_tile(storage);
   while (lv < L && m > limit) {

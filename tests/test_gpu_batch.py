@@ -209,8 +209,8 @@ def test_batch_16bit_storage(gpu, oracle, storage):
 
 def test_batch_custom_programs_periodic_and_max_level(gpu, oracle):
     """Caller-defined step programs in a batch, the Periodic boundary on lines
-    that fit one workgroup (longer ones are refused), and levels = 0 (max
-    level), against the oracle."""
+    that fit one workgroup and on longer ones, and levels = 0 (max level),
+    against the oracle."""
     import torch
     import lifted_amd as la
     from helpers import MOCK_WAVELETS, oracle_steps
@@ -232,5 +232,20 @@ def test_batch_custom_programs_periodic_and_max_level(gpu, oracle):
             else:
                 ref = oracle.lwt(x[o:o + n], w, tf, bc, level=[L if L else oracle.max_level(w, n)])
             assert_parity(y[o:o + n], ref, f"bc={bc} tf={tf} n={n} w={w} L={L}")
-    with pytest.raises(la.LfxError):
-        BatchPlan([(0, 300000, 100907, 3)], 0, mode=2, dtype=torch.float32)
+
+    # Periodic signals longer than the LDS limit (bands in the plan's global
+    # scratch), power-of-two and other lengths, with short ones beside them,
+    # float32 and float64, fused and plain
+    g2 = np.random.default_rng(19)
+    lens = [300000, 65536, 1000, (1 << 18) + 3, 40000]
+    sigs2, off = [], 0
+    for i, n in enumerate(lens):
+        sigs2.append((off, n, [100907, 2, 10204, 3, 10608][i], [3, 8, 4, 12, 0][i]))
+        off += n
+    for dt, tf, plain in itertools.product((np.float32, np.float64), (0, 1, 2, 3), (False, True)):
+        x2 = g2.standard_normal(off).astype(dt)
+        tdt = torch.float32 if dt == np.float32 else torch.float64
+        y2 = BatchPlan(sigs2, tf, mode=2, dtype=tdt, plain=plain).execute(torch.from_numpy(x2).to(gpu)).cpu().numpy()
+        for (o, n, w, L) in sigs2:
+            ref = oracle.lwt(x2[o:o + n], w, tf, 2, level=[L if L else oracle.max_level(w, n)])
+            assert_parity(y2[o:o + n], ref, f"Periodic {dt.__name__} tf={tf} plain={plain} n={n} w={w} L={L}")

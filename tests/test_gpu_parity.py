@@ -175,12 +175,13 @@ def test_half_storage_single_axis_matches_fp32_rounded(gpu, oracle, storage):
 def test_config1_db2_65536_fp64_8_levels(gpu, oracle):
     """1-D float64 signal of 65536 N(0,1) samples, Daubechies2, 8 levels,
     forward then inverse (the reference's CPU-runnable workload): coefficients
-    within 1e-12 of the oracle and perfect reconstruction, for every BC the
-    tiled path takes and the level-by-level path."""
+    equal to the oracle and perfect reconstruction, for all five boundaries
+    (Periodic runs the whole line with its bands in global scratch), separable
+    and level-by-level."""
     import lifted_amd as la
     x = np.random.default_rng(20261201).standard_normal(65536)
     t = _gpu(x, gpu)
-    for bc, sep in itertools.product((0, 1, 3, 4), (True, False)):
+    for bc, sep in itertools.product((0, 1, 2, 3, 4), (True, False)):
         lvl = [8] if sep else 8
         ref = oracle.lwt(x, 2, F, bc, level=lvl)
         w = la.lwt(t, 2, F, mode=bc, level=lvl)
@@ -202,6 +203,34 @@ def test_long_lines_tiled(gpu, oracle, tf):
         ref = oracle.lwt(x, w, tf, bc, level=[L])
         got = la.lwt(_gpu(x, gpu), w, tf, mode=bc, level=[L]).cpu().numpy()
         _check(got, ref, dtype, f"w={w} bc={bc} n={n} L={L} tf={tf}", axes=[0], level=[L])
+
+
+@pytest.mark.parametrize("tf", [F, I, FA, IA])
+def test_long_lines_periodic(gpu, oracle, tf):
+    """Periodic lines longer than one workgroup's LDS: one workgroup per line
+    with its bands in global scratch, so the wrap can read the far end of
+    every band (with the reference's unsigned modulo for negative indices at
+    lengths that are not powers of two). Power-of-two and odd lengths, up to
+    2^20 + 7 samples, float32 and float64, several lines along a strided axis,
+    level-by-level, and in place; all against the oracle."""
+    import torch
+    import lifted_amd as la
+    g = rng(200 + tf)
+    for (w, n, L, dtype) in [(100907, (1 << 20) + 7, 6, np.float32), (2, 65536, 14, np.float64),
+                             (10608, 40001, 5, np.float32), (3, 1 << 17, 10, np.float32),
+                             (100503, 30011, 7, np.float64)]:
+        x = g.standard_normal(n).astype(dtype)
+        ref = oracle.lwt(x, w, tf, 2, level=[L])
+        got = la.lwt(_gpu(x, gpu), w, tf, mode=2, level=[L]).cpu().numpy()
+        _check(got, ref, dtype, f"w={w} n={n} L={L} tf={tf}", axes=[0], level=[L])
+    x = g.standard_normal((30000, 5)).astype(np.float32)
+    for lvl in ([4], 3):
+        ref = oracle.lwt(x, 10204, tf, 2, axes=[0], level=lvl)
+        t = torch.from_numpy(x).to(gpu)
+        _check(la.lwt(t, 10204, tf, mode=2, axes=[0], level=lvl).cpu().numpy(), ref, np.float32,
+               f"strided tf={tf} level={lvl}", axes=[0], level=lvl)
+        la.lwt(t, 10204, tf, mode=2, axes=[0], level=lvl, out=t)
+        _check(t.cpu().numpy(), ref, np.float32, f"in place tf={tf} level={lvl}", axes=[0], level=lvl)
 
 
 def test_long_lines_2d_strided_axis(gpu, oracle):
