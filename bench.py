@@ -4,13 +4,16 @@ Workload (the headline configuration): 4096 1-D float32 signals, lengths
 round(2^U(10, 20)), each with a wavelet drawn uniformly from the 33 non-Lazy
 predefined wavelets and a level count drawn uniformly from [3, max_level];
 values iid N(0, 1); Zero boundary; total capped at 2^30 samples. One step =
-forward transform of every signal, then the forward-adjoint of the result
-(two batched plans, three stages each). Inputs are resident in HBM before the
-timed region.
+forward transform of every signal, then the forward-adjoint of the result:
+two batched plans run as one chain pipelined over two signal chunks (each
+plan's big stage, the finest levels, runs chunk after chunk while the other
+chunk's coarse-level stages run beside it). Inputs are resident in HBM before
+the timed region.
 
 Reports Gsamples/s (2 x samples per step / time); the HBM roofline of the
-dominant kernel, timed with HIP events recorded around each of its launches on
-the stream it runs on, inside the timed region; its HBM traffic from two
+dominant kernel (its big-stage launches), timed with HIP events recorded around
+each launch on the stream it runs on, over eager steps of the same chain; its
+HBM traffic from two
 rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) scaled by the same counters
 read on a device copy of known size; the plain per-level kernels (one launch
 per level) for comparison; and the CPU restatement of the reference timed on
@@ -137,36 +140,69 @@ def time_plans(plans, x, bufs, steps, warmup, per_launch=False):
     return total_ms / steps, stage_ms, launches
 
 
-def dominant_kernel(plans, launches):
-    """Kernel (symbol) with the largest time per step: its launches per step,
-    algorithmic bytes per launch and average launch duration."""
+def time_chain(chain, x, bufs, steps, warmup):
+    """Eager chain executions with HIP events around every launch (on the
+    stream each launch goes to). Returns (ms per step, per-launch records
+    averaged over the steps)."""
+    import torch
+    stream = torch.cuda.current_stream()
+    infos = chain.launch_info()
+
+    def new_ev():
+        e = torch.cuda.Event(enable_timing=True)
+        e.record(stream)  # creates the underlying event
+        return e
+
+    for _ in range(warmup):
+        chain.execute(x, bufs)
+    torch.cuda.synchronize()
+    lev = [[(new_ev(), new_ev()) for _ in infos] for _ in range(steps)]
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0.record(stream)
+    for k in range(steps):
+        chain.execute_timed(x, bufs, lev[k])
+    t1.record(stream)
+    torch.cuda.synchronize()
+    launches = []
+    for i, info in enumerate(infos):
+        ms = sum(lev[k][i][0].elapsed_time(lev[k][i][1]) for k in range(steps)) / steps
+        launches.append(dict(info, ms=ms))
+    return t0.elapsed_time(t1) / steps, launches
+
+
+def chain_dominant(chain, launches):
+    """The kernel (transform, launch kind) with the most time per step, over
+    its big-stage launches (the finest levels of every signal: most of the
+    kernel's bytes; its coarse-level launches run beside the other chunk's big
+    stage and are listed separately in `launches`)."""
     by = {}
     for r in launches:
-        key = (int(plans[r["plan"]].transform), r["kind"])
-        d = by.setdefault(key, {"launches": 0, "bytes": 0.0, "ms": 0.0, "items": 0})
-        d["launches"] += 1
-        d["bytes"] += r["alg_bytes"]
+        key = (chain.transforms[r["transform_index"]], r["kind"])
+        d = by.setdefault(key, {"ms": 0.0})
         d["ms"] += r["ms"]
-        d["items"] += r["items"]
     key = max(by, key=lambda k: by[k]["ms"])
-    d = by[key]
-    return {"symbol": KIND_NAMES.get(key, str(key)), "launches_per_step": d["launches"],
-            "alg_bytes_per_launch": d["bytes"] / d["launches"], "avg_launch_ms": d["ms"] / d["launches"],
-            "ms_per_step": d["ms"]}
+    big = [r for r in launches if (chain.transforms[r["transform_index"]], r["kind"]) == key and r["big"]]
+    if not big:
+        big = [r for r in launches if (chain.transforms[r["transform_index"]], r["kind"]) == key]
+    n = len(big)
+    return {"symbol": KIND_NAMES.get(key, str(key)), "launches_per_step": n,
+            "alg_bytes_per_launch": sum(r["alg_bytes"] for r in big) / n,
+            "avg_launch_ms": sum(r["ms"] for r in big) / n, "items": [r["items"] for r in big],
+            "ms_per_step_all_launches": by[key]["ms"]}
 
 
-def time_graph_step(plans, x, bufs, steps, warmup):
-    """ms per step of the plans captured once in a hipGraph and replayed."""
+def time_graph(run, steps, warmup):
+    """ms per step of `run` (enqueues one step on the current stream)
+    captured once in a hipGraph and replayed."""
     import torch
     s = torch.cuda.Stream()
     s.wait_stream(torch.cuda.current_stream())
     g = torch.cuda.CUDAGraph()
     with torch.cuda.stream(s):
         with torch.cuda.graph(g, stream=s):
-            src = x
-            for p, b in zip(plans, bufs):
-                p.execute(src, b)
-                src = b
+            run()
     torch.cuda.current_stream().wait_stream(s)
     for _ in range(warmup):
         g.replay()
@@ -211,9 +247,10 @@ def measure_copies(x, reps=5):
     return out
 
 
-def _pmc_pass(counter, child_args, tmp):
+def _pmc_pass(counter, child_args, tmp, grids=None):
     """One rocprofv3 counter pass over a child run of this script; returns
-    {kernel symbol: (dispatches, summed counter value)}."""
+    {kernel symbol: (dispatches, summed counter value)}, counting only
+    dispatches whose grid size (work-items) is in `grids` when given."""
     out = os.path.join(tmp, counter + "_" + child_args[-1].lstrip("-"))
     cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "run", "--",
            sys.executable, "-u", os.path.abspath(__file__)] + child_args
@@ -231,17 +268,21 @@ def _pmc_pass(counter, child_args, tmp):
     agg = {}
     with open(path) as fh:
         for row in csv.DictReader(fh):
+            if grids is not None and int(float(row.get("Grid_Size", 0))) not in grids:
+                continue
             k = row["Kernel_Name"]
             n, v = agg.get(k, (0, 0.0))
             agg[k] = (n + 1, v + float(row["Counter_Value"]))
     return agg
 
 
-def measure_traffic(total, symbol, tuning):
-    """HBM bytes per launch of `symbol` for one step of the workload (plans
-    built with the planning knobs `tuning`, as in the timed steps), from
-    FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by the ratio of
-    known bytes to the counter on a device copy of the input size."""
+def measure_traffic(total, symbol, tuning, grids=None):
+    """HBM bytes per launch of `symbol` for one step of the workload (the
+    chain built with the planning knobs `tuning`, as in the timed steps; only
+    the dispatches whose grid size is in `grids` -- the big-stage launches --
+    when given), from FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each
+    scaled by the ratio of known bytes to the counter on a device copy of the
+    input size."""
     if shutil.which("rocprofv3") is None:
         return None, "rocprofv3 not found"
     tmp = tempfile.mkdtemp(dir="/tmp", prefix="lfx_pmc_")
@@ -252,7 +293,7 @@ def measure_traffic(total, symbol, tuning):
             ck = [k for k in copy if "copyBuffer" in k or "copy" in k.lower()]
             n, v = copy[ck[0]]
             factor = (total * 4 / 1024.0) / (v / n)
-            work = _pmc_pass(counter, ["--pmc-child", "--tuning-json", json.dumps(tuning)], tmp)
+            work = _pmc_pass(counter, ["--pmc-child", "--tuning-json", json.dumps(tuning)], tmp, grids)
             wk = [k for k in work if k.startswith(symbol)]
             if not wk:
                 raise RuntimeError(f"{symbol} not in the {counter} pass")
@@ -335,22 +376,21 @@ def cpu_baseline(sigs, total, x_host, reps=5):
 
 def pmc_child(tuning):
     """One step of the workload, untimed (run under rocprofv3 counter passes),
-    with the plans built from the planning knobs the timed steps used."""
+    with the chain built from the planning knobs the timed steps used."""
     import torch
     import lifted_amd as la
-    from lifted_amd.batch import BatchPlan
+    from lifted_amd.batch import BatchChain
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     sigs, total = make_workload()
     gen = torch.Generator(device=dev)
     gen.manual_seed(SEED)
     x = torch.randn(total, generator=gen, device=dev, dtype=torch.float32)
-    y = torch.empty_like(x)
-    z = torch.empty_like(x)
-    fwd = BatchPlan(sigs, la.Transform.Forward, dtype=torch.float32, tuning=tuning)
-    adj = BatchPlan(sigs, la.Transform.ForwardAdjoint, dtype=torch.float32, tuning=tuning)
-    fwd.execute(x, y)
-    adj.execute(y, z)
+    bufs = [torch.empty_like(x), torch.empty_like(x)]
+    knobs = {k: v for k, v in tuning.items() if k != "chunks"}
+    chain = BatchChain(sigs, (la.Transform.Forward, la.Transform.ForwardAdjoint), dtype=torch.float32,
+                       chunks=tuning.get("chunks", 2), tuning=knobs)
+    chain.execute(x, bufs)
     torch.cuda.synchronize()
     print("pmc child done", flush=True)
 
@@ -572,46 +612,61 @@ def main():
     x = torch.randn(total, generator=gen, device=dev, dtype=torch.float32)
     y = torch.empty_like(x)
     z = torch.empty_like(x)
-    # planning knobs chosen by the autotuner on this data (defaults first)
+    # planning knobs chosen by the autotuner on this data (defaults first):
+    # the forward and forward-adjoint plans as one pipelined chain, with the
+    # chunk count searched too
+    from lifted_amd.batch import BatchChain
     from lifted_amd.tune import tune_batch
+    TF = (la.Transform.Forward, la.Transform.ForwardAdjoint)
     t_tune = time.perf_counter()
-    best, tried = tune_batch(sigs, (la.Transform.Forward, la.Transform.ForwardAdjoint), x, reps=5, warmup=2)
+    best, tried = tune_batch(sigs, TF, x, reps=5, warmup=2, chain=True)
     t_tune = time.perf_counter() - t_tune
     print(f"[bench] autotune {t_tune:.1f}s: " + ", ".join(f"{k}={ms:.3f}ms" for k, ms in tried), file=sys.stderr,
           flush=True)
+    knobs = {k: v for k, v in best.items() if k != "chunks"}
     t_plan = time.perf_counter()
-    fwd = BatchPlan(sigs, la.Transform.Forward, dtype=torch.float32, tuning=best)
-    adj = BatchPlan(sigs, la.Transform.ForwardAdjoint, dtype=torch.float32, tuning=best)
+    chain = BatchChain(sigs, TF, dtype=torch.float32, chunks=best.get("chunks", 2), tuning=knobs)
     t_plan = time.perf_counter() - t_plan
-    plans = [fwd, adj]
-    print(f"[bench] {len(sigs)} signals, {total} samples, plans in {t_plan:.1f}s, stages "
-          f"{fwd.num_stages}+{adj.num_stages}", file=sys.stderr, flush=True)
-    # eager steps with HIP events around every launch (per-kernel roofline)
-    eager_ms, stage_ms, launches = time_plans(plans, x, [y, z], args.steps, args.warmup, per_launch=True)
-    dom = dominant_kernel(plans, launches)
+    bufs = [y, z]
+    print(f"[bench] {len(sigs)} signals, {total} samples, chain of {len(TF)} plans over {chain.num_chunks} "
+          f"chunks built in {t_plan:.1f}s", file=sys.stderr, flush=True)
+    # eager chain steps with HIP events around every launch (per-kernel roofline)
+    eager_ms, launches = time_chain(chain, x, bufs, args.steps, args.warmup)
+    dom = chain_dominant(chain, launches)
     achieved = dom["alg_bytes_per_launch"] / (dom["avg_launch_ms"] * 1e-3)
     print(f"[bench] eager: {eager_ms:.3f} ms/step, {2 * total / (eager_ms * 1e-3) / 1e9:.2f} Gsamples/s; dominant "
-          f"{dom['symbol']} {dom['avg_launch_ms']:.3f} ms/launch, {achieved / 1e9:.0f} GB/s", file=sys.stderr,
+          f"{dom['symbol']} big stages {dom['avg_launch_ms']:.3f} ms/launch, {achieved / 1e9:.0f} GB/s", file=sys.stderr,
           flush=True)
     eager_out = z.clone()
-    # the timed steps: the same two plans captured once as a hipGraph and
-    # replayed K times (the way a serving loop runs them); its output must
-    # equal the eager steps' bit for bit, else the eager steps are reported
+    # the timed steps: the chain captured once as a hipGraph and replayed K
+    # times (the way a serving loop runs it); its output must equal the eager
+    # steps' bit for bit, else the eager steps are reported
     z.zero_()
-    step_mode = ("forward + forward-adjoint plans captured once in a hipGraph, replayed K times "
-                 "(output checked equal to the eager steps)")
+    step_mode = (f"forward + forward-adjoint plans as one chain pipelined over {chain.num_chunks} signal chunks, "
+                 "captured once in a hipGraph, replayed K times (output checked equal to the eager steps)")
     try:
-        ms = time_graph_step(plans, x, [y, z], args.steps, args.warmup)
+        ms = time_graph(lambda: chain.execute(x, bufs), args.steps, args.warmup)
         if not torch.equal(z, eager_out):
             raise RuntimeError("graph replay output differs from the eager steps")
         print(f"[bench] graph replay: {ms:.3f} ms/step, {2 * total / (ms * 1e-3) / 1e9:.2f} Gsamples/s "
               "(output equal to eager)", file=sys.stderr, flush=True)
     except RuntimeError as e:
         ms = eager_ms
-        step_mode = f"eager steps (hipGraph replay not used: {e})"
+        step_mode = f"eager chain steps (hipGraph replay not used: {e})"
         print(f"[bench] {step_mode}", file=sys.stderr, flush=True)
     del eager_out
     value = 2 * total / (ms * 1e-3) / 1e9
+    # the same two plans run one after the other (no pipelining): stage times
+    fwd = BatchPlan(sigs, la.Transform.Forward, dtype=torch.float32, tuning=knobs)
+    adj = BatchPlan(sigs, la.Transform.ForwardAdjoint, dtype=torch.float32, tuning=knobs)
+    plans = [fwd, adj]
+    seq_ms, stage_ms, _ = time_plans(plans, x, bufs, args.steps, args.warmup)
+    try:
+        seq_graph_ms = time_graph(lambda: (fwd.execute(x, y), adj.execute(y, z)), args.steps, args.warmup)
+    except RuntimeError:
+        seq_graph_ms = None
+    print(f"[bench] plans in turn (no pipelining): {seq_ms:.3f} ms/step eager, {seq_graph_ms} graph",
+          file=sys.stderr, flush=True)
     # copies of the input size: the achievable streaming rate on this device
     # (torch's copy, and the library's reference copies: one float4 per thread,
     # one 2048-float window per wavefront like the register tiles)
@@ -629,41 +684,57 @@ def main():
         "copy_GBps": copy_gbs,
         "step_mode": step_mode,
         "eager": {"ms_per_step": round(eager_ms, 4), "Gsamples_per_s": round(2 * total / (eager_ms * 1e-3) / 1e9, 3),
-                  "note": "the same steps launched eagerly with HIP events around every launch (roofline source)"},
+                  "note": "the same chain launched eagerly with HIP events around every launch (roofline source)"},
+        "unpipelined": {"eager_ms_per_step": round(seq_ms, 4),
+                        "graph_ms_per_step": None if seq_graph_ms is None else round(seq_graph_ms, 4),
+                        "note": "the forward plan, then the forward-adjoint plan, each over all signals"},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
-                     "kernel": dom["symbol"].rstrip("("), "launches_per_step": dom["launches_per_step"],
+                     "kernel": dom["symbol"].rstrip("(") + " (big-stage launches)",
+                     "launches_per_step": dom["launches_per_step"],
                      "avg_launch_ms": round(dom["avg_launch_ms"], 4),
-                     "alg_bytes_per_launch": round(dom["alg_bytes_per_launch"])},
-        "stages": [{"plan": "fwd" if p == 0 else "adj", "stage": s, "ms": round(t, 4),
-                    **plans[p].stage_info(s)} for p, s, t in stage_ms],
-        "launches": [{"plan": "fwd" if r["plan"] == 0 else "adj", "stage": r["stage"], "kind": r["kind"],
-                      "threads": r["threads"], "items": r["items"], "alg_bytes": round(r["alg_bytes"]),
-                      "ms": round(r["ms"], 4)} for r in launches],
+                     "alg_bytes_per_launch": round(dom["alg_bytes_per_launch"]),
+                     "items_per_launch": dom["items"]},
+        "stages_unpipelined": [{"plan": "fwd" if p == 0 else "adj", "stage": s, "ms": round(t, 4),
+                                **plans[p].stage_info(s)} for p, s, t in stage_ms],
+        "launches": [{"transform": int(chain.transforms[r["transform_index"]]), "chunk": r["chunk"],
+                      "stage": r["stage"], "big": r["big"], "kind": r["kind"], "threads": r["threads"],
+                      "items": r["items"], "alg_bytes": round(r["alg_bytes"]), "ms": round(r["ms"], 4)}
+                     for r in launches],
     }
     if not args.no_plain:
         # plain = one launch per level: the same register-tile kernels run one
         # level per pass (the band round-trips through HBM between levels),
-        # short bands one level per LDS launch; timed like the fused step
-        # (hipGraph replay) and eagerly. The LDS-only plain plans (the generic
-        # step interpreter, one level per launch) are a secondary figure.
+        # short bands one level per LDS launch; as plans in turn and as the same
+        # pipelined chain, both graph-replayed like the fused step (the faster
+        # one is the comparison). The LDS-only plain plans (the generic step
+        # interpreter, one level per launch) are a secondary figure.
         pf = BatchPlan(sigs, la.Transform.Forward, dtype=torch.float32, plain=True)
         pa = BatchPlan(sigs, la.Transform.ForwardAdjoint, dtype=torch.float32, plain=True)
         pms_eager, _, _ = time_plans([pf, pa], x, [y, z], args.plain_steps, 1)
         try:
-            pms = time_graph_step([pf, pa], x, [y, z], args.plain_steps, 1)
+            pms_plans = time_graph(lambda: (pf.execute(x, y), pa.execute(y, z)), args.plain_steps, 1)
         except RuntimeError:
-            pms = pms_eager
-        res["plain_ms_per_step"] = round(pms, 4)
-        res["plain_eager_ms_per_step"] = round(pms_eager, 4)
-        res["vs_plain"] = round(pms / ms, 3)
+            pms_plans = pms_eager
         res["plain_stages"] = pf.num_stages + pa.num_stages
-        res["plain_note"] = ("one launch per level with the same register-tile kernels (levels_per_pass = 1), "
-                             "short bands one level per LDS launch; hipGraph replay like the fused step")
-        print(f"[bench] plain (register tiles, one level per launch): {pms:.3f} ms/step graph, {pms_eager:.3f} "
-              f"eager ({pf.num_stages}+{pa.num_stages} stages)", file=sys.stderr, flush=True)
         pf.close()
         pa.close()
+        pc = BatchChain(sigs, TF, dtype=torch.float32, chunks=chain.num_chunks, plain=True)
+        try:
+            pms_chain = time_graph(lambda: pc.execute(x, bufs), args.plain_steps, 1)
+        except RuntimeError:
+            pms_chain = float("inf")
+        pc.close()
+        pms = min(pms_plans, pms_chain)
+        res["plain_ms_per_step"] = round(pms, 4)
+        res["plain_detail"] = {"plans_graph_ms": round(pms_plans, 4), "plans_eager_ms": round(pms_eager, 4),
+                               "chain_graph_ms": round(pms_chain, 4)}
+        res["vs_plain"] = round(pms / ms, 3)
+        res["plain_note"] = ("one launch per level with the same register-tile kernels (levels_per_pass = 1), "
+                             "short bands one level per LDS launch; the faster of plans in turn and the pipelined "
+                             "chain, both hipGraph-replayed like the fused step")
+        print(f"[bench] plain (register tiles, one level per launch): plans {pms_plans:.3f} ms, chain "
+              f"{pms_chain:.3f} ms (graph)", file=sys.stderr, flush=True)
         qf = BatchPlan(sigs, la.Transform.Forward, dtype=torch.float32, plain=True, lds_only=True)
         qa = BatchPlan(sigs, la.Transform.ForwardAdjoint, dtype=torch.float32, plain=True, lds_only=True)
         qms, _, _ = time_plans([qf, qa], x, [y, z], args.plain_steps, 1)
@@ -672,6 +743,7 @@ def main():
         print(f"[bench] plain LDS-only: {qms:.3f} ms/step eager", file=sys.stderr, flush=True)
         qf.close()
         qa.close()
+    chain.close()
     fwd.close()
     adj.close()
     if not args.no_cpu:
@@ -680,7 +752,7 @@ def main():
     if not args.no_traffic:
         del x, y, z
         torch.cuda.empty_cache()
-        traffic, note = measure_traffic(total, dom["symbol"], best)
+        traffic, note = measure_traffic(total, dom["symbol"], best, grids={64 * i for i in dom["items"]})
         res["roofline"]["traffic"] = None if traffic is None else round(traffic)
         res["traffic_detail"] = note
         print(f"[bench] traffic: {traffic} ({note})", file=sys.stderr, flush=True)

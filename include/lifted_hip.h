@@ -175,6 +175,42 @@ int lfx_batch_launch_info(lfx_batch_plan plan, int32_t stage, int32_t max_launch
 int lfx_batch_execute_stage_timed(lfx_batch_plan plan, int32_t stage, const void* in_dev, void* out_dev, void* stream,
                                   int32_t nevents, void* const* ev_start, void* const* ev_end);
 
+/* ---- chains: a sequence of batched transforms pipelined over signal chunks ----
+ * A chain runs transforms[0..n) in sequence on one signal list (in -> outs[0]
+ * -> outs[1] ...), each as a batched plan, with the signals split into
+ * nchunks groups of about equal length. Each plan's "big" stage (the finest
+ * levels of every signal: stage 0 of a forward-stepping plan, the last stage
+ * of a reverse one) runs one chunk after another, and each chunk's short,
+ * latency-bound stages run beside the next chunk's big stage on a stream the
+ * chain owns, forked from and joined back to `stream` (graph capture of
+ * `stream` captures them too). Results equal running the plans in turn. One
+ * chain must not be executed from two host threads at once. No counterpart
+ * in the reference. */
+typedef struct lfx_batch_chain_s* lfx_batch_chain;
+int lfx_batch_chain_create(const lfx_signal* signals, int32_t nsignals, const uint32_t* transforms,
+                           int32_t ntransforms, uint32_t bc, int32_t storage, uint32_t flags, int32_t nchunks,
+                           lfx_batch_chain* chain);
+/* outs[i]: output of transform i (and input of transform i + 1); a step's
+ * input and output must not alias. */
+int lfx_batch_chain_execute(lfx_batch_chain chain, const void* in_dev, void* const* out_devs, void* stream);
+int lfx_batch_chain_num_chunks(lfx_batch_chain chain);
+/* One kernel launch of a chain execution. */
+typedef struct {
+  int32_t chunk;           /* signal chunk */
+  int32_t transform_index; /* position in the chain's transform list */
+  int32_t stage;           /* stage of that chunk's plan */
+  int32_t big;             /* 1: the plan's big stage */
+  lfx_launch_info launch;
+} lfx_chain_launch_info;
+/* Launches of one execution in launch order (per transform, per chunk, per
+ * stage, the stage's launches); fills info[0..min(n, max)) and returns n. */
+int lfx_batch_chain_launch_info(lfx_batch_chain chain, int32_t max_launches, lfx_chain_launch_info* info);
+/* lfx_batch_chain_execute with caller-created hipEvent_t pairs recorded
+ * around each launch (lfx_batch_chain_launch_info order) on its stream. */
+int lfx_batch_chain_execute_timed(lfx_batch_chain chain, const void* in_dev, void* const* out_devs, void* stream,
+                                  int32_t nevents, void* const* ev_start, void* const* ev_end);
+void lfx_batch_chain_destroy(lfx_batch_chain chain);
+
 /* Tile length (samples at a pass's finest level, 0 = default) and levels per
  * tiled pass (0 = default) used by plans and transforms created afterwards on
  * this thread. */

@@ -148,3 +148,121 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
+
+
+class ChainLaunchInfo(ctypes.Structure):
+    """lfx_chain_launch_info: one kernel launch of a chain execution."""
+    _fields_ = [("chunk", ctypes.c_int32), ("transform_index", ctypes.c_int32), ("stage", ctypes.c_int32),
+                ("big", ctypes.c_int32), ("launch", LaunchInfo)]
+
+
+def _bind_chain():
+    L = _bind()
+    if not hasattr(L, "_chain_bound"):
+        vp = ctypes.c_void_p
+        L.lfx_batch_chain_create.argtypes = [ctypes.POINTER(LfxSignal), ctypes.c_int32, ctypes.POINTER(ctypes.c_uint32),
+                                             ctypes.c_int32, ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32,
+                                             ctypes.c_int32, ctypes.POINTER(vp)]
+        L.lfx_batch_chain_execute.argtypes = [vp, vp, ctypes.POINTER(vp), vp]
+        L.lfx_batch_chain_num_chunks.argtypes = [vp]
+        L.lfx_batch_chain_launch_info.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ChainLaunchInfo)]
+        L.lfx_batch_chain_execute_timed.argtypes = [vp, vp, ctypes.POINTER(vp), vp, ctypes.c_int32,
+                                                    ctypes.POINTER(vp), ctypes.POINTER(vp)]
+        L.lfx_batch_chain_destroy.argtypes = [vp]
+        L.lfx_batch_chain_destroy.restype = None
+        L._chain_bound = True
+    return L
+
+
+class BatchChain:
+    """A sequence of batched transforms of one signal list (x -> outs[0] ->
+    outs[1] ...), pipelined over `chunks` groups of signals: each plan's big
+    stage (the finest levels of every signal) runs one chunk after another,
+    and each chunk's short coarse-level stages run beside the next chunk's big
+    stage on a second stream. Results equal running BatchPlan per transform.
+
+    signals: (offset, length, wavelet, levels) tuples; transforms: Transform
+    values in order; tuning: planning knobs (lifted_amd.tune) for its plans.
+    """
+
+    def __init__(self, signals, transforms, mode=0, dtype=None, chunks=2, plain=False, lds_only=False, tuning=None):
+        import torch
+        if tuning:
+            from . import tune
+            with tune.tuning(**tuning):
+                self.__init__(signals, transforms, mode, dtype, chunks, plain, lds_only)
+            return
+        L = _bind_chain()
+        sigs = list(signals)
+        arr = (LfxSignal * max(len(sigs), 1))()
+        for i, (off, n, w, lv) in enumerate(sigs):
+            arr[i].offset, arr[i].length, arr[i].wavelet, arr[i].levels = int(off), int(n), _wavelet_id(w), int(lv)
+        self.transforms = [int(t) for t in transforms]
+        tf = (ctypes.c_uint32 * len(self.transforms))(*self.transforms)
+        self.dtype = dtype or torch.float32
+        storage = _storage_torch(self.dtype)
+        flags = (LFX_FLAG_PLAIN if plain else 0) | (_lib.LFX_FLAG_LDS_ONLY if lds_only else 0)
+        h = ctypes.c_void_p()
+        _lib.check(L.lfx_batch_chain_create(arr, len(sigs), tf, len(self.transforms), int(mode), storage, flags,
+                                            int(chunks), ctypes.byref(h)))
+        self._h = h
+        self.signals = sigs
+        self.total = max((o + n for o, n, _, _ in sigs), default=0)
+        self.num_chunks = L.lfx_batch_chain_num_chunks(h)
+
+    def _outs(self, x, outs):
+        import torch
+        if outs is None:
+            outs = [torch.empty_like(x) for _ in self.transforms]
+        if len(outs) != len(self.transforms):
+            raise ValueError("one output per transform is needed")
+        for a in [x] + list(outs):
+            if a.dtype != self.dtype or not a.is_cuda or not a.is_contiguous() or a.numel() < self.total:
+                raise ValueError("packed arrays must be contiguous device tensors of the chain's dtype and length")
+        return outs
+
+    def execute(self, x, outs=None):
+        """Runs every transform; returns the list of outputs (the last one is the result)."""
+        import torch
+        outs = self._outs(x, outs)
+        ptrs = (ctypes.c_void_p * len(outs))(*[o.data_ptr() for o in outs])
+        st = torch.cuda.current_stream(x.device).cuda_stream
+        _lib.check(_bind_chain().lfx_batch_chain_execute(self._h, x.data_ptr(), ptrs, st))
+        return outs
+
+    def launch_info(self):
+        """Launches of one execution in launch order: dicts with chunk,
+        transform_index, stage, big, kind, threads, items, alg_bytes."""
+        L = _bind_chain()
+        n = _lib.check(L.lfx_batch_chain_launch_info(self._h, 0, None))
+        arr = (ChainLaunchInfo * max(n, 1))()
+        _lib.check(L.lfx_batch_chain_launch_info(self._h, n, arr))
+        kinds = {0: "lds", 1: "register", 2: "rows"}
+        return [{"chunk": a.chunk, "transform_index": a.transform_index, "stage": a.stage, "big": bool(a.big),
+                 "kind": kinds[a.launch.kind], "threads": a.launch.threads, "items": a.launch.nitems,
+                 "alg_bytes": a.launch.alg_bytes} for a in arr[:n]]
+
+    def execute_timed(self, x, outs, events):
+        """execute, recording events[i] = (start, end) (torch.cuda.Event with
+        timing, recorded once beforehand so that they exist) around launch i
+        (launch_info order) on the stream it goes to."""
+        import torch
+        outs = self._outs(x, outs)
+        ptrs = (ctypes.c_void_p * len(outs))(*[o.data_ptr() for o in outs])
+        n = len(events)
+        starts = (ctypes.c_void_p * max(n, 1))(*[e[0].cuda_event for e in events])
+        ends = (ctypes.c_void_p * max(n, 1))(*[e[1].cuda_event for e in events])
+        st = torch.cuda.current_stream(x.device).cuda_stream
+        _lib.check(_bind_chain().lfx_batch_chain_execute_timed(self._h, x.data_ptr(), ptrs, st, n, starts, ends))
+        return outs
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _bind_chain().lfx_batch_chain_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

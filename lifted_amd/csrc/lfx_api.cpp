@@ -964,11 +964,16 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
         in_off = s_off;
       }
     } else {
-      // coarse -> fine: groups in reverse order
+      // coarse -> fine: groups in reverse order. A signal's whole-line group
+      // (its coarsest levels) runs in stage 0 with those of all signals (one
+      // latency-bound LDS launch), and its tiled groups are aligned on the
+      // plan's last stage, so that the finest group of every signal -- most
+      // of the plan's bytes -- shares one register launch.
       int s_arr = kArrIn;
       long long s_off = sg.offset;
       for (int gi = 0; gi < G; ++gi) {
         const int g = G - 1 - gi;
+        const int stage = gs[g].tile ? gi + (Gmax - G) : gi;
         const Group& gr = gs[g];
         const bool finest = g == 0;
         int o_arr;
@@ -990,7 +995,7 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
             it.arr[1] = o_arr; it.off[1] = o_off;
             it.arr[2] = s_arr; it.off[2] = s_off;
             const TileArgs ta{gr.m0, sp.b - sp.a, sp.H, 0, gr.k, 0};
-            add_item(gi, it, (double)tile_rev_window(ta), sp.kind == kSpecReg ? 0 : tile_lds_bytes(ta, false, cs),
+            add_item(stage, it, (double)tile_rev_window(ta), sp.kind == kSpecReg ? 0 : tile_lds_bytes(ta, false, cs),
                      (sp.b - sp.a) * ((double)es + arr_bytes(o_arr)), sg.wavelet);
           }
         } else {
@@ -1004,7 +1009,7 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
             it.a = scr_elems;
             scr_elems += gline_elems(gr.m0);
           }
-          add_item(gi, it, (double)gr.m0, gr.global ? kGlobalLineClass : line_lds_bytes(gr.m0, cs),
+          add_item(stage, it, (double)gr.m0, gr.global ? kGlobalLineClass : line_lds_bytes(gr.m0, cs),
                    gr.m0 * ((double)es + arr_bytes(o_arr)), sg.wavelet);
         }
         s_arr = o_arr;
@@ -1229,6 +1234,182 @@ static void batch_stage(BatchPlanImpl* p, int st, const void* in, void* out, hip
       hip_check(hipStreamWaitEvent(s, p->join[i], 0), "hipStreamWaitEvent");
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Chains: a sequence of transforms of one signal list (in -> out_0 -> out_1
+// ...), each a batched plan, pipelined over groups ("chunks") of signals.
+//
+// Each plan has one "big" stage that holds most of its bytes: stage 0 of a
+// forward plan (the finest levels of every signal) and the last stage of a
+// reverse plan; its other stages are short, latency-bound launches (coarse
+// levels, whole short bands). Chunk c runs its plans in order on lane c (lane
+// 0 is the caller's stream, the others are streams the chain owns); the big
+// stages of all (chunk, plan) pairs run one after another -- big(c, t) waits
+// for big(c - 1, t), big(0, t) for big(last, t - 1) -- so while one chunk's
+// big stage streams HBM, the other chunks' short stages run beside it instead
+// of leaving the GPU idle between stages. Every signal still sees its own
+// transforms in order, so results equal plan-by-plan execution.
+
+struct ChainImpl {
+  int dev = 0;
+  int nchunks = 0, ntf = 0;
+  std::vector<std::vector<BatchPlanImpl*>> plans;  // [chunk][transform]
+  std::vector<hipStream_t> lanes;                  // lanes 1.. (lane 0 = caller's stream)
+  std::vector<std::vector<hipEvent_t>> big_done;   // [chunk][transform]
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> done;                    // per lane 1..
+};
+
+static int big_stage(const BatchPlanImpl* p) { return p->fwd ? 0 : (int)p->stages.size() - 1; }
+
+static void free_chain(ChainImpl* c) {
+  for (hipStream_t q : c->lanes) (void)hipStreamSynchronize(q);
+  for (auto& v : c->plans)
+    for (BatchPlanImpl* p : v)
+      if (p) free_plan(p);
+  for (hipStream_t q : c->lanes) (void)hipStreamDestroy(q);
+  for (auto& v : c->big_done)
+    for (hipEvent_t e : v)
+      if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->done)
+    if (e) (void)hipEventDestroy(e);
+  if (c->fork) (void)hipEventDestroy(c->fork);
+  delete c;
+}
+
+static ChainImpl* chain_create(const lfx_signal* sigs, int nsig, const uint32_t* tfs, int ntf, int bc, int storage,
+                               uint32_t flags, int nchunks) {
+  if (ntf < 1 || ntf > 8 || !tfs) fail(LFX_EINVAL, "a chain has 1..8 transforms");
+  if (nchunks < 1 || nchunks > 8) fail(LFX_EINVAL, "a chain has 1..8 chunks");
+  if (nsig < 0 || (nsig > 0 && !sigs)) fail(LFX_EINVAL, "bad signal list");
+  auto c = std::unique_ptr<ChainImpl, void (*)(ChainImpl*)>(new ChainImpl, free_chain);
+  hip_check(hipGetDevice(&c->dev), "hipGetDevice");
+  // chunks: consecutive signals of about equal total length
+  long long total = 0;
+  for (int i = 0; i < nsig; ++i) total += std::max<long long>(sigs[i].length, 0);
+  std::vector<int> cut{0};
+  long long acc = 0;
+  for (int i = 0; i < nsig && (int)cut.size() < nchunks; ++i) {
+    acc += std::max<long long>(sigs[i].length, 0);
+    if (acc * nchunks >= total * (long long)cut.size() && i + 1 < nsig) cut.push_back(i + 1);
+  }
+  cut.push_back(nsig);
+  c->nchunks = (int)cut.size() - 1;
+  c->ntf = ntf;
+  // the chunks' plans run their short launches on the lanes unless the
+  // caller's tuning names side streams
+  const int saved_streams = g_tuning.streams;
+  if (g_tuning.streams < 0) g_tuning.streams = 0;
+  try {
+    for (int k = 0; k < c->nchunks; ++k) {
+      c->plans.emplace_back();
+      for (int t = 0; t < ntf; ++t)
+        c->plans.back().push_back(batch_create(sigs + cut[k], cut[k + 1] - cut[k], (int)tfs[t], bc, storage, flags));
+    }
+  } catch (...) {
+    g_tuning.streams = saved_streams;
+    throw;
+  }
+  g_tuning.streams = saved_streams;
+  hip_check(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming), "hipEventCreate");
+  for (int k = 0; k < c->nchunks; ++k) {
+    c->big_done.emplace_back();
+    for (int t = 0; t < ntf; ++t) {
+      hipEvent_t e = nullptr;
+      hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+      c->big_done.back().push_back(e);
+    }
+    if (k > 0) {
+      hipStream_t q = nullptr;
+      hipEvent_t e = nullptr;
+      hip_check(hipStreamCreateWithFlags(&q, hipStreamNonBlocking), "hipStreamCreate");
+      c->lanes.push_back(q);
+      hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+      c->done.push_back(e);
+    }
+  }
+  return c.release();
+}
+
+// ev (optional): one start/end event pair per kernel launch, in the order
+// chain_launch_info lists them.
+static void chain_execute(ChainImpl* c, const void* in, void* const* outs, hipStream_t s,
+                          const std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev) {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev != c->dev) fail(LFX_EINVAL, "chain was created on another device");
+  for (int t = 0; t < c->ntf; ++t) {
+    if (!outs[t]) fail(LFX_EINVAL, "null output array");
+    if (outs[t] == (t == 0 ? in : outs[t - 1])) fail(LFX_EINVAL, "a chain step's input and output must not alias");
+  }
+  auto lane = [&](int k) { return k == 0 ? s : c->lanes[k - 1]; };
+  if (c->nchunks > 1) {
+    hip_check(hipEventRecord(c->fork, s), "hipEventRecord");
+    for (hipStream_t q : c->lanes) hip_check(hipStreamWaitEvent(q, c->fork, 0), "hipStreamWaitEvent");
+  }
+  size_t ei = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> sub;
+  for (int t = 0; t < c->ntf; ++t) {
+    const void* src = t == 0 ? in : outs[t - 1];
+    for (int k = 0; k < c->nchunks; ++k) {
+      BatchPlanImpl* p = c->plans[k][t];
+      const hipStream_t q = lane(k);
+      const int big = big_stage(p);
+      for (int st = 0; st < (int)p->stages.size(); ++st) {
+        const bool is_big = st == big && c->nchunks > 1;
+        if (is_big) {
+          const hipEvent_t prev = k > 0 ? c->big_done[k - 1][t] : (t > 0 ? c->big_done[c->nchunks - 1][t - 1] : nullptr);
+          if (prev) hip_check(hipStreamWaitEvent(q, prev, 0), "hipStreamWaitEvent");
+        }
+        const std::vector<std::pair<hipEvent_t, hipEvent_t>>* sev = nullptr;
+        if (ev) {
+          const BatchStage& bs = p->stages[st];
+          const size_t n = bs.launches.size() + (bs.nreg > 0 ? 1 : 0) + (bs.nrows > 0 ? 1 : 0);
+          if (ei + n > ev->size()) fail(LFX_EINVAL, "one start and one end event per launch are needed");
+          sub.assign(ev->begin() + ei, ev->begin() + ei + n);
+          ei += n;
+          sev = &sub;
+        }
+        batch_stage(p, st, src, outs[t], q, sev);
+        if (is_big) hip_check(hipEventRecord(c->big_done[k][t], q), "hipEventRecord");
+      }
+    }
+  }
+  for (int k = 1; k < c->nchunks; ++k) {
+    hip_check(hipEventRecord(c->done[k - 1], c->lanes[k - 1]), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(s, c->done[k - 1], 0), "hipStreamWaitEvent");
+  }
+}
+
+// Launch records of a chain execution, in launch order: per transform, per
+// chunk, per stage, the stage's launches (lfx_batch_launch_info order).
+struct ChainLaunch {
+  int chunk, tf, stage;
+  lfx_launch_info info;
+};
+static std::vector<ChainLaunch> chain_launches(const ChainImpl* c) {
+  std::vector<ChainLaunch> out;
+  for (int t = 0; t < c->ntf; ++t)
+    for (int k = 0; k < c->nchunks; ++k) {
+      const BatchPlanImpl* p = c->plans[k][t];
+      for (int st = 0; st < (int)p->stages.size(); ++st) {
+        const BatchStage& bs = p->stages[st];
+        for (const BatchLaunch& bl : bs.launches) {
+          lfx_launch_info r{};
+          r.kind = 0;
+          r.threads = bl.threads > 0 ? bl.threads : block_threads();
+          r.nitems = bl.nitems;
+          r.alg_bytes = bl.bytes;
+          r.lds_bytes = (int64_t)bl.lds;
+          out.push_back(ChainLaunch{k, t, st, r});
+        }
+        if (bs.nreg > 0) out.push_back(ChainLaunch{k, t, st, lfx_launch_info{1, 64 * kRegWaves, bs.nreg, bs.reg_bytes, 0}});
+        if (bs.nrows > 0)
+          out.push_back(ChainLaunch{k, t, st, lfx_launch_info{2, 64 * kRegWaves, bs.nrows, bs.rows_bytes, 0}});
+      }
+    }
+  return out;
 }
 
 // ---------------------------------------------------------------------------
@@ -1706,6 +1887,68 @@ void lfx_batch_plan_destroy(lfx_batch_plan plan) {
   BatchPlanImpl* p = reinterpret_cast<BatchPlanImpl*>(plan);
   if (!p) return;
   free_plan(p);
+}
+
+int lfx_batch_chain_create(const lfx_signal* signals, int32_t nsignals, const uint32_t* transforms,
+                           int32_t ntransforms, uint32_t bc, int32_t storage, uint32_t flags, int32_t nchunks,
+                           lfx_batch_chain* chain) {
+  return guarded([&] {
+    if (!chain) fail(LFX_EINVAL, "null chain pointer");
+    *chain = reinterpret_cast<lfx_batch_chain>(
+        chain_create(signals, nsignals, transforms, ntransforms, (int)bc, storage, flags, nchunks));
+  });
+}
+
+int lfx_batch_chain_execute(lfx_batch_chain chain, const void* in_dev, void* const* out_devs, void* stream) {
+  return guarded([&] {
+    ChainImpl* c = reinterpret_cast<ChainImpl*>(chain);
+    if (!c || !out_devs) fail(LFX_EINVAL, "null chain or output list");
+    chain_execute(c, in_dev, out_devs, (hipStream_t)stream, nullptr);
+  });
+}
+
+int lfx_batch_chain_launch_info(lfx_batch_chain chain, int32_t max_launches, lfx_chain_launch_info* info) {
+  int count = 0;
+  const int rc = guarded([&] {
+    ChainImpl* c = reinterpret_cast<ChainImpl*>(chain);
+    if (!c) fail(LFX_EINVAL, "null chain");
+    const std::vector<ChainLaunch> v = chain_launches(c);
+    for (int i = 0; i < (int)v.size() && i < max_launches && info; ++i) {
+      info[i].chunk = v[i].chunk;
+      info[i].transform_index = v[i].tf;
+      info[i].stage = v[i].stage;
+      info[i].big = v[i].stage == big_stage(c->plans[v[i].chunk][v[i].tf]) ? 1 : 0;
+      info[i].launch = v[i].info;
+    }
+    count = (int)v.size();
+  });
+  return rc == LFX_OK ? count : rc;
+}
+
+int lfx_batch_chain_execute_timed(lfx_batch_chain chain, const void* in_dev, void* const* out_devs, void* stream,
+                                  int32_t nevents, void* const* ev_start, void* const* ev_end) {
+  return guarded([&] {
+    ChainImpl* c = reinterpret_cast<ChainImpl*>(chain);
+    if (!c || !out_devs) fail(LFX_EINVAL, "null chain or output list");
+    const size_t n = chain_launches(c).size();
+    if (nevents < (int32_t)n || !ev_start || !ev_end) fail(LFX_EINVAL, "one start and one end event per launch are needed");
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev(n);
+    for (size_t i = 0; i < n; ++i) {
+      if (!ev_start[i] || !ev_end[i]) fail(LFX_EINVAL, "null event");
+      ev[i] = {(hipEvent_t)ev_start[i], (hipEvent_t)ev_end[i]};
+    }
+    chain_execute(c, in_dev, out_devs, (hipStream_t)stream, &ev);
+  });
+}
+
+int lfx_batch_chain_num_chunks(lfx_batch_chain chain) {
+  ChainImpl* c = reinterpret_cast<ChainImpl*>(chain);
+  return c ? c->nchunks : LFX_EINVAL;
+}
+
+void lfx_batch_chain_destroy(lfx_batch_chain chain) {
+  ChainImpl* c = reinterpret_cast<ChainImpl*>(chain);
+  if (c) free_chain(c);
 }
 
 static void check_tuning(long long tile, int levels_per_pass) {

@@ -67,11 +67,18 @@ def tuning(**knobs):
         set_tuning(**old)
 
 
-def default_space(dtype):
-    """Candidate knob values searched by tune_batch for a storage type."""
+def default_space(dtype, chain=False):
+    """Candidate knob values searched by tune_batch for a storage type.
+    chain=True: the plans run as a BatchChain, and "chunks" (signal chunks of
+    the pipeline; 1 = no pipelining) is searched too."""
     import torch
     if dtype == torch.float64:
-        return {"tile": [0, 2048, 8192], "levels_per_pass": [0, 4], "side_streams": [-1, 0]}
+        space = {"tile": [0, 2048, 8192], "levels_per_pass": [0, 4], "side_streams": [-1, 0]}
+        if chain:
+            space = {"chunks": [2, 1], "tile": [0, 2048, 8192], "levels_per_pass": [0, 4]}
+        return space
+    if chain:
+        return {"chunks": [2, 1, 3], "levels_per_pass": [0, 3], "line_limit": [0, 32]}
     return {"levels_per_pass": [0, 3], "line_limit": [0, 32, 1024], "side_streams": [-1, 0]}
 
 
@@ -80,6 +87,9 @@ def _time_plans(plans, x, bufs, reps, warmup):
     stream = torch.cuda.current_stream(x.device)
 
     def run():
+        if len(plans) == 1 and hasattr(plans[0], "transforms"):  # a BatchChain
+            plans[0].execute(x, bufs)
+            return
         src = x
         for p, b in zip(plans, bufs):
             p.execute(src, b)
@@ -101,8 +111,20 @@ def _time_plans(plans, x, bufs, reps, warmup):
     return times[len(times) // 2]
 
 
+def build(signals, transforms, knobs, mode=0, dtype=None):
+    """The executables a candidate stands for: a list of BatchPlans (one per
+    transform) or, when the knobs name "chunks", one BatchChain."""
+    from .batch import BatchChain, BatchPlan
+    knobs = dict(knobs)
+    chunks = knobs.pop("chunks", None)
+    with tuning(**knobs):
+        if chunks is not None:
+            return [BatchChain(signals, transforms, mode=mode, dtype=dtype, chunks=chunks)]
+        return [BatchPlan(signals, tf, mode=mode, dtype=dtype) for tf in transforms]
+
+
 def tune_batch(signals, transforms, x, mode=0, space=None, reps=5, warmup=2, verbose=False, min_gain=0.02,
-               rounds=2):
+               rounds=2, chain=False):
     """Search planning knobs for batched plans of `signals`.
 
     transforms: transform types run in sequence (x -> out_0 -> out_1 ...),
@@ -118,8 +140,7 @@ def tune_batch(signals, transforms, x, mode=0, space=None, reps=5, warmup=2, ver
     within run-to-run noise, and keeping the defaults then keeps the chosen
     plans the same from run to run."""
     import torch
-    from .batch import BatchPlan
-    space = dict(space if space is not None else default_space(x.dtype))
+    space = dict(space if space is not None else default_space(x.dtype, chain))
     keys = list(space)
     cands = [dict(zip(keys, values)) for values in itertools.product(*(space[k] for k in keys))]
     best_ms = [float("inf")] * len(cands)
@@ -127,8 +148,7 @@ def tune_batch(signals, transforms, x, mode=0, space=None, reps=5, warmup=2, ver
     for rnd in range(max(1, int(rounds))):
         for i, knobs in enumerate(cands):
             t0 = time.perf_counter()
-            with tuning(**knobs):
-                plans = [BatchPlan(signals, tf, mode=mode, dtype=x.dtype) for tf in transforms]
+            plans = build(signals, transforms, knobs, mode=mode, dtype=x.dtype)
             try:
                 ms = _time_plans(plans, x, bufs, reps, warmup)
             finally:
