@@ -11,7 +11,13 @@ stream-ordered on torch's current stream; float32, float64, bfloat16 and
 float16 -- the 16-bit types are computed in float32) or a numpy array
 (float32 / float64; copied to the GPU and back, the reference's calling
 convention). ``level`` int -> level-by-level transform, sequence -> separable
-with one level count per axis. There is no CPU fallback.
+with one level count per axis.
+
+``device='cpu'`` runs host arrays (numpy, or torch CPU tensors) through the
+CPU product path (liblifted_cpu.so: the same line bodies compiled for the
+host, lines split over ``n_threads`` threads, 0 = all cores); results equal
+the GPU's bit for bit. It is chosen explicitly: the GPU path never falls back
+to it, and a missing HIP library fails loudly.
 """
 import enum
 import numbers
@@ -181,11 +187,58 @@ def _is_torch(x):
     return isinstance(x, torch.Tensor)
 
 
-def _lwt(x, wavelet, mode, op, axes, level, n_threads=1, out=None, plain=False):
+def _lwt_cpu(x, wid, bc, op, axes_, level_, levels_, n_threads, out):
+    """device='cpu': numpy arrays (or torch CPU tensors) through
+    liblifted_cpu.so, the lines of each axis split over n_threads threads
+    (0: all cores), as pylifted's n_threads does (_wavelets.pyx:177-268)."""
+    import numpy as np
+    sep = levels_ is not None
+    if _is_torch(x):
+        if x.is_cuda:
+            raise ValueError("device='cpu' takes host arrays (numpy, or torch CPU tensors)")
+        import torch
+        if x.dtype not in (torch.float32, torch.float64):
+            raise TypeError(f"unsupported dtype {x.dtype}: float32 or float64")
+        if out is None:
+            out = torch.empty_like(x)
+        if out.dtype != x.dtype or out.is_cuda or tuple(out.shape) != tuple(x.shape):
+            raise ValueError("`out` must be a CPU tensor of the input's dtype and shape")
+        storage = _lib.LFX_F32 if x.dtype == torch.float32 else _lib.LFX_F64
+        d = _lib.make_desc(wid, op, bc, list(x.shape), list(x.stride()), list(out.stride()), axes_,
+                           levels_ if sep else [level_], sep, storage)
+        _lib.cpu_check(_lib.cpu_lib().lfx_cpu_lwt(d, x.data_ptr(), out.data_ptr(), int(n_threads)))
+        return out
+    if not isinstance(x, np.ndarray):
+        raise TypeError("x must be a numpy array or a torch tensor")
+    if x.dtype not in (np.float32, np.float64):
+        raise TypeError(f"unsupported dtype {x.dtype}: float32 or float64")
+    if out is None:
+        out = np.empty(x.shape, dtype=x.dtype, order="F" if (x.flags.f_contiguous and not
+                                                              x.flags.c_contiguous) else "C")
+    if out.dtype != x.dtype:
+        raise ValueError(f"`out.dtype`:{out.dtype} does not match `input.dtype`:{x.dtype}.")
+    for i in range(x.ndim):
+        if x.shape[i] != out.shape[i]:
+            raise ValueError(f"`out.shape` does not match input along axis {i}.")
+    es = x.itemsize
+    storage = _lib.LFX_F32 if x.dtype == np.float32 else _lib.LFX_F64
+    d = _lib.make_desc(wid, op, bc, list(x.shape), [s // es for s in x.strides], [s // es for s in out.strides],
+                       axes_, levels_ if sep else [level_], sep, storage)
+    _lib.cpu_check(_lib.cpu_lib().lfx_cpu_lwt(d, x.ctypes.data, out.ctypes.data, int(n_threads)))
+    return out
+
+
+def _lwt(x, wavelet, mode, op, axes, level, n_threads=1, out=None, plain=False, device=None):
     wid = _wavelet_id(wavelet)
     bc = _mode_id(mode)
     axes_ = get_axes(axes, x.ndim)
     level_, levels_ = get_level(level, axes_)
+    if device == "cpu":
+        if plain:
+            raise ValueError("plain applies to the GPU kernels only")
+        return _lwt_cpu(x, wid, bc, op, axes_, level_, levels_, n_threads, out)
+    if device not in (None, "gpu", "cuda"):
+        raise ValueError(f"device must be None / 'gpu' (the HIP kernels) or 'cpu', not {device!r}")
     sep = levels_ is not None
     flags = LFX_FLAG_PLAIN if plain else 0
     L = _lib.lib()
@@ -230,27 +283,35 @@ def _lwt(x, wavelet, mode, op, axes, level, n_threads=1, out=None, plain=False):
     return out
 
 
-def lwt_forward(x, wavelet, mode=BoundaryCondition.Zero, axes=None, level=1, n_threads=1, out=None, plain=False):
-    return _lwt(x, wavelet, mode, Transform.Forward, axes, level, n_threads, out=out, plain=plain)
+def lwt_forward(x, wavelet, mode=BoundaryCondition.Zero, axes=None, level=1, n_threads=1, out=None,
+        plain=False, device=None):
+    return _lwt(x, wavelet, mode, Transform.Forward, axes, level, n_threads, out=out, plain=plain,
+                device=device)
 
 
-def lwt_inverse(x, wavelet, mode=BoundaryCondition.Zero, axes=None, level=1, n_threads=1, out=None, plain=False):
-    return _lwt(x, wavelet, mode, Transform.Inverse, axes, level, n_threads, out=out, plain=plain)
+def lwt_inverse(x, wavelet, mode=BoundaryCondition.Zero, axes=None, level=1, n_threads=1, out=None,
+        plain=False, device=None):
+    return _lwt(x, wavelet, mode, Transform.Inverse, axes, level, n_threads, out=out, plain=plain,
+                device=device)
 
 
 def lwt_forward_adjoint(x, wavelet, mode=BoundaryCondition.Zero, axes=None, level=1, n_threads=1, out=None,
-                        plain=False):
-    return _lwt(x, wavelet, mode, Transform.ForwardAdjoint, axes, level, n_threads, out=out, plain=plain)
+        plain=False, device=None):
+    return _lwt(x, wavelet, mode, Transform.ForwardAdjoint, axes, level, n_threads, out=out, plain=plain,
+                device=device)
 
 
 def lwt_inverse_adjoint(x, wavelet, mode=BoundaryCondition.Zero, axes=None, level=1, n_threads=1, out=None,
-                        plain=False):
-    return _lwt(x, wavelet, mode, Transform.InverseAdjoint, axes, level, n_threads, out=out, plain=plain)
+        plain=False, device=None):
+    return _lwt(x, wavelet, mode, Transform.InverseAdjoint, axes, level, n_threads, out=out, plain=plain,
+                device=device)
 
 
-def lwt(x, wavelet, transform, mode=BoundaryCondition.Zero, axes=None, level=1, out=None, plain=False):
-    """Any of the four transforms by Transform value."""
-    return _lwt(x, wavelet, mode, int(transform), axes, level, 1, out=out, plain=plain)
+def lwt(x, wavelet, transform, mode=BoundaryCondition.Zero, axes=None, level=1, out=None, plain=False, device=None,
+        n_threads=1):
+    """Any of the four transforms by Transform value. device='cpu' runs the
+    CPU product path on host arrays with n_threads threads (0: all cores)."""
+    return _lwt(x, wavelet, mode, int(transform), axes, level, n_threads, out=out, plain=plain, device=device)
 
 
 def max_level(wavelet, n):

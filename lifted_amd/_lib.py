@@ -110,3 +110,29 @@ def make_desc(wavelet, transform, bc, shape, stride_in, stride_out, axes, levels
     d.flags = int(flags)
     d._keep = keep  # keep the arrays alive with the struct
     return d
+
+
+CPU_LIB_PATH = os.path.join(HERE, "liblifted_cpu.so")
+_cpu = None
+
+
+def cpu_lib():
+    """liblifted_cpu.so (include/lifted_cpu.h): the CPU product path."""
+    global _cpu
+    if _cpu is None:
+        if not os.path.exists(CPU_LIB_PATH):
+            raise ImportError(f"{CPU_LIB_PATH} is missing: build it with `python -m lifted_amd.build`")
+        L = ctypes.CDLL(CPU_LIB_PATH)
+        L.lfx_cpu_lwt.argtypes = [ctypes.POINTER(LfxDesc), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]
+        L.lfx_cpu_lwt.restype = ctypes.c_int
+        L.lfx_cpu_last_error.restype = ctypes.c_char_p
+        L.lfx_cpu_max_level.argtypes = [ctypes.c_uint32, ctypes.c_int64]
+        L.lfx_cpu_max_level.restype = ctypes.c_int64
+        _cpu = L
+    return _cpu
+
+
+def cpu_check(rc):
+    if rc < 0:
+        raise LfxError(rc, cpu_lib().lfx_cpu_last_error().decode())
+    return rc

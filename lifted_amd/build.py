@@ -99,6 +99,29 @@ def build(verbose=False, incremental=False):
     return OUT
 
 
+CPU_OUT = os.path.join(HERE, "liblifted_cpu.so")
+
+
+def build_cpu(verbose=False):
+    """Build lifted_amd/liblifted_cpu.so: the CPU product path (the kernels'
+    line bodies compiled for the host with g++, explicit FMAs, no contraction)."""
+    cxx = shutil.which("g++") or "g++"
+    cmd = [cxx, "-O3", "-std=c++17", "-march=x86-64-v3", "-ffp-contract=off", "-fPIC", "-shared",
+           "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas", "-Wno-ignored-attributes",
+           os.path.join(CSRC, "lfx_cpu.cpp"), os.path.join(CSRC, "lfx_table.cpp"), "-o", CPU_OUT + ".tmp",
+           "-lpthread"]
+    if verbose:
+        print(" ".join(cmd))
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise subprocess.CalledProcessError(r.returncode, cmd)
+    os.replace(CPU_OUT + ".tmp", CPU_OUT)
+    return CPU_OUT
+
+
 if __name__ == "__main__":
     build(verbose="-v" in sys.argv, incremental="--incr" in sys.argv)
+    build_cpu(verbose="-v" in sys.argv)
     print(OUT)
+    print(CPU_OUT)

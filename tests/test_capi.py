@@ -14,10 +14,11 @@ from helpers import WAVELETS, WIDTHS
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared_symbols():
+def _declared_symbols(header=None):
+    """Functions declared in include/*.h (or in one header)."""
     syms = set()
     for fn in os.listdir(os.path.join(REPO, "include")):
-        if not fn.endswith(".h"):
+        if not fn.endswith(".h") or (header is not None and fn != header):
             continue
         text = open(os.path.join(REPO, "include", fn)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
@@ -26,18 +27,26 @@ def _declared_symbols():
     return syms
 
 
-def test_library_exports_every_declared_symbol():
-    from lifted_amd import _lib
-    L = _lib.lib()
-    syms = _declared_symbols()
-    assert len(syms) >= 10
-    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True,
-                         check=True).stdout
+def _check_exports(path, syms, L):
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
     exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
     missing = sorted(s for s in syms if s not in exported)
     assert not missing, missing
     for s in syms:
         getattr(L, s)
+
+
+def test_library_exports_every_declared_symbol():
+    """liblifted_hip.so exports every function of include/lifted_hip.h, and
+    liblifted_cpu.so every function of include/lifted_cpu.h."""
+    from lifted_amd import _lib
+    syms = _declared_symbols("lifted_hip.h")
+    assert len(syms) >= 10
+    _check_exports(_lib.LIB_PATH, syms, _lib.lib())
+    cpu = _declared_symbols("lifted_cpu.h")
+    assert len(cpu) >= 3
+    _check_exports(_lib.CPU_LIB_PATH, cpu, _lib.cpu_lib())
+    assert _declared_symbols() == syms | cpu
 
 
 def test_max_level_matches_oracle(oracle):
