@@ -34,6 +34,10 @@ SOURCES = [
     ("lfx_rows_tf1.hip", True),
     ("lfx_rows_tf2.hip", True),
     ("lfx_rows_tf3.hip", True),
+    ("lfx_lvl2d_tf0.hip", True),
+    ("lfx_lvl2d_tf1.hip", True),
+    ("lfx_lvl2d_tf2.hip", True),
+    ("lfx_lvl2d_tf3.hip", True),
     ("lfx_api.cpp", False),
     ("lfx_table.cpp", False),
 ]

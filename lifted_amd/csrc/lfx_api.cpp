@@ -35,6 +35,7 @@
 #include "lfx_schedule.hpp"
 #include "lfx_noalloc.hpp"
 #include "lfx_plan.hpp"
+#include "lfx_lvl2d_plan.hpp"
 
 namespace lfx {
 
@@ -452,6 +453,49 @@ static void transform_lines(const Plan& p, const WaveletDefL& w, const void* pro
     transform_lines_t<float>(p, w, prog, fwd, shp, ax, L, src, sst, dst, dstr, st, ctx);
 }
 
+// A level-by-level transform over two axes as one launch per level of
+// two-axis register tiles (lfx_lvl2d.hpp): float32 arithmetic (float32 or
+// 16-bit storage), table wavelets, Zero boundary, the second-named axis with
+// unit stride on both sides (coalesced lanes), input and output distinct, and
+// every level's bands long enough for the tiles. Returns false (nothing
+// launched) when the call does not qualify.
+static bool lbl2d(const Plan& p, const WaveletDefL& w, long long L, const void* in, void* out, hipStream_t st) {
+  if (g_tuning.nd_batch == 0 || p.axes.size() != 2 || p.bc != kZero || p.storage == kF64 ||
+      (p.flags & LFX_FLAG_PLAIN) || p.wavelet >= kCustomBase || !has_lift_steps(w) || L < 1 || in == out)
+    return false;
+  const int ra = p.axes[0], ca = p.axes[1];
+  if (ra == ca || p.sin[ca] != 1 || p.sout[ca] != 1) return false;
+  // the arrays' memory spans must not overlap (levels read the input while
+  // they write the output)
+  const long long es_ = (long long)storage_bytes(p.storage);
+  long long ilo = 0, ihi = 0, olo = 0, ohi = 0;
+  for (int k = 0; k < p.ndim; ++k) {
+    if (p.shape[k] == 0) return false;
+    const long long ei = (p.shape[k] - 1) * p.sin[k], eo = (p.shape[k] - 1) * p.sout[k];
+    (ei < 0 ? ilo : ihi) += ei;
+    (eo < 0 ? olo : ohi) += eo;
+  }
+  const char* ib = static_cast<const char*>(in);
+  const char* ob = static_cast<const char*>(out);
+  if (ib + ilo * es_ <= ob + (ohi + 1) * es_ - 1 && ob + olo * es_ <= ib + (ihi + 1) * es_ - 1) return false;
+  const bool fwd = p.tf == kForward || p.tf == kInverseAdjoint;
+  const DProgram<float> hp = resolve<float>(w, p.tf);
+  // plan once without buffers to learn whether every level admits tiles
+  if (lvl2d_plan(hp, fwd, p.ndim, p.shape, ra, ca, L, in, p.sin, out, p.sout, p.storage, nullptr, nullptr,
+                 nullptr, p.wavelet).empty())
+    return false;
+  const void* prog = device_program(p.wavelet, p.tf, kF32);
+  const size_t es = storage_bytes(p.storage);
+  const size_t wsb = ((size_t)lvl2d_ws_elems(p.ndim, p.shape, ra, ca) * es + 255) / 256 * 256;
+  WsLease lease;
+  lease_workspace(L > 1 ? 2 * wsb : 256, st, lease);
+  char* ws = (char*)lease.p;
+  const auto levels = lvl2d_plan(hp, fwd, p.ndim, p.shape, ra, ca, L, in, p.sin, out, p.sout, p.storage, ws,
+                                 ws + wsb, prog, p.wavelet);
+  for (const auto& lv : levels) hip_check(launch_lvl2d(p.tf, lv, st), "two-axis level kernel");
+  return true;
+}
+
 // Device program of a no-allocation call: its slot in the fixed tables.
 static const void* fixed_program(const RunCtx& ctx, uint32_t wavelet, int tf, int storage) {
   const int i = wavelet < kCustomBase ? table_index(wavelet) : -1;
@@ -525,6 +569,7 @@ static void run_plan(const Plan& p, const void* in, void* out, hipStream_t st, c
     if (in != out) copy_box(p.storage, p.ndim, p.shape, in, p.sin, out, p.sout, st);
     return;
   }
+  if (!ctx && lbl2d(p, w, L, in, out, st)) return;
   if (!fwd_stepping && in != out) {
     copy_box(p.storage, p.ndim, p.shape, in, p.sin, out, p.sout, st);
     src = out;

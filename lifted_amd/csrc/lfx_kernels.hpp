@@ -37,6 +37,23 @@ inline hipError_t launch_reg(int tf, const void* progs, const BatchItem* items, 
   }
   return hipErrorInvalidValue;
 }
+// One level of a two-axis level-by-level transform (l2d::Level, lfx_lvl2d.hpp).
+namespace l2d {
+struct Level;
+}
+hipError_t launch_lvl2d_0(const l2d::Level& L, hipStream_t st);
+hipError_t launch_lvl2d_1(const l2d::Level& L, hipStream_t st);
+hipError_t launch_lvl2d_2(const l2d::Level& L, hipStream_t st);
+hipError_t launch_lvl2d_3(const l2d::Level& L, hipStream_t st);
+inline hipError_t launch_lvl2d(int tf, const l2d::Level& L, hipStream_t st) {
+  switch (tf) {
+    case 0: return launch_lvl2d_0(L, st);
+    case 1: return launch_lvl2d_1(L, st);
+    case 2: return launch_lvl2d_2(L, st);
+    case 3: return launch_lvl2d_3(L, st);
+  }
+  return hipErrorInvalidValue;
+}
 // Row items (float32, Zero boundary): four equal short lines per wavefront.
 hipError_t launch_rows_0(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
                          void* scr, hipStream_t st);

@@ -1,0 +1,50 @@
+// Two-axis level kernel of one transform type (included once per transform
+// by lfx_lvl2d_tf{0,1,2,3}.hip, so the table's instantiations compile in
+// parallel).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "lfx_kernels.hpp"
+#include "lfx_lvl2d.hpp"
+#include "lfx_xcd.hpp"
+
+namespace lfx {
+
+// One level of a two-axis level-by-level transform (lfx_lvl2d.hpp): one
+// wavefront per tile, any table wavelet (its id selects the compiled step
+// shapes; a wave-uniform branch).
+template <int TF>
+__global__ void __launch_bounds__(64) k_lvl2d(l2d::Level L) {
+  const long long w = (long long)xcd_block(blockIdx.x, gridDim.x);
+  if (w >= L.ntiles) return;
+  switch (L.wavelet) {
+#define LFX_WAVELET(nm, idv, w_)                                 \
+  case idv:                                                     \
+    l2d::level_tile<reg::LaneGpu, wt::WL_##nm, TF>(L, w);       \
+    return;
+#define LFX_GENERIC(...)
+#define LFX_UNIT(...)
+#define LFX_REPEAT(...)
+#define LFX_SCALE(...)
+#define LFX_END(nm)
+#include "wavelet_table.inc"
+#undef LFX_WAVELET
+#undef LFX_GENERIC
+#undef LFX_UNIT
+#undef LFX_REPEAT
+#undef LFX_SCALE
+#undef LFX_END
+    default:
+      return;
+  }
+}
+
+template <int TF>
+hipError_t launch_lvl2d_tf(const l2d::Level& L, hipStream_t st) {
+  if (L.ntiles <= 0) return hipSuccess;
+  if (L.ntiles > 0x7fffffffll) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_lvl2d<TF>), dim3((unsigned)L.ntiles), dim3(64), 0, st, L);
+  return hipGetLastError();
+}
+
+}  // namespace lfx

@@ -44,8 +44,8 @@ constexpr int kLevels = kRegLevels;           // levels per register tile
 constexpr int kPerLane = kRegPerLane;         // samples per lane at the finest level
 constexpr int kWindow = (int)kRegWindow;      // kLanes * kPerLane samples
 constexpr int kUnits = (int)kRegUnits;        // the window in level-4 units
-static_assert(kLevels == 4 && kWindow == kLanes * kPerLane && (kPerLane == 16 || kPerLane == 32),
-              "register tiles run four levels on 16 or 32 samples per lane");
+static_assert(kLevels == 4 && kWindow == kLanes * kPerLane && (kPerLane == 16 || kPerLane == 32 || kPerLane == 64),
+              "register tiles run four levels on 16, 32 or 64 samples per lane");
 
 // Lane policies. ROWS = 1: the wavefront holds one window of a band. ROWS = 4
 // ("row mode"): it holds four equal lines of at most 512 samples, one per row
@@ -174,6 +174,40 @@ struct LaneCpuT {
         if (r < nr && g >= lo && g < hi) dev::store_as<S, float>(p + r * rs, pos0 + g, v[e].v[l]);
       }
   }
+  // per-lane values, selects and visits (lane-indexed functions)
+  template <class F>
+  static V gen(F f) {
+    V r;
+    for (int l = 0; l < kLanes; ++l) r.v[l] = f(l);
+    return r;
+  }
+  template <class F>
+  static V where(F pred, const V& a, const V& b) {
+    V r;
+    for (int l = 0; l < kLanes; ++l) r.v[l] = pred(l) ? a.v[l] : b.v[l];
+    return r;
+  }
+  template <class F>
+  static void each(const V& v, F f) {
+    for (int l = 0; l < kLanes; ++l) f(l, v.v[l]);
+  }
+  // value of v held by lane src(l), for every lane l
+  template <class F>
+  static V permute(const V& v, F src) {
+    V r;
+    for (int l = 0; l < kLanes; ++l) r.v[l] = v.v[src(l)];
+    return r;
+  }
+  // the same mask on every lane: bit e iff p0 + e lies in [0, n)
+  static M umask(int E, long long p0, long long n) {
+    M r;
+    for (int l = 0; l < kLanes; ++l) {
+      r.m[l] = 0;
+      for (int e = 0; e < E; ++e)
+        if (p0 + e >= 0 && p0 + e < n) r.m[l] |= 1u << e;
+    }
+    return r;
+  }
 };
 using LaneCpu = LaneCpuT<1>;
 using RowCpu = LaneCpuT<4>;
@@ -290,9 +324,9 @@ struct LaneGpuT {
     }
   }
   // Up to 16 values per lane go through the buffer at once (64 rows of E + 1
-  // floats); 32 per lane go in two rounds of 1024 values, round h filling
-  // the chunks of lanes [32 h, 32 h + 32) (32 rows of 33 floats), so the
-  // buffer never exceeds 64 x 17 floats.
+  // floats). E = 32 or 64 per lane go in E / 16 rounds of 1024 values: round
+  // h fills the chunks of lanes [h P, h P + P), P = 1024 / E (P rows of E + 1
+  // floats), so the buffer never exceeds 64 x 17 floats.
   template <int E>
   static __device__ __forceinline__ void xpose_in(const V (&t)[E], V (&x)[E], float* lds) {
     const int l = lane();
@@ -307,16 +341,20 @@ struct LaneGpuT {
       for (int e = 0; e < E; ++e) x[e] = lds[l * (E + 1) + e];
       wave_sync();
     } else {
-      static_assert(E == 32, "register chunks hold at most 32 samples per lane");
+      static_assert(E == 32 || E == 64, "register chunks hold at most 64 samples per lane");
+      constexpr int P = 1024 / E;
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        // value j of round h: u = (16 h + j) 64 + l -> lane u / 32 = 32 h + 2 j + l / 32, slot l % 32
+      for (int h = 0; h < E / 16; ++h) {
+        // value j of round h: u = (16 h + j) 64 + l -> lane u / E (row u / E - h P), slot u % E
 #pragma unroll
-        for (int j = 0; j < 16; ++j) lds[(2 * j + (l >> 5)) * 33 + (l & 31)] = t[16 * h + j];
+        for (int j = 0; j < 16; ++j) {
+          const int u = (16 * h + j) * kLanes + l;
+          lds[(u / E - h * P) * (E + 1) + (u % E)] = t[16 * h + j];
+        }
         wave_sync();
-        if ((l >> 5) == h) {
+        if (l / P == h) {
 #pragma unroll
-          for (int e = 0; e < 32; ++e) x[e] = lds[(l & 31) * 33 + e];
+          for (int e = 0; e < E; ++e) x[e] = lds[(l - h * P) * (E + 1) + e];
         }
         wave_sync();
       }
@@ -375,23 +413,46 @@ struct LaneGpuT {
       }
       wave_sync();
     } else {
-      static_assert(E == 32, "register chunks hold at most 32 samples per lane");
-      // two rounds of 1024 values (see xpose_in)
+      static_assert(E == 32 || E == 64, "register chunks hold at most 64 samples per lane");
+      // E / 16 rounds of 1024 values (see xpose_in)
+      constexpr int P = 1024 / E;
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        if ((l >> 5) == h) {
+      for (int h = 0; h < E / 16; ++h) {
+        if (l / P == h) {
 #pragma unroll
-          for (int e = 0; e < 32; ++e) lds[(l & 31) * 33 + e] = v[e];
+          for (int e = 0; e < E; ++e) lds[(l - h * P) * (E + 1) + e] = v[e];
         }
         wave_sync();
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
           const int u = (16 * h + j) * kLanes + l;
-          st_one<E>(p, pos0, g0, lo, hi, t0, t1, u, lds[(2 * j + (l >> 5)) * 33 + (l & 31)], rs, nr);
+          st_one<E>(p, pos0, g0, lo, hi, t0, t1, u, lds[(u / E - h * P) * (E + 1) + (u % E)], rs, nr);
         }
         wave_sync();
       }
     }
+  }
+  template <class F>
+  static __device__ __forceinline__ V gen(F f) {
+    return f(lane());
+  }
+  template <class F>
+  static __device__ __forceinline__ V where(F pred, V a, V b) {
+    return pred(lane()) ? a : b;
+  }
+  template <class F>
+  static __device__ __forceinline__ void each(V v, F f) {
+    f(lane(), v);
+  }
+  // value of v held by lane src(lane) (one ds_bpermute)
+  template <class F>
+  static __device__ __forceinline__ V permute(V v, F src) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src(lane()) << 2, __builtin_bit_cast(int, v)));
+  }
+  static __device__ __forceinline__ M umask(int E, long long p0, long long n) {
+    M r = 0;
+    for (int e = 0; e < E; ++e) r |= (p0 + e >= 0 && p0 + e < n) ? (1u << e) : 0u;
+    return r;
   }
 };
 using LaneGpu = LaneGpuT<1>;

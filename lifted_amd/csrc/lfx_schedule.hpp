@@ -289,4 +289,36 @@ std::vector<ItemSpec> pass_items(const DProgram<C>& hp, bool fwd, const PassPlan
   return out;
 }
 
+
+// ---------------------------------------------------------------------------
+// Two-axis level tiles (lfx_lvl2d.hpp): halo (fine positions, even) of a
+// window of W positions along one axis of band length m, for one level of
+// program hp under the Zero boundary, so that every tile owns an exact range;
+// -1 when no halo up to W / 4 works, or the band is too short for masked
+// windows (zero_cover_ok).
+template <class C>
+long long lvl2d_axis_halo(const DProgram<C>& hp, bool fwd, long long m, long long W) {
+  if (!zero_cover_ok(hp, m, 1)) return -1;
+  const long long M = 1ll << 40, a0 = 1ll << 36;
+  long long H = -1;
+  for (long long h = 0; 4 * h <= W; h += 2) {
+    const bool ok = fwd ? fwd_tile_ok(hp, M, 1, a0, a0 + W - 2 * h, h, kZero)
+                        : rev_tile_ok(hp, M, 1, a0, a0 + W - 2 * h, h / 2, kZero);
+    if (ok) {
+      H = h;
+      break;
+    }
+  }
+  if (H < 0) return -1;
+  // tiles next to the band ends lose what the boundary reaches: check them
+  // with the exact rule (interior tiles are translates of the checked one)
+  const long long O = W - 2 * H, T = (m + O - 1) / O;
+  for (long long t : probe_tiles(T)) {
+    const long long a = t * O, b = std::min(a + O, m);
+    const bool ok = fwd ? fwd_tile_ok(hp, m, 1, a, b, H, kZero) : rev_tile_ok(hp, m, 1, a, b, H / 2, kZero);
+    if (!ok) return -1;
+  }
+  return H;
+}
+
 }  // namespace lfx

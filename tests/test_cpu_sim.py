@@ -274,3 +274,59 @@ def test_boundary_maps_closed_form(sim):
     out = r.stdout.decode()
     assert r.returncode == 0 and out.startswith("ok"), out + r.stderr.decode()
     print(out.strip())
+
+
+def _lvl2d_sim(w, tf, x, L, storage=0):
+    sim2 = os.path.join(HERE, "cpu_sim", "build", "sim_lvl2d")
+    B, R, C = x.shape
+    data = x.astype(np.float32).tobytes() if storage == 0 else x.tobytes()
+    r = subprocess.run([sim2, str(w), str(tf), str(storage), str(B), str(R), str(C), str(L)], input=data,
+                       capture_output=True)
+    if r.returncode == 3:
+        return None, r.stderr.decode()
+    assert r.returncode == 0, r.stderr.decode()[-3000:]
+    dt = np.float32 if storage == 0 else np.uint16
+    return np.frombuffer(r.stdout, dtype=dt).reshape(x.shape), r.stderr.decode()
+
+
+def test_two_axis_level_tiles_match_oracle(sim, oracle):
+    """The two-axis level tiles (lfx_lvl2d.hpp) on the 64-lane CPU model,
+    driven by the library's plan (every tile of every level, with the
+    workspace ping-pong): level-by-level transforms over the last two axes of
+    batched 2-D arrays, odd and even sizes, band-end tiles, all transforms,
+    bit for bit against the oracle; shapes whose bands are too short are
+    refused (the library then takes the chained path)."""
+    g = np.random.default_rng(2024)
+    cases = [(100907, (1, 300, 200), 1), (100907, (2, 300, 200), 3), (2, (1, 129, 257), 4), (10608, (1, 500, 260), 2),
+             (1, (2, 64, 64), 3), (3, (3, 97, 131), 3), (105, (1, 250, 333), 3), (10202, (2, 65, 66), 4),
+             (10301, (1, 120, 90), 2), (100907, (1, 1000, 70), 5)]
+    planned = 0
+    for w, shape, L in cases:
+        x = g.standard_normal(shape).astype(np.float32)
+        for tf in (0, 1, 2, 3):
+            got, info = _lvl2d_sim(w, tf, x, L)
+            if got is None:
+                continue
+            planned += 1
+            ref = oracle.lwt(x, w, tf, 0, axes=[1, 2], level=L)
+            np.testing.assert_array_equal(got, ref, err_msg=f"w={w} shape={shape} L={L} tf={tf}\n{info}")
+    assert planned >= 30
+    # too short for the tiles: refused
+    got, info = _lvl2d_sim(100503, 0, g.standard_normal((1, 77, 1000)).astype(np.float32), 5)
+    assert got is None and "no plan" in info
+
+
+def test_two_axis_level_tiles_bf16(sim, oracle):
+    """bf16 storage through the two-axis tiles: every level rounds its result
+    once to bf16 (its ss quadrant goes through a bf16 workspace), which is the
+    oracle applied one level at a time with that rounding."""
+    import torch
+    from helpers import oracle_level_by_level_rounded, round_to
+    x = round_to(np.random.default_rng(5).standard_normal((2, 200, 150)), "bfloat16")
+    xb = torch.from_numpy(x).to(torch.bfloat16).view(torch.uint16).numpy()
+    for tf in (0, 1):
+        got, info = _lvl2d_sim(100907, tf, xb, 3, storage=2)
+        assert got is not None, info
+        gotf = torch.from_numpy(got.copy()).view(torch.bfloat16).float().numpy()
+        ref = oracle_level_by_level_rounded(oracle, x, 100907, tf, 0, [1, 2], 3, "bfloat16")
+        np.testing.assert_array_equal(gotf, ref, err_msg=f"tf={tf}")
