@@ -223,6 +223,9 @@ typedef struct {
   int32_t levels_per_pass; /* levels per tiled pass (1..12), 0 = default (4 for register tiles) */
   int32_t side_streams;    /* side streams of a batched plan (0..8), -1 = default (3) */
   int64_t line_limit;      /* register-tiled plans: longest band handed to the whole-line kernels, 0 = default */
+  int32_t across;          /* ND calls: strided float32 axes of at most 512 samples (Zero boundary) read in
+                              place by the across-line kernel instead of transposed; 0 = off (default), 1 = on */
+  int32_t pad;
 } lfx_tuning;
 int lfx_set_tuning_ex(const lfx_tuning* t);
 int lfx_get_tuning(lfx_tuning* t);

@@ -38,6 +38,10 @@ SOURCES = [
     ("lfx_lvl2d_tf1.hip", True),
     ("lfx_lvl2d_tf2.hip", True),
     ("lfx_lvl2d_tf3.hip", True),
+    ("lfx_across_tf0.hip", True),
+    ("lfx_across_tf1.hip", True),
+    ("lfx_across_tf2.hip", True),
+    ("lfx_across_tf3.hip", True),
     ("lfx_api.cpp", False),
     ("lfx_table.cpp", False),
 ]

@@ -1,0 +1,95 @@
+// Lines along a strided axis ("across" lines, the reference's Across step
+// variants: src/lifted-steps-inl.hpp:502, :756, :1013, :1180, chosen at
+// src/lifted-drivers-inl.hpp:475-482) without a transpose pass: a workgroup
+// takes kLines lines that are neighbours along the array's contiguous dim,
+// reads them into an LDS slab position by position (kLines consecutive
+// elements per position: coalesced), runs every level of every line from the
+// slab with the row-mode register tiles (lfx_reg.hpp: four lines per
+// wavefront, 16 lanes of 32 samples per line, DPP row shifts, Zero boundary),
+// in place, and writes the slab back the same way. Each sample is read once
+// and written once per axis; the slab never leaves the CU.
+//
+// Slab layout: line c, sample p at virtual float index c * kMaxN + p, stored
+// at phys(index) = index + index / 32 + index / 512. The 32-sample chunks of
+// a lane then start in distinct banks (reads and writes of the row tiles are
+// conflict-free), and the lines sit 529 floats apart (an odd number of banks),
+// so the position-by-position loads and stores are conflict-free too.
+#pragma once
+#include "lfx_across_types.hpp"
+#include "lfx_lvl2d.hpp"
+
+namespace lfx {
+namespace acr {
+
+// First line of block `blk` along the contiguous dim and the element offsets
+// of its slice in the input and the output.
+LFX_HDI void block_origin(const Args& A, long long blk, long long& c0, long long& oi, long long& oo) {
+  long long b = blk / A.ncb;
+  c0 = (blk % A.ncb) * kLines;
+  oi = c0;
+  oo = c0;
+  for (int kk = 0; kk < kMaxDims; ++kk) {
+    const int k = kMaxDims - 1 - kk;
+    if (k < A.nb) {
+      const long long q = b % A.bshape[k];
+      b /= A.bshape[k];
+      oi += q * A.bs_in[k];
+      oo += q * A.bs_out[k];
+    }
+  }
+}
+
+// Thread tid of kThreads moves line tid % kLines of a block at positions
+// tid / kLines + j kThreads / kLines (j < kPer): a wavefront's access covers
+// kLines consecutive elements (coalesced) at 64 / kLines positions.
+//
+// fetch: v <- the block's lines (zeros past the dim's end or the line's end);
+// the loads are only issued here, so a caller can overlap them with work on
+// the previous block. put: slab <- v.
+__device__ __forceinline__ void fetch(const Args& A, long long c0, long long oi, int tid, float (&v)[kPer]) {
+  const int c = tid % kLines, p0 = tid / kLines, n = (int)A.n;
+  const bool cin = c < A.nc - c0;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int p = p0 + j * (kThreads / kLines);
+    v[j] = (cin && p < n) ? l2d::load_any(A.in, A.st_in, oi + p * A.sax_in + c) : 0.0f;
+  }
+}
+__device__ __forceinline__ void put(const Args& A, float* slab, int tid, const float (&v)[kPer]) {
+  const int c = tid % kLines, p0 = tid / kLines, n = (int)A.n;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int p = p0 + j * (kThreads / kLines);
+    if (p < n) slab[phys(c * kMaxN + p)] = v[j];
+  }
+}
+
+// The block's lines <- slab (same thread mapping).
+__device__ __forceinline__ void store_block(const Args& A, const float* slab, long long c0, long long oo, int tid) {
+  const int c = tid % kLines, p0 = tid / kLines, n = (int)A.n;
+  if (c >= A.nc - c0) return;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int p = p0 + j * (kThreads / kLines);
+    if (p < n) l2d::store_any(A.out, A.st_out, oo + p * A.sax_out + c, slab[phys(c * kMaxN + p)]);
+  }
+}
+
+// Every level of the four lines of slab rows [4 w, 4 w + 4) (virtual base
+// vb = slab-virtual index 4 w kMaxN, passed as a pointer the policy maps).
+template <class P, class W, int TF>
+__device__ __forceinline__ void run_lines(const Args& A, float* vb) {
+  const DProgram<float>* prog = static_cast<const DProgram<float>*>(A.prog);
+  const reg::RowArgs ra{{kMaxN, kMaxN, kMaxN}, 4};
+  if constexpr (TF == kForward || TF == kInverseAdjoint) {
+    for (int i = 0; i < A.npass; ++i)
+      reg::fwd_tile<P, W, TF>(prog, A.pm[i], A.pk[i], 0, A.pm[i], 0, (const float*)vb, vb, vb, nullptr, ra);
+  } else {
+    for (int i = A.npass - 1; i >= 0; --i)
+      reg::rev_tile<P, W, TF>(prog, A.pm[i], A.pk[i], 0, A.pm[i], 0, (const float*)vb, (const float*)vb, vb, nullptr,
+                              ra);
+  }
+}
+
+}  // namespace acr
+}  // namespace lfx

@@ -1,0 +1,6 @@
+// Across-line kernel for transform type 2 (see lfx_across_kernels.hpp).
+#include "lfx_across_kernels.hpp"
+
+namespace lfx {
+hipError_t launch_across_2(const acr::Args& A, hipStream_t st) { return launch_across_tf<2>(A, st); }
+}  // namespace lfx

@@ -1,0 +1,6 @@
+// Across-line kernel for transform type 3 (see lfx_across_kernels.hpp).
+#include "lfx_across_kernels.hpp"
+
+namespace lfx {
+hipError_t launch_across_3(const acr::Args& A, hipStream_t st) { return launch_across_tf<3>(A, st); }
+}  // namespace lfx

@@ -35,6 +35,7 @@
 #include "lfx_schedule.hpp"
 #include "lfx_noalloc.hpp"
 #include "lfx_plan.hpp"
+#include "lfx_across_plan.hpp"
 #include "lfx_lvl2d_plan.hpp"
 
 namespace lfx {
@@ -254,6 +255,7 @@ struct Tuning {
   int streams = -1;      // side streams of a batched stage (-1: default)
   int nd_batch = -1;     // ND lines through batched plans (0: off)
   long long small = -1;  // LDS bytes up to which batch items run as one-wave workgroups (-1: default)
+  int across = 0;        // strided axes through the across-line kernel (1: on)
 };
 static Tuning env_tuning() {
   Tuning t;
@@ -263,6 +265,7 @@ static Tuning env_tuning() {
   if (const char* e = std::getenv("LFX_STREAMS")) t.streams = std::atoi(e);
   if (const char* e = std::getenv("LFX_SMALL")) t.small = std::atoll(e);
   if (const char* e = std::getenv("LFX_ND_BATCH")) t.nd_batch = std::atoi(e);
+  if (const char* e = std::getenv("LFX_ACROSS")) t.across = std::atoi(e);
   return t;
 }
 static thread_local Tuning g_tuning = env_tuning();
@@ -1517,9 +1520,74 @@ static int fast_dim(int ndim, const long long* shp, const long long* str) {
   return f < 0 ? ndim - 1 : f;
 }
 
+// A dim other than ax, longer than 1, with unit stride in str (-1: none).
+static int unit_dim(int ndim, const long long* shp, const long long* str, int ax) {
+  for (int k = ndim - 1; k >= 0; --k)
+    if (k != ax && shp[k] > 1 && str[k] == 1) return k;
+  return -1;
+}
+
+// Every line of a strided axis `ax` through the across-line kernel
+// (lfx_across.hpp), reading the lines where they are: float32 arithmetic
+// (float32 or 16-bit storage on either side), Zero boundary, table wavelets,
+// lines of at most acr::kMaxN samples whose every pass of up to four levels
+// admits the row tiles, and a dim other than ax with unit stride on both
+// sides (the workgroup's lines). src == dst is allowed with identical strides
+// (a workgroup reads its lines whole before writing them); otherwise the two
+// must not overlap. Returns false (nothing launched) when the call does not
+// qualify.
+static bool across_axis(const Plan& p, const long long* shp, int ax, long long L, const void* src,
+                        const long long* sst, int st_in, void* dst, const long long* dstr, int st_out,
+                        hipStream_t st) {
+  if (g_tuning.nd_batch == 0 || g_tuning.across != 1) return false;
+  if (p.bc != kZero || p.storage == kF64 || (p.flags & LFX_FLAG_PLAIN) || p.wavelet >= kCustomBase ||
+      p.wavelet == 0)
+    return false;
+  const long long n = shp[ax];
+  if (n < 2 || n > acr::kMaxN || L < 1) return false;
+  const int c = unit_dim(p.ndim, shp, sst, ax);
+  if (c < 0 || dstr[c] != 1) return false;
+  long long total = 1;
+  for (int k = 0; k < p.ndim; ++k) total *= shp[k];
+  if (total <= 0) return false;
+  if (src == dst) {
+    for (int k = 0; k < p.ndim; ++k)
+      if (shp[k] > 1 && sst[k] != dstr[k]) return false;
+  } else {
+    long long ilo = 0, ihi = 0, olo = 0, ohi = 0;
+    for (int k = 0; k < p.ndim; ++k) {
+      const long long ei = (shp[k] - 1) * sst[k], eo = (shp[k] - 1) * dstr[k];
+      (ei < 0 ? ilo : ihi) += ei;
+      (eo < 0 ? olo : ohi) += eo;
+    }
+    const char* ib = static_cast<const char*>(src);
+    const char* ob = static_cast<const char*>(dst);
+    const long long bi = (long long)storage_bytes(st_in), bo = (long long)storage_bytes(st_out);
+    if (ib + ilo * bi < ob + (ohi + 1) * bo && ob + olo * bo < ib + (ihi + 1) * bi) return false;
+  }
+  const WaveletDefL& w = find_wavelet(p.wavelet);
+  if (!has_lift_steps(w)) return false;
+  const bool fwd = p.tf == kForward || p.tf == kInverseAdjoint;
+  const DProgram<float> hp = resolve<float>(w, p.tf);
+  acr::Args A{};
+  if (!across_passes(hp, fwd, n, L, A)) return false;
+  across_geometry(p.ndim, shp, ax, c, sst, dstr, A);
+  if (A.nblocks > 0x7fffffffll) return false;
+  A.in = src;
+  A.out = dst;
+  A.st_in = st_in;
+  A.st_out = st_out;
+  A.prog = device_program(p.wavelet, p.tf, kF32);
+  A.wavelet = p.wavelet;
+  hip_check(launch_across(p.tf, A, st), "across-line kernel");
+  return true;
+}
+
 static bool lines_fast(const Plan& p, const long long* shp, int ax, long long L, const void* src, const long long* sst,
                        void* dst, const long long* dstr, hipStream_t st) {
   if (g_tuning.nd_batch == 0) return false;
+  // a strided axis: its lines are read where they are
+  if (sst[ax] != 1 && across_axis(p, shp, ax, L, src, sst, p.storage, dst, dstr, p.storage, st)) return true;
   if (p.storage == kF64 || p.bc == kPeriodic || (p.flags & LFX_FLAG_PLAIN) || p.wavelet >= kCustomBase ||
       p.wavelet == 0)
     return false;
@@ -1639,6 +1707,27 @@ static bool sep_chain(const Plan& p, const long long* shape, const long long* si
     const long long n = shape[ax], nlines = total / n;
     const std::vector<long long> wstr = layout(ax);
     const bool direct = cur_buf < 0 ? (p.storage == kF32 && same(cur_str.data(), wstr)) : same(cur_str.data(), wstr);
+    if (!direct) {
+      // A strided axis whose lines have a unit-stride neighbour dim: the
+      // across-line kernel reads them where they are, into a workspace of
+      // the same dim order (in place once in a workspace), or straight into
+      // the output for the last axis.
+      const int cu = unit_dim(p.ndim, shape, cur_str.data(), ax);
+      const int st_cur = cur_buf < 0 ? p.storage : kF32;
+      if (cu >= 0 && last) {
+        if (across_axis(p, shape, ax, L[ia], cur, cur_str.data(), st_cur, out, sout, p.storage, st)) return true;
+      } else if (cu >= 0) {
+        const int ob = cur_buf < 0 ? 0 : cur_buf;
+        const std::vector<long long> ostr = cur_buf < 0 ? layout(cu) : cur_str;
+        if (across_axis(p, shape, ax, L[ia], cur, cur_str.data(), st_cur, buf[ob], ostr.data(), kF32, st)) {
+          cur = buf[ob];
+          cur_buf = ob;
+          cur_fast = cu;
+          cur_str = ostr;
+          continue;
+        }
+      }
+    }
     const void* pin = cur;
     int in_buf = cur_buf;
     if (!direct) {
@@ -1979,10 +2068,12 @@ int lfx_set_tuning_ex(const lfx_tuning* t) {
     check_tuning(t->tile, t->levels_per_pass);
     if (t->side_streams < -1 || t->side_streams > 8) fail(LFX_EINVAL, "side_streams must be in [-1, 8]");
     if (t->line_limit < 0 || (t->line_limit > 0 && t->line_limit < 16)) fail(LFX_EINVAL, "line_limit must be 0 or >= 16");
+    if (t->across != 0 && t->across != 1) fail(LFX_EINVAL, "across must be 0 or 1");
     g_tuning.tile = t->tile;
     g_tuning.k = t->levels_per_pass;
     g_tuning.streams = t->side_streams;
     g_tuning.line = t->line_limit;
+    g_tuning.across = t->across;
   });
 }
 
@@ -1993,6 +2084,8 @@ int lfx_get_tuning(lfx_tuning* t) {
     t->levels_per_pass = g_tuning.k;
     t->side_streams = g_tuning.streams;
     t->line_limit = g_tuning.line;
+    t->across = g_tuning.across;
+    t->pad = 0;
   });
 }
 

@@ -54,6 +54,24 @@ inline hipError_t launch_lvl2d(int tf, const l2d::Level& L, hipStream_t st) {
   }
   return hipErrorInvalidValue;
 }
+// Every line of a strided axis in blocks of lines staged in LDS
+// (acr::Args, lfx_across.hpp).
+namespace acr {
+struct Args;
+}
+hipError_t launch_across_0(const acr::Args& A, hipStream_t st);
+hipError_t launch_across_1(const acr::Args& A, hipStream_t st);
+hipError_t launch_across_2(const acr::Args& A, hipStream_t st);
+hipError_t launch_across_3(const acr::Args& A, hipStream_t st);
+inline hipError_t launch_across(int tf, const acr::Args& A, hipStream_t st) {
+  switch (tf) {
+    case 0: return launch_across_0(A, st);
+    case 1: return launch_across_1(A, st);
+    case 2: return launch_across_2(A, st);
+    case 3: return launch_across_3(A, st);
+  }
+  return hipErrorInvalidValue;
+}
 // Row items (float32, Zero boundary): four equal short lines per wavefront.
 hipError_t launch_rows_0(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
                          void* scr, hipStream_t st);

@@ -2,17 +2,19 @@
 // reference runs, per level, a single-level transform along each axis of the
 // sub-box in turn (src/lifted-drivers-inl.hpp:488-610), reading and writing
 // the whole sub-box each time. Here one wavefront owns a tile of the sub-box:
-// its 64 lanes are 64 adjacent positions along the contiguous axis (so every
-// load and store is coalesced), and each lane holds R consecutive positions
-// of the other axis in registers. The register axis is transformed within
-// each lane (reg::level, as the 1-D register tiles do), the lane axis across
-// lanes (even lanes hold s, odd lanes d; stencil neighbours are lane shifts),
-// in the order the axes were given. A tile's window includes a halo on each
-// axis (sized on the host by the lfx_tiles.hpp rules, so host and device
-// agree) and stores only the positions it owns, all exact; under the Zero
-// boundary, windows over a band end run masked (values outside the band are
-// zero and re-zeroed after every step, which is what skipping out-of-range
-// terms computes).
+// its 64 lanes run along the contiguous axis, each lane holding kLaneW
+// consecutive positions of it (so every load and store is coalesced, kLaneW
+// elements per lane in one vector access where aligned), and kRows
+// consecutive positions of the other axis in registers. The register axis is
+// transformed within each lane (reg::level over its rows, one column at a
+// time), the lane axis across lanes (reg::level over the lane chunks: each
+// lane holds kLaneE s and kLaneE d values of a row, stencil neighbours in
+// other lanes are lane shifts), in the order the axes were given. A tile's
+// window includes a halo on each axis (sized on the host by the lfx_tiles.hpp
+// rules, so host and device agree) and stores only the positions it owns, all
+// exact; under the Zero boundary, windows over a band end run masked (values
+// outside the band are zero and re-zeroed after every step, which is what
+// skipping out-of-range terms computes).
 //
 // Forward-stepping levels (Forward, InverseAdjoint) read the sub-box and write
 // the three detail quadrants to their places in the output and the ss quadrant
@@ -49,56 +51,150 @@ __device__ __forceinline__ void store_any(void* p, int st, long long i, float v)
   static_cast<float*>(p)[i] = v;
 }
 
-// The lane axis of one register row: lane l holds fine position w0 + l (w0
-// even), an s value on even lanes and a d value on odd lanes; one level of
-// the wavelet's steps (transform TF) across lanes. The x value of band index
-// i + off + k for the lane of band index i sits 2 (off + k) + 1 - 2 par lanes
-// away (par: parity of the updated lanes). Masked: positions outside the
-// band are re-zeroed after each step.
-template <class P, class W, int TF>
-__device__ __forceinline__ void lane_level(typename P::V& v, const float* cf_all, bool masked, long long w0,
-                                           long long ns, long long nd) {
-  using V = typename P::V;
-  using L = typename W::list;
-  reg::sfor<L::size>([&](auto I) {
-    constexpr int i = decltype(I)::value;
-    using ST = wt::Exec<L, TF, i>;
-    const float* cf = cf_all + wt::CoefOff<L, TF, i>::value;
-    if constexpr (ST::kind == kScale) {
-      const float sm = cf[0], dm = cf[1];
-      v = P::where([](int l) { return (l & 1) == 0; }, P::mul(v, sm), P::mul(v, dm));
+#ifndef LFX_CPU_SIM
+// kLaneW (= 2) consecutive elements at i (even, array aligned) as one access
+__device__ __forceinline__ void load_pair(const void* p, int st, long long i, float& a, float& b) {
+  if (st == kF32) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    const f2 v = *reinterpret_cast<const f2*>(static_cast<const float*>(p) + i);
+    a = v.x;
+    b = v.y;
+  } else if (st == kBF16) {
+    const uint32_t u = *reinterpret_cast<const uint32_t*>(static_cast<const uint16_t*>(p) + i);
+    a = __uint_as_float(u << 16);
+    b = __uint_as_float(u & 0xffff0000u);
+  } else {
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    const h2 v = *reinterpret_cast<const h2*>(static_cast<const uint16_t*>(p) + i);
+    a = (float)v.x;
+    b = (float)v.y;
+  }
+}
+__device__ __forceinline__ void store_pair(void* p, int st, long long i, float a, float b) {
+  if (st == kF32) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    f2 v;
+    v.x = a;
+    v.y = b;
+    *reinterpret_cast<f2*>(static_cast<float*>(p) + i) = v;
+  } else if (st == kBF16) {
+    dev::bf16_t h[2];
+    dev::store_as<dev::bf16_t, float>(h, 0, a);
+    dev::store_as<dev::bf16_t, float>(h, 1, b);
+    *reinterpret_cast<uint32_t*>(static_cast<uint16_t*>(p) + i) = (uint32_t)h[0].v | ((uint32_t)h[1].v << 16);
+  } else {
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    h2 v;
+    v.x = (_Float16)a;
+    v.y = (_Float16)b;
+    *reinterpret_cast<h2*>(static_cast<uint16_t*>(p) + i) = v;
+  }
+}
+#endif
+
+// x[k] of lane l = fine position w0 + kLaneW l + k of the row at element
+// offset b of A (0 when !row_in or outside [0, m)); `inside`: the whole window
+// lies in [0, m) (wave-uniform).
+template <class P>
+__device__ __forceinline__ void ld_fine(const Arr& A, long long b, long long w0, long long m, bool row_in, bool inside,
+                                        typename P::V (&x)[kLaneW]) {
+#ifndef LFX_CPU_SIM
+  static_assert(kLaneW == 2, "vector accesses take pairs");
+  if (row_in && inside && A.v) {
+    const long long i = b + w0 + kLaneW * (long long)(threadIdx.x & 63);
+    load_pair(A.p, A.st, i, x[0], x[1]);
+    return;
+  }
+#endif
+  reg::sfor<kLaneW>([&](auto K) {
+    constexpr int k = decltype(K)::value;
+    x[k] = P::gen([&](int l) {
+      const long long pc = w0 + (long long)kLaneW * l + k;
+      return (row_in && pc >= 0 && pc < m) ? load_any(A.p, A.st, b + pc) : 0.0f;
+    });
+  });
+}
+
+// A.p[b + pc] = x[k] for the fine positions pc = w0 + kLaneW l + k in [lo, hi)
+// (a lane whose pair lies wholly in the range stores it as one access).
+template <class P>
+__device__ __forceinline__ void st_fine(const Arr& A, long long b, long long w0, long long lo, long long hi,
+                                        const typename P::V (&x)[kLaneW]) {
+#ifndef LFX_CPU_SIM
+  if (A.v) {
+    const long long pc = w0 + kLaneW * (long long)(threadIdx.x & 63);
+    if (pc >= lo && pc + 1 < hi) {
+      store_pair(A.p, A.st, b + pc, x[0], x[1]);
     } else {
-      constexpr int par = ST::y_is_d ? 1 : 0;
-      V acc;
-      if constexpr (ST::kind == kGeneric) {
-        acc = v;
-        reg::sfor<ST::n>([&](auto K) {
-          constexpr int k = decltype(K)::value;
-          acc = P::fma(cf[k], P::template from<2 * (ST::off + k) + 1 - 2 * par>(v), acc);
-        });
-      } else if constexpr (ST::kind == kRepeat) {
-        V t = P::template from<2 * ST::off + 1 - 2 * par>(v);
-        reg::sfor<ST::n - 1>([&](auto K) {
-          constexpr int k = decltype(K)::value + 1;
-          t = P::add(t, P::template from<2 * (ST::off + k) + 1 - 2 * par>(v));
-        });
-        acc = P::fma(cf[0], t, v);
-      } else {
-        const V x = P::template from<2 * ST::off + 1 - 2 * par>(v);
-        acc = ST::sub ? P::sub(v, x) : P::add(v, x);
-      }
-      if (masked) {
-        // updated lanes keep acc inside their band, 0 outside; others keep v
-        v = P::where([&](int l) { return (l & 1) == par; },
-                     P::where([&](int l) {
-                       const long long b = (w0 + l) >> 1;
-                       return b >= 0 && b < (par ? nd : ns);
-                     }, acc, P::gen([](int) { return 0.0f; })),
-                     v);
-      } else {
-        v = P::where([&](int l) { return (l & 1) == par; }, acc, v);
-      }
+      if (pc >= lo && pc < hi) store_any(A.p, A.st, b + pc, x[0]);
+      if (pc + 1 >= lo && pc + 1 < hi) store_any(A.p, A.st, b + pc + 1, x[1]);
     }
+    return;
+  }
+#endif
+  reg::sfor<kLaneW>([&](auto K) {
+    constexpr int k = decltype(K)::value;
+    P::each(x[k], [&](int l, float val) {
+      const long long pc = w0 + (long long)kLaneW * l + k;
+      if (pc >= lo && pc < hi) store_any(A.p, A.st, b + pc, val);
+    });
+  });
+}
+
+// x[k] of lane l = band index j0 + kLaneE l + k of the band at element offset
+// b of A (0 when !ok or outside [0, n)).
+template <class P>
+__device__ __forceinline__ void ld_band(const Arr& A, long long b, long long j0, long long n, bool ok,
+                                        typename P::V (&x)[kLaneE]) {
+  const bool all = ok && j0 >= 0 && j0 + 64 * kLaneE <= n;
+  reg::sfor<kLaneE>([&](auto K) {
+    constexpr int k = decltype(K)::value;
+    if (all) {
+      x[k] = P::gen([&](int l) { return load_any(A.p, A.st, b + j0 + (long long)kLaneE * l + k); });
+    } else {
+      x[k] = P::gen([&](int l) {
+        const long long j = j0 + (long long)kLaneE * l + k;
+        return (ok && j >= 0 && j < n) ? load_any(A.p, A.st, b + j) : 0.0f;
+      });
+    }
+  });
+}
+
+// A.p[b + j] = x[k] for the band indices j = j0 + kLaneE l + k in [lo, hi).
+template <class P>
+__device__ __forceinline__ void st_band(const Arr& A, long long b, long long j0, long long lo, long long hi,
+                                        const typename P::V (&x)[kLaneE]) {
+  const bool all = j0 >= lo && j0 + 64 * kLaneE <= hi;
+  reg::sfor<kLaneE>([&](auto K) {
+    constexpr int k = decltype(K)::value;
+    P::each(x[k], [&](int l, float val) {
+      const long long j = j0 + (long long)kLaneE * l + k;
+      if (all || (j >= lo && j < hi)) store_any(A.p, A.st, b + j, val);
+    });
+  });
+}
+
+// One level of the wavelet's steps (transform TF) along the register axis of
+// every column the lane holds: rows 2e / 2e + 1 of X are s[e] / d[e].
+template <class P, class W, int TF>
+__device__ __forceinline__ void rows_level(typename P::V (&X)[kRows][kLaneW], const float* cf, bool masked,
+                                           const typename P::M& ms, const typename P::M& md) {
+  using V = typename P::V;
+  constexpr int E = kRows / 2;
+  reg::sfor<kLaneW>([&](auto K) {
+    constexpr int k = decltype(K)::value;
+    V s[E], d[E];
+    reg::sfor<E>([&](auto I) {
+      constexpr int e = decltype(I)::value;
+      s[e] = X[2 * e][k];
+      d[e] = X[2 * e + 1][k];
+    });
+    reg::level<P, W, TF, E>(s, d, cf, masked, ms, md);
+    reg::sfor<E>([&](auto I) {
+      constexpr int e = decltype(I)::value;
+      X[2 * e][k] = s[e];
+      X[2 * e + 1][k] = d[e];
+    });
   });
 }
 
@@ -114,17 +210,13 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, const Span
   constexpr int NC = wt::NCoef<typename W::list, TF>::value;
   float cf[NC > 0 ? NC : 1];
   P::template coefs<NC>(prog->coef, cf);
-  V s[E], d[E];
+  // X[q][k]: row r.w0 + q, fine column c.w0 + kLaneW lane + k
+  V X[kRows][kLaneW];
+  const bool cin = c.w0 >= 0 && c.w0 + kCols <= c.m;
   reg::sfor<kRows>([&](auto R) {
     constexpr int q = decltype(R)::value;
     const long long pr = r.w0 + q;
-    const bool row_in = pr >= 0 && pr < r.m;
-    const V v = P::gen([&](int l) {
-      const long long pc = c.w0 + l;
-      return (row_in && pc >= 0 && pc < c.m) ? load_any(A0.p, A0.st, o0 + pr * A0.sr + pc) : 0.0f;
-    });
-    if constexpr (q % 2 == 0) s[q / 2] = v;
-    else d[q / 2] = v;
+    ld_fine<P>(A0, o0 + pr * A0.sr, c.w0, c.m, pr >= 0 && pr < r.m, cin, X[q]);
   });
   const bool mr = r.w0 < 0 || r.w0 + kRows > r.m;
   typename P::M ms{}, md{};
@@ -132,37 +224,38 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, const Span
     ms = P::umask(E, r.w0 / 2, r.ns());
     md = P::umask(E, r.w0 / 2, r.nd());
   }
-  reg::level<P, W, TF, E>(s, d, cf, mr, ms, md);
-  const bool mc = c.w0 < 0 || c.w0 + 64 > c.m;
-  // stores: row band index q = w0/2 + e. The s values (even lanes) and d
-  // values (odd lanes) of a row are first gathered to lanes 0..31 and 32..63,
-  // so each half of the wave stores one contiguous run: lane t holds column
-  // band index j = w0/2 + t % 32 of the s (t < 32) or d (t >= 32) half.
-  const long long rq0 = r.w0 / 2, cj0 = c.w0 / 2;
+  rows_level<P, W, TF>(X, cf, mr, ms, md);
+  // lane axis, row by row (a row is stored, and dead, once transformed): lane
+  // l holds column band indices j0 + kLaneE l + k of the s and d halves
+  const bool mc = !cin;
+  typename P::M msc{}, mdc{};
+  const long long j0 = c.w0 / 2, rq0 = r.w0 / 2;
+  if (mc) {
+    msc = P::band_mask(kLaneE, j0, c.ns());
+    mdc = P::band_mask(kLaneE, j0, c.nd());
+  }
   const long long nsr = r.ns(), nsc = c.ns();
-  auto store_row = [&](const V& v, long long q, bool srow, bool row_ok) {
-    if (!row_ok) return;
-    const V u = P::permute(v, [](int t) { return t < 32 ? 2 * t : 2 * (t - 32) + 1; });
-    P::each(u, [&](int t, float val) {
-      const long long j = cj0 + (t & 31);
-      const bool scol = t < 32;
-      if (j < c.lo() || j >= (scol ? c.hi_s() : c.hi_d())) return;
-      if (srow && scol) {
-        store_any(A2.p, A2.st, o2 + q * A2.sr + j, val);
-      } else {
-        const long long row = srow ? q : nsr + q, col = scol ? j : nsc + j;
-        store_any(A1.p, A1.st, o1 + row * A1.sr + col, val);
-      }
+  reg::sfor<kRows>([&](auto R) {
+    constexpr int q = decltype(R)::value;
+    V ls[kLaneE], ld[kLaneE];
+    reg::sfor<kLaneE>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      ls[k] = X[q][2 * k];
+      ld[k] = X[q][2 * k + 1];
     });
-  };
-  // each row: its lane-axis level, then its stores (the row is then dead)
-  reg::sfor<E>([&](auto K) {
-    constexpr int e = decltype(K)::value;
-    const long long q = rq0 + e;
-    lane_level<P, W, TF>(s[e], cf, mc, c.w0, c.ns(), c.nd());
-    store_row(s[e], q, true, q >= r.lo() && q < r.hi_s());
-    lane_level<P, W, TF>(d[e], cf, mc, c.w0, c.ns(), c.nd());
-    store_row(d[e], q, false, q >= r.lo() && q < r.hi_d());
+    reg::level<P, W, TF, kLaneE>(ls, ld, cf, mc, msc, mdc);
+    const long long rq = rq0 + q / 2;
+    if constexpr (q % 2 == 0) {
+      if (rq >= r.lo() && rq < r.hi_s()) {
+        st_band<P>(A2, o2 + rq * A2.sr, j0, c.lo(), c.hi_s(), ls);
+        st_band<P>(A1, o1 + rq * A1.sr + nsc, j0, c.lo(), c.hi_d(), ld);
+      }
+    } else {
+      if (rq >= r.lo() && rq < r.hi_d()) {
+        st_band<P>(A1, o1 + (nsr + rq) * A1.sr, j0, c.lo(), c.hi_s(), ls);
+        st_band<P>(A1, o1 + (nsr + rq) * A1.sr + nsc, j0, c.lo(), c.hi_d(), ld);
+      }
+    }
   });
 }
 
@@ -177,28 +270,30 @@ __device__ __forceinline__ void rev_tile(const DProgram<float>* prog, const Span
   constexpr int NC = wt::NCoef<typename W::list, TF>::value;
   float cf[NC > 0 ? NC : 1];
   P::template coefs<NC>(prog->coef, cf);
-  const long long rq0 = r.w0 / 2, cj0 = c.w0 / 2;
+  const long long rq0 = r.w0 / 2, j0 = c.w0 / 2;
   const long long nsr = r.ns(), ndr = r.nd(), nsc = c.ns(), ndc = c.nd();
-  // loads: lanes 0..31 read one contiguous run of the s half (column band
-  // index j = w0/2 + t), lanes 32..63 one of the d half; one gather then puts
-  // s_j on lane 2j - w0 and d_j on the lane after it
-  auto interleave = [](int l) { return (l & 1) ? 32 + (l >> 1) : (l >> 1); };
-  V s[E], d[E];
-  reg::sfor<E>([&](auto K) {
-    constexpr int e = decltype(K)::value;
+  // X[2e][2k] / X[2e][2k + 1]: s row rq0 + e, s / d column band index
+  // j0 + kLaneE lane + k; X[2e + 1][...]: the d row
+  V X[kRows][kLaneW];
+  reg::sfor<E>([&](auto I) {
+    constexpr int e = decltype(I)::value;
     const long long q = rq0 + e;
     const bool s_in = q >= 0 && q < nsr, d_in = q >= 0 && q < ndr;
-    s[e] = P::permute(P::gen([&](int t) {
-      const long long j = cj0 + (t & 31);
-      if (t < 32) return (s_in && j >= 0 && j < nsc) ? load_any(A2.p, A2.st, o2 + q * A2.sr + j) : 0.0f;
-      return (s_in && j >= 0 && j < ndc) ? load_any(A0.p, A0.st, o0 + q * A0.sr + (nsc + j)) : 0.0f;
-    }), interleave);
-    d[e] = P::permute(P::gen([&](int t) {
-      const long long j = cj0 + (t & 31);
-      const bool ok = d_in && j >= 0 && j < (t < 32 ? nsc : ndc);
-      const long long col = t < 32 ? j : nsc + j;
-      return ok ? load_any(A0.p, A0.st, o0 + (nsr + q) * A0.sr + col) : 0.0f;
-    }), interleave);
+    V a[kLaneE], b[kLaneE];
+    ld_band<P>(A2, o2 + q * A2.sr, j0, nsc, s_in, a);
+    ld_band<P>(A0, o0 + q * A0.sr + nsc, j0, ndc, s_in, b);
+    reg::sfor<kLaneE>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      X[2 * e][2 * k] = a[k];
+      X[2 * e][2 * k + 1] = b[k];
+    });
+    ld_band<P>(A0, o0 + (nsr + q) * A0.sr, j0, nsc, d_in, a);
+    ld_band<P>(A0, o0 + (nsr + q) * A0.sr + nsc, j0, ndc, d_in, b);
+    reg::sfor<kLaneE>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      X[2 * e + 1][2 * k] = a[k];
+      X[2 * e + 1][2 * k + 1] = b[k];
+    });
   });
   const bool mr = r.w0 < 0 || r.w0 + kRows > r.m;
   typename P::M ms{}, md{};
@@ -206,20 +301,35 @@ __device__ __forceinline__ void rev_tile(const DProgram<float>* prog, const Span
     ms = P::umask(E, rq0, nsr);
     md = P::umask(E, rq0, ndr);
   }
-  reg::level<P, W, TF, E>(s, d, cf, mr, ms, md);
-  const bool mc = c.w0 < 0 || c.w0 + 64 > c.m;
+  rows_level<P, W, TF>(X, cf, mr, ms, md);
+  // X[q] now holds fine row r.w0 + q
+  const bool mc = c.w0 < 0 || c.w0 + kCols > c.m;
+  typename P::M msc{}, mdc{};
+  if (mc) {
+    msc = P::band_mask(kLaneE, j0, nsc);
+    mdc = P::band_mask(kLaneE, j0, ndc);
+  }
   const long long cb = c.b >= c.m ? c.m : c.b;
   const long long rb = r.b >= r.m ? r.m : r.b;
   reg::sfor<kRows>([&](auto R) {
     constexpr int q = decltype(R)::value;
-    V v = (q % 2 == 0) ? s[q / 2] : d[q / 2];
-    lane_level<P, W, TF>(v, cf, mc, c.w0, nsc, ndc);
+    V ls[kLaneE], ld[kLaneE];
+    reg::sfor<kLaneE>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      ls[k] = X[q][2 * k];
+      ld[k] = X[q][2 * k + 1];
+    });
+    reg::level<P, W, TF, kLaneE>(ls, ld, cf, mc, msc, mdc);
     const long long pr = r.w0 + q;
-    if (pr >= r.a && pr < rb)
-      P::each(v, [&](int l, float val) {
-        const long long pc = c.w0 + l;
-        if (pc >= c.a && pc < cb) store_any(A1.p, A1.st, o1 + pr * A1.sr + pc, val);
+    if (pr >= r.a && pr < rb) {
+      V y[kLaneW];
+      reg::sfor<kLaneE>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        y[2 * k] = ls[k];
+        y[2 * k + 1] = ld[k];
       });
+      st_fine<P>(A1, o1 + pr * A1.sr, c.w0, c.a, cb, y);
+    }
   });
 }
 

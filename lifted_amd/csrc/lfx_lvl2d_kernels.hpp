@@ -1,6 +1,7 @@
-// Two-axis level kernel of one transform type (included once per transform
-// by lfx_lvl2d_tf{0,1,2,3}.hip, so the table's instantiations compile in
-// parallel).
+// Two-axis level kernels of one transform type (included once per transform
+// by lfx_lvl2d_tf{0,1,2,3}.hip, so the transforms compile in parallel): one
+// kernel per table wavelet (its compiled step shapes), chosen on the host, so
+// that each is compiled and register-allocated on its own.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -11,17 +12,23 @@
 namespace lfx {
 
 // One level of a two-axis level-by-level transform (lfx_lvl2d.hpp): one
-// wavefront per tile, any table wavelet (its id selects the compiled step
-// shapes; a wave-uniform branch).
-template <int TF>
+// wavefront per tile.
+template <int TF, class W>
 __global__ void __launch_bounds__(64) k_lvl2d(l2d::Level L) {
   const long long w = (long long)xcd_block(blockIdx.x, gridDim.x);
   if (w >= L.ntiles) return;
+  l2d::level_tile<reg::LaneGpu, W, TF>(L, w);
+}
+
+template <int TF>
+hipError_t launch_lvl2d_tf(const l2d::Level& L, hipStream_t st) {
+  if (L.ntiles <= 0) return hipSuccess;
+  if (L.ntiles > 0x7fffffffll) return hipErrorInvalidValue;
   switch (L.wavelet) {
-#define LFX_WAVELET(nm, idv, w_)                                 \
-  case idv:                                                     \
-    l2d::level_tile<reg::LaneGpu, wt::WL_##nm, TF>(L, w);       \
-    return;
+#define LFX_WAVELET(nm, idv, w_)                                                                 \
+  case idv:                                                                                     \
+    hipLaunchKernelGGL((k_lvl2d<TF, wt::WL_##nm>), dim3((unsigned)L.ntiles), dim3(64), 0, st, L); \
+    break;
 #define LFX_GENERIC(...)
 #define LFX_UNIT(...)
 #define LFX_REPEAT(...)
@@ -35,15 +42,8 @@ __global__ void __launch_bounds__(64) k_lvl2d(l2d::Level L) {
 #undef LFX_SCALE
 #undef LFX_END
     default:
-      return;
+      return hipErrorInvalidValue;
   }
-}
-
-template <int TF>
-hipError_t launch_lvl2d_tf(const l2d::Level& L, hipStream_t st) {
-  if (L.ntiles <= 0) return hipSuccess;
-  if (L.ntiles > 0x7fffffffll) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k_lvl2d<TF>), dim3((unsigned)L.ntiles), dim3(64), 0, st, L);
   return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@
 // 16-bit transform rounds every level's result once, like the other
 // level-by-level paths.
 #pragma once
+#include <cstdint>
 #include <vector>
 
 #include "lfx_lvl2d_types.hpp"
@@ -54,7 +55,7 @@ inline std::vector<l2d::Level> lvl2d_plan(const DProgram<float>& hp, bool fwd, i
   std::vector<long long> Hr(L), Hc(L);
   for (long long l = 0; l < L; ++l) {
     Hr[l] = lvl2d_axis_halo(hp, fwd, mr[l], l2d::kRows);
-    Hc[l] = lvl2d_axis_halo(hp, fwd, mc[l], 64);
+    Hc[l] = lvl2d_axis_halo(hp, fwd, mc[l], l2d::kCols);
     if (Hr[l] < 0 || Hc[l] < 0) return lv;
   }
   // batch dims: every other dim
@@ -67,6 +68,15 @@ inline std::vector<l2d::Level> lvl2d_plan(const DProgram<float>& hp, bool fwd, i
       base.nbatch *= shape[k];
       bdims.push_back(k);
     }
+  const long long es = storage == kF32 ? 4 : 2;
+  // vector accesses: the pointer and every stride but the lane axis's are
+  // multiples of kLaneW elements
+  auto vec_ok = [&](const void* p, const l2d::Arr& a) {
+    if (reinterpret_cast<uintptr_t>(p) % (uintptr_t)(l2d::kLaneW * es) != 0 || a.sr % l2d::kLaneW != 0) return 0;
+    for (int i = 0; i < base.nb; ++i)
+      if (a.bs[i] % l2d::kLaneW != 0) return 0;
+    return 1;
+  };
   auto user = [&](const void* p, const long long* str) {
     l2d::Arr a{};
     a.p = const_cast<void*>(p);
@@ -74,6 +84,7 @@ inline std::vector<l2d::Level> lvl2d_plan(const DProgram<float>& hp, bool fwd, i
     a.sr = str[ra];
     a.sc = str[ca];
     for (size_t i = 0; i < bdims.size(); ++i) a.bs[i] = str[bdims[i]];
+    a.v = vec_ok(p, a);
     return a;
   };
   auto dense = [&](void* p, long long rows, long long cols) {
@@ -87,6 +98,7 @@ inline std::vector<l2d::Level> lvl2d_plan(const DProgram<float>& hp, bool fwd, i
       a.bs[i] = acc;
       acc *= shape[bdims[i]];
     }
+    a.v = vec_ok(p, a);
     return a;
   };
   void* ws[2] = {ws0, ws1};
@@ -98,7 +110,7 @@ inline std::vector<l2d::Level> lvl2d_plan(const DProgram<float>& hp, bool fwd, i
     v.Hr = Hr[l];
     v.Hc = Hc[l];
     v.Or = l2d::kRows - 2 * Hr[l];
-    v.Oc = 64 - 2 * Hc[l];
+    v.Oc = l2d::kCols - 2 * Hc[l];
     v.Tr = (v.mr + v.Or - 1) / v.Or;
     v.Tc = (v.mc + v.Oc - 1) / v.Oc;
     v.ntiles = v.nbatch * v.Tr * v.Tc;

@@ -10,14 +10,23 @@ namespace lfx {
 namespace l2d {
 
 constexpr int kRows = 32;  // register-axis window (positions per lane)
+// Lane axis: each lane holds kLaneE values of the s band and kLaneE of the d
+// band, i.e. kLaneW consecutive fine positions; the wavefront's window is
+// kCols fine positions.
+constexpr int kLaneE = 1;
+constexpr int kLaneW = 2 * kLaneE;
+constexpr int kCols = 64 * kLaneW;
 
 // An array of one tile: base pointer and storage type, element strides of the
 // register axis (sr) and lane axis (sc; the kernels require 1: lanes are
-// contiguous), and of the batch dims.
+// contiguous), and of the batch dims. v: the base pointer and every stride
+// other than sc are multiples of kLaneW elements, so a lane's kLaneW
+// consecutive fine positions (starting at an even position) are one aligned
+// vector access.
 struct Arr {
   void* p;
   int32_t st;
-  int32_t pad;
+  int32_t v;
   long long sr, sc;
   long long bs[kMaxDims];
 };

@@ -10,6 +10,8 @@ when the plan is created (lfx_set_tuning_ex in the C ABI):
                      its register launch
     tile             tile length of LDS-tiled passes (float64 and other plans
                      without register tiles)
+    across           ND calls: strided float32 axes of at most 512 samples
+                     read in place by the across-line kernel (0 / 1)
 
 tune_batch() builds the plans of a signal list for every candidate setting,
 times them on the device with the caller's data, and returns the fastest.
@@ -22,12 +24,12 @@ import time
 
 from . import _lib
 
-FIELDS = ("tile", "levels_per_pass", "side_streams", "line_limit")
+FIELDS = ("tile", "levels_per_pass", "side_streams", "line_limit", "across")
 
 
 class LfxTuning(ctypes.Structure):
     _fields_ = [("tile", ctypes.c_int64), ("levels_per_pass", ctypes.c_int32), ("side_streams", ctypes.c_int32),
-                ("line_limit", ctypes.c_int64)]
+                ("line_limit", ctypes.c_int64), ("across", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 def _bind():
@@ -48,9 +50,9 @@ def get_tuning():
     return {f: getattr(t, f) for f in FIELDS}
 
 
-def set_tuning(tile=0, levels_per_pass=0, side_streams=-1, line_limit=0):
+def set_tuning(tile=0, levels_per_pass=0, side_streams=-1, line_limit=0, across=0):
     """Planning knobs for plans created afterwards on this thread."""
-    t = LfxTuning(int(tile), int(levels_per_pass), int(side_streams), int(line_limit))
+    t = LfxTuning(int(tile), int(levels_per_pass), int(side_streams), int(line_limit), int(across), 0)
     _lib.check(_bind().lfx_set_tuning_ex(ctypes.byref(t)))
 
 
@@ -58,7 +60,7 @@ def set_tuning(tile=0, levels_per_pass=0, side_streams=-1, line_limit=0):
 def tuning(**knobs):
     """Temporarily set planning knobs (unnamed ones keep their defaults)."""
     old = get_tuning()
-    cfg = {"tile": 0, "levels_per_pass": 0, "side_streams": -1, "line_limit": 0}
+    cfg = {"tile": 0, "levels_per_pass": 0, "side_streams": -1, "line_limit": 0, "across": 0}
     cfg.update(knobs)
     set_tuning(**cfg)
     try:

@@ -330,3 +330,42 @@ def test_two_axis_level_tiles_bf16(sim, oracle):
         gotf = torch.from_numpy(got.copy()).view(torch.bfloat16).float().numpy()
         ref = oracle_level_by_level_rounded(oracle, x, 100907, tf, 0, [1, 2], 3, "bfloat16")
         np.testing.assert_array_equal(gotf, ref, err_msg=f"tf={tf}")
+
+
+def _across_sim(w, tf, x, L, storage=0):
+    sim3 = os.path.join(HERE, "cpu_sim", "build", "sim_across")
+    B, N, C = x.shape
+    data = x.astype(np.float32).tobytes() if storage == 0 else x.tobytes()
+    r = subprocess.run([sim3, str(w), str(tf), str(storage), str(B), str(N), str(C), str(L)], input=data,
+                       capture_output=True)
+    if r.returncode == 3:
+        return None, r.stderr.decode()
+    assert r.returncode == 0, r.stderr.decode()[-3000:]
+    dt = np.float32 if storage == 0 else np.uint16
+    return np.frombuffer(r.stdout, dtype=dt).reshape(x.shape), r.stderr.decode()
+
+
+def test_across_lines_match_oracle(sim, oracle):
+    """The across-line kernel (lfx_across.hpp) on the 64-lane CPU model with a
+    model of its LDS slab (same padded layout), driven by the library's
+    planning: lines along a strided axis, blocks of 32 neighbouring lines
+    (partial last blocks), every pass of up to four levels in place in the
+    slab, all transforms, bit for bit against the oracle; lines too short for
+    the row tiles' passes are refused."""
+    g = np.random.default_rng(77)
+    cases = [(100907, (2, 512, 40), 4), (100907, (1, 300, 33), 6), (2, (3, 257, 70), 5), (10608, (1, 500, 31), 2),
+             (1, (2, 64, 64), 3), (105, (1, 129, 96), 3), (10301, (1, 90, 5), 1), (10, (2, 512, 17), 8)]
+    planned = 0
+    for w, shape, L in cases:
+        x = g.standard_normal(shape).astype(np.float32)
+        for tf in (0, 1, 2, 3):
+            got, info = _across_sim(w, tf, x, L)
+            if got is None:
+                continue
+            planned += 1
+            ref = oracle.lwt(x, w, tf, 0, axes=[1], level=[L])
+            np.testing.assert_array_equal(got, ref, err_msg=f"w={w} shape={shape} L={L} tf={tf}\n{info}")
+    assert planned >= 24
+    # a band too short for a pass's stencil: refused
+    got, info = _across_sim(10, 0, g.standard_normal((1, 40, 8)).astype(np.float32), 3)
+    assert got is None and "no plan" in info

@@ -500,3 +500,52 @@ def test_two_axis_level_by_level(gpu, oracle, storage):
         xi = x.clone()
         la.lwt(xi, 100907, tf, axes=[0, 1], level=3, out=xi)
         assert torch.equal(a, xi), f"{storage} in place tf={tf}"
+
+
+def test_across_strided_axes(gpu, oracle):
+    """With the across knob on, strided axes of at most 512 samples (Zero
+    boundary) run through the across-line kernel, which reads 32 neighbouring
+    lines per workgroup where they lie: single strided axes (partial blocks
+    of lines, several batch dims, row and column ranges of a larger array,
+    in place, more than four levels in two passes), separable and level-by-level transforms whose strided axes chain through
+    it, all transforms, bit for bit against the oracle; 16-bit storage rounded
+    once (single axis) against the float32 oracle on the 16-bit input."""
+    from lifted_amd import tune
+    with tune.tuning(across=1):
+        _across_cases(gpu, oracle)
+
+
+def _across_cases(gpu, oracle):
+    import torch
+    import lifted_amd as la
+    g = rng(512)
+    cases = [((3, 300, 45), [1], [6], 100907), ((512, 40), [0], [8], 2), ((5, 2, 130, 7), [2], [3], 10),
+             ((64, 96, 80), [0, 1, 2], [3, 3, 3], 100907), ((40, 50, 36), [1, 0], [2, 3], 10204),
+             ((96, 70, 33), [0, 2, 1], 3, 100907)]
+    for (shape, axes, lvl, w), tf in itertools.product(cases, (F, I, FA, IA)):
+        x = g.standard_normal(shape).astype(np.float32)
+        level = lvl if isinstance(lvl, int) else list(lvl)
+        ref = oracle.lwt(x, w, tf, 0, axes=axes, level=level)
+        got = la.lwt(_gpu(x, gpu), w, tf, axes=axes, level=level).cpu().numpy()
+        _check(got, ref, np.float32, f"shape={shape} axes={axes} level={level} w={w} tf={tf}", axes=axes,
+               level=level)
+    # views: a row range, a column range (row pitch above the line count), and in place
+    x = g.standard_normal((256, 70)).astype(np.float32)
+    xt = _gpu(x, gpu)
+    for tf in (F, FA):
+        for sl in (np.s_[10:200, :], np.s_[:, 3:50]):
+            v = x[sl]
+            ref = oracle.lwt(np.ascontiguousarray(v), 100907, tf, 0, axes=[0], level=[4])
+            got = la.lwt(xt[sl], 100907, tf, axes=[0], level=[4])
+            _check(got.cpu().numpy(), ref, np.float32, f"view {sl} tf={tf}")
+        y = xt.clone()
+        la.lwt(y, 100907, tf, axes=[0], level=[4], out=y)
+        _check(y.cpu().numpy(), oracle.lwt(x, 100907, tf, 0, axes=[0], level=[4]), np.float32, f"in place tf={tf}")
+    # 16-bit storage over one strided axis: read and written in 16 bits by the kernel
+    for storage, tf in itertools.product(("bfloat16", "float16"), (F, I, FA)):
+        tdt = getattr(torch, storage)
+        h = _gpu(g.standard_normal((4, 200, 90)).astype(np.float32), gpu).to(tdt)
+        ref = torch.from_numpy(oracle.lwt(h.float().cpu().numpy(), 100907, tf, 0, axes=[1],
+                                          level=[5])).to(tdt).float().numpy()
+        got = la.lwt(h, 100907, tf, axes=[1], level=[5]).float().cpu().numpy()
+        np.testing.assert_array_equal(got, ref, err_msg=f"{storage} tf={tf}")
