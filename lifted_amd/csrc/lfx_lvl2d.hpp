@@ -51,128 +51,100 @@ __device__ __forceinline__ void store_any(void* p, int st, long long i, float v)
   static_cast<float*>(p)[i] = v;
 }
 
+// Rows of one array as a tile sees them: the byte address of the tile
+// window's origin (row 0, column 0 of the window; it may lie before the
+// array, whose elements outside the band are never accessed) and the row
+// pitch in bytes. Every access is that wave-uniform origin plus a 32-bit byte
+// offset (the pitch times a compile-time row, plus the lane's column), so the
+// 64-bit index arithmetic runs once per tile and not once per access (the
+// host keeps a slice's byte offsets within 31 bits).
+struct Io {
+  char* org;
+  int rowb;  // bytes between rows
+  int es;    // element bytes
+  int st;    // storage type
+  int v;     // pairs are aligned (Arr::v)
+};
+__device__ __forceinline__ Io make_io(const Arr& A, long long o, long long row0, long long col0) {
+  Io x;
+  x.st = A.st;
+  x.es = A.st == kF32 ? 4 : 2;
+  x.v = A.v;
+  x.org = static_cast<char*>(A.p) + (o + row0 * A.sr + col0) * (long long)x.es;
+  x.rowb = (int)(A.sr * x.es);
+  return x;
+}
+__device__ __forceinline__ float ldx(const Io& a, unsigned off) {
+  const char* q = a.org + off;
+  if (a.st == kF32) return *reinterpret_cast<const float*>(q);
+  const uint32_t h = *reinterpret_cast<const uint16_t*>(q);
+  if (a.st == kBF16) return __uint_as_float(h << 16);
 #ifndef LFX_CPU_SIM
-// kLaneW (= 2) consecutive elements at i (even, array aligned) as one access
-__device__ __forceinline__ void load_pair(const void* p, int st, long long i, float& a, float& b) {
-  if (st == kF32) {
-    typedef float f2 __attribute__((ext_vector_type(2)));
-    const f2 v = *reinterpret_cast<const f2*>(static_cast<const float*>(p) + i);
-    a = v.x;
-    b = v.y;
-  } else if (st == kBF16) {
-    const uint32_t u = *reinterpret_cast<const uint32_t*>(static_cast<const uint16_t*>(p) + i);
-    a = __uint_as_float(u << 16);
-    b = __uint_as_float(u & 0xffff0000u);
+  _Float16 f;
+  const uint16_t hh = (uint16_t)h;
+  __builtin_memcpy(&f, &hh, 2);
+  return (float)f;
+#else
+  return 0.0f;
+#endif
+}
+__device__ __forceinline__ void stx(const Io& a, unsigned off, float v) {
+  char* q = a.org + off;
+  if (a.st == kF32) {
+    *reinterpret_cast<float*>(q) = v;
+  } else if (a.st == kBF16) {
+    dev::store_as<dev::bf16_t, float>(reinterpret_cast<dev::bf16_t*>(q), 0, v);
   } else {
-    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
-    const h2 v = *reinterpret_cast<const h2*>(static_cast<const uint16_t*>(p) + i);
-    a = (float)v.x;
-    b = (float)v.y;
+#ifndef LFX_CPU_SIM
+    dev::store_as<dev::f16_t, float>(reinterpret_cast<dev::f16_t*>(q), 0, v);
+#endif
   }
 }
-__device__ __forceinline__ void store_pair(void* p, int st, long long i, float a, float b) {
-  if (st == kF32) {
+#ifndef LFX_CPU_SIM
+// two consecutive elements (pair-aligned) as one access
+__device__ __forceinline__ void ld2x(const Io& a, unsigned off, float& x0, float& x1) {
+  const char* q = a.org + off;
+  if (a.st == kF32) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    const f2 v = *reinterpret_cast<const f2*>(q);
+    x0 = v.x;
+    x1 = v.y;
+  } else if (a.st == kBF16) {
+    const uint32_t u = *reinterpret_cast<const uint32_t*>(q);
+    x0 = __uint_as_float(u << 16);
+    x1 = __uint_as_float(u & 0xffff0000u);
+  } else {
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    const h2 v = *reinterpret_cast<const h2*>(q);
+    x0 = (float)v.x;
+    x1 = (float)v.y;
+  }
+}
+__device__ __forceinline__ void st2x(const Io& a, unsigned off, float x0, float x1) {
+  char* q = a.org + off;
+  if (a.st == kF32) {
     typedef float f2 __attribute__((ext_vector_type(2)));
     f2 v;
-    v.x = a;
-    v.y = b;
-    *reinterpret_cast<f2*>(static_cast<float*>(p) + i) = v;
-  } else if (st == kBF16) {
+    v.x = x0;
+    v.y = x1;
+    *reinterpret_cast<f2*>(q) = v;
+  } else if (a.st == kBF16) {
     dev::bf16_t h[2];
-    dev::store_as<dev::bf16_t, float>(h, 0, a);
-    dev::store_as<dev::bf16_t, float>(h, 1, b);
-    *reinterpret_cast<uint32_t*>(static_cast<uint16_t*>(p) + i) = (uint32_t)h[0].v | ((uint32_t)h[1].v << 16);
+    dev::store_as<dev::bf16_t, float>(h, 0, x0);
+    dev::store_as<dev::bf16_t, float>(h, 1, x1);
+    *reinterpret_cast<uint32_t*>(q) = (uint32_t)h[0].v | ((uint32_t)h[1].v << 16);
   } else {
     typedef _Float16 h2 __attribute__((ext_vector_type(2)));
     h2 v;
-    v.x = (_Float16)a;
-    v.y = (_Float16)b;
-    *reinterpret_cast<h2*>(static_cast<uint16_t*>(p) + i) = v;
+    v.x = (_Float16)x0;
+    v.y = (_Float16)x1;
+    *reinterpret_cast<h2*>(q) = v;
   }
 }
 #endif
 
-// x[k] of lane l = fine position w0 + kLaneW l + k of the row at element
-// offset b of A (0 when !row_in or outside [0, m)); `inside`: the whole window
-// lies in [0, m) (wave-uniform).
-template <class P>
-__device__ __forceinline__ void ld_fine(const Arr& A, long long b, long long w0, long long m, bool row_in, bool inside,
-                                        typename P::V (&x)[kLaneW]) {
-#ifndef LFX_CPU_SIM
-  static_assert(kLaneW == 2, "vector accesses take pairs");
-  if (row_in && inside && A.v) {
-    const long long i = b + w0 + kLaneW * (long long)(threadIdx.x & 63);
-    load_pair(A.p, A.st, i, x[0], x[1]);
-    return;
-  }
-#endif
-  reg::sfor<kLaneW>([&](auto K) {
-    constexpr int k = decltype(K)::value;
-    x[k] = P::gen([&](int l) {
-      const long long pc = w0 + (long long)kLaneW * l + k;
-      return (row_in && pc >= 0 && pc < m) ? load_any(A.p, A.st, b + pc) : 0.0f;
-    });
-  });
-}
-
-// A.p[b + pc] = x[k] for the fine positions pc = w0 + kLaneW l + k in [lo, hi)
-// (a lane whose pair lies wholly in the range stores it as one access).
-template <class P>
-__device__ __forceinline__ void st_fine(const Arr& A, long long b, long long w0, long long lo, long long hi,
-                                        const typename P::V (&x)[kLaneW]) {
-#ifndef LFX_CPU_SIM
-  if (A.v) {
-    const long long pc = w0 + kLaneW * (long long)(threadIdx.x & 63);
-    if (pc >= lo && pc + 1 < hi) {
-      store_pair(A.p, A.st, b + pc, x[0], x[1]);
-    } else {
-      if (pc >= lo && pc < hi) store_any(A.p, A.st, b + pc, x[0]);
-      if (pc + 1 >= lo && pc + 1 < hi) store_any(A.p, A.st, b + pc + 1, x[1]);
-    }
-    return;
-  }
-#endif
-  reg::sfor<kLaneW>([&](auto K) {
-    constexpr int k = decltype(K)::value;
-    P::each(x[k], [&](int l, float val) {
-      const long long pc = w0 + (long long)kLaneW * l + k;
-      if (pc >= lo && pc < hi) store_any(A.p, A.st, b + pc, val);
-    });
-  });
-}
-
-// x[k] of lane l = band index j0 + kLaneE l + k of the band at element offset
-// b of A (0 when !ok or outside [0, n)).
-template <class P>
-__device__ __forceinline__ void ld_band(const Arr& A, long long b, long long j0, long long n, bool ok,
-                                        typename P::V (&x)[kLaneE]) {
-  const bool all = ok && j0 >= 0 && j0 + 64 * kLaneE <= n;
-  reg::sfor<kLaneE>([&](auto K) {
-    constexpr int k = decltype(K)::value;
-    if (all) {
-      x[k] = P::gen([&](int l) { return load_any(A.p, A.st, b + j0 + (long long)kLaneE * l + k); });
-    } else {
-      x[k] = P::gen([&](int l) {
-        const long long j = j0 + (long long)kLaneE * l + k;
-        return (ok && j >= 0 && j < n) ? load_any(A.p, A.st, b + j) : 0.0f;
-      });
-    }
-  });
-}
-
-// A.p[b + j] = x[k] for the band indices j = j0 + kLaneE l + k in [lo, hi).
-template <class P>
-__device__ __forceinline__ void st_band(const Arr& A, long long b, long long j0, long long lo, long long hi,
-                                        const typename P::V (&x)[kLaneE]) {
-  const bool all = j0 >= lo && j0 + 64 * kLaneE <= hi;
-  reg::sfor<kLaneE>([&](auto K) {
-    constexpr int k = decltype(K)::value;
-    P::each(x[k], [&](int l, float val) {
-      const long long j = j0 + (long long)kLaneE * l + k;
-      if (all || (j >= lo && j < hi)) store_any(A.p, A.st, b + j, val);
-    });
-  });
-}
+// v clamped to [0, cap] (window-relative bounds of a range)
+LFX_HD int wclamp(long long v, int cap) { return v < 0 ? 0 : (v > cap ? cap : (int)v); }
 
 // One level of the wavelet's steps (transform TF) along the register axis of
 // every column the lane holds: rows 2e / 2e + 1 of X are s[e] / d[e].
@@ -199,26 +171,43 @@ __device__ __forceinline__ void rows_level(typename P::V (&X)[kRows][kLaneW], co
 }
 
 // Forward-stepping level of one tile (register axis first, then lane axis).
-// in: a[0] at (row, col) offsets (ir, ic) of the sub-box; details to a[1]
-// (rows s | d at [0, ns_r) / [ns_r, m_r), same for columns), the ss quadrant
-// to a[2] at its (s row, s col).
+// in: a[0] (the sub-box); details to a[1] (rows s | d at [0, ns_r) /
+// [ns_r, m_r), same for columns), the ss quadrant to a[2] at its (s row,
+// s col). o0..o2: the slice's element offsets.
 template <class P, class W, int TF>
 __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, const Span& r, const Span& c, const Arr& A0,
                                          long long o0, const Arr& A1, long long o1, const Arr& A2, long long o2) {
   using V = typename P::V;
   constexpr int E = kRows / 2;
+  constexpr int NJ = 64 * kLaneE;
   constexpr int NC = wt::NCoef<typename W::list, TF>::value;
   float cf[NC > 0 ? NC : 1];
   P::template coefs<NC>(prog->coef, cf);
-  // X[q][k]: row r.w0 + q, fine column c.w0 + kLaneW lane + k
+  // X[q][k]: row r.w0 + q, fine column c.w0 + kLaneW lane + k; in-band rows
+  // [rl, rh) and columns [cl, ch) of the window
+  const int rl = wclamp(-r.w0, kRows), rh = wclamp(r.m - r.w0, kRows);
+  const int cl = wclamp(-c.w0, kCols), ch = wclamp(c.m - c.w0, kCols);
+  const bool cin = cl == 0 && ch == kCols;
+  const Io in = make_io(A0, o0, r.w0, c.w0);
   V X[kRows][kLaneW];
-  const bool cin = c.w0 >= 0 && c.w0 + kCols <= c.m;
   reg::sfor<kRows>([&](auto R) {
     constexpr int q = decltype(R)::value;
-    const long long pr = r.w0 + q;
-    ld_fine<P>(A0, o0 + pr * A0.sr, c.w0, c.m, pr >= 0 && pr < r.m, cin, X[q]);
+    const bool row_in = q >= rl && q < rh;
+#ifndef LFX_CPU_SIM
+    if (row_in && cin && in.v) {
+      ld2x(in, (unsigned)(q * in.rowb + kLaneW * in.es * (int)(threadIdx.x & 63)), X[q][0], X[q][1]);
+      return;
+    }
+#endif
+    reg::sfor<kLaneW>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      X[q][k] = P::gen([&](int l) {
+        const int pc = kLaneW * l + k;
+        return (row_in && pc >= cl && pc < ch) ? ldx(in, (unsigned)(q * in.rowb + pc * in.es)) : 0.0f;
+      });
+    });
   });
-  const bool mr = r.w0 < 0 || r.w0 + kRows > r.m;
+  const bool mr = rl > 0 || rh < kRows;
   typename P::M ms{}, md{};
   if (mr) {
     ms = P::umask(E, r.w0 / 2, r.ns());
@@ -235,8 +224,24 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, const Span
     mdc = P::band_mask(kLaneE, j0, c.nd());
   }
   const long long nsr = r.ns(), nsc = c.ns();
+  // owned window-relative ranges: band rows [el, esh) / [el, edh), band
+  // columns [jl, jsh) / [jl, jdh)
+  const int el = wclamp(r.lo() - rq0, E), esh = wclamp(r.hi_s() - rq0, E), edh = wclamp(r.hi_d() - rq0, E);
+  const int jl = wclamp(c.lo() - j0, NJ), jsh = wclamp(c.hi_s() - j0, NJ), jdh = wclamp(c.hi_d() - j0, NJ);
+  const Io ss = make_io(A2, o2, rq0, j0), sd = make_io(A1, o1, rq0, nsc + j0);
+  const Io ds = make_io(A1, o1, nsr + rq0, j0), dd = make_io(A1, o1, nsr + rq0, nsc + j0);
+  auto put = [&](const Io& a, int e, int hi, const V (&x)[kLaneE]) {
+    reg::sfor<kLaneE>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      P::each(x[k], [&](int l, float val) {
+        const int jj = kLaneE * l + k;
+        if (jj >= jl && jj < hi) stx(a, (unsigned)(e * a.rowb + jj * a.es), val);
+      });
+    });
+  };
   reg::sfor<kRows>([&](auto R) {
     constexpr int q = decltype(R)::value;
+    constexpr int e = q / 2;
     V ls[kLaneE], ld[kLaneE];
     reg::sfor<kLaneE>([&](auto K) {
       constexpr int k = decltype(K)::value;
@@ -244,16 +249,15 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, const Span
       ld[k] = X[q][2 * k + 1];
     });
     reg::level<P, W, TF, kLaneE>(ls, ld, cf, mc, msc, mdc);
-    const long long rq = rq0 + q / 2;
     if constexpr (q % 2 == 0) {
-      if (rq >= r.lo() && rq < r.hi_s()) {
-        st_band<P>(A2, o2 + rq * A2.sr, j0, c.lo(), c.hi_s(), ls);
-        st_band<P>(A1, o1 + rq * A1.sr + nsc, j0, c.lo(), c.hi_d(), ld);
+      if (e >= el && e < esh) {
+        put(ss, e, jsh, ls);
+        put(sd, e, jdh, ld);
       }
     } else {
-      if (rq >= r.lo() && rq < r.hi_d()) {
-        st_band<P>(A1, o1 + (nsr + rq) * A1.sr, j0, c.lo(), c.hi_s(), ls);
-        st_band<P>(A1, o1 + (nsr + rq) * A1.sr + nsc, j0, c.lo(), c.hi_d(), ld);
+      if (e >= el && e < edh) {
+        put(ds, e, jsh, ls);
+        put(dd, e, jdh, ld);
       }
     }
   });
@@ -267,35 +271,50 @@ __device__ __forceinline__ void rev_tile(const DProgram<float>* prog, const Span
                                          long long o0, const Arr& A1, long long o1, const Arr& A2, long long o2) {
   using V = typename P::V;
   constexpr int E = kRows / 2;
+  constexpr int NJ = 64 * kLaneE;
   constexpr int NC = wt::NCoef<typename W::list, TF>::value;
   float cf[NC > 0 ? NC : 1];
   P::template coefs<NC>(prog->coef, cf);
   const long long rq0 = r.w0 / 2, j0 = c.w0 / 2;
   const long long nsr = r.ns(), ndr = r.nd(), nsc = c.ns(), ndc = c.nd();
+  // in-band window-relative ranges: s rows [el, esh), d rows [el, edh); s
+  // columns [jl, jsh), d columns [jl, jdh)
+  const int el = wclamp(-rq0, E), esh = wclamp(nsr - rq0, E), edh = wclamp(ndr - rq0, E);
+  const int jl = wclamp(-j0, NJ), jsh = wclamp(nsc - j0, NJ), jdh = wclamp(ndc - j0, NJ);
+  const Io ss = make_io(A2, o2, rq0, j0), sd = make_io(A0, o0, rq0, nsc + j0);
+  const Io ds = make_io(A0, o0, nsr + rq0, j0), dd = make_io(A0, o0, nsr + rq0, nsc + j0);
+  auto get = [&](const Io& a, int e, bool row_ok, int hi, V (&x)[kLaneE]) {
+    reg::sfor<kLaneE>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      x[k] = P::gen([&](int l) {
+        const int jj = kLaneE * l + k;
+        return (row_ok && jj >= jl && jj < hi) ? ldx(a, (unsigned)(e * a.rowb + jj * a.es)) : 0.0f;
+      });
+    });
+  };
   // X[2e][2k] / X[2e][2k + 1]: s row rq0 + e, s / d column band index
   // j0 + kLaneE lane + k; X[2e + 1][...]: the d row
   V X[kRows][kLaneW];
   reg::sfor<E>([&](auto I) {
     constexpr int e = decltype(I)::value;
-    const long long q = rq0 + e;
-    const bool s_in = q >= 0 && q < nsr, d_in = q >= 0 && q < ndr;
+    const bool s_in = e >= el && e < esh, d_in = e >= el && e < edh;
     V a[kLaneE], b[kLaneE];
-    ld_band<P>(A2, o2 + q * A2.sr, j0, nsc, s_in, a);
-    ld_band<P>(A0, o0 + q * A0.sr + nsc, j0, ndc, s_in, b);
+    get(ss, e, s_in, jsh, a);
+    get(sd, e, s_in, jdh, b);
     reg::sfor<kLaneE>([&](auto K) {
       constexpr int k = decltype(K)::value;
       X[2 * e][2 * k] = a[k];
       X[2 * e][2 * k + 1] = b[k];
     });
-    ld_band<P>(A0, o0 + (nsr + q) * A0.sr, j0, nsc, d_in, a);
-    ld_band<P>(A0, o0 + (nsr + q) * A0.sr + nsc, j0, ndc, d_in, b);
+    get(ds, e, d_in, jsh, a);
+    get(dd, e, d_in, jdh, b);
     reg::sfor<kLaneE>([&](auto K) {
       constexpr int k = decltype(K)::value;
       X[2 * e + 1][2 * k] = a[k];
       X[2 * e + 1][2 * k + 1] = b[k];
     });
   });
-  const bool mr = r.w0 < 0 || r.w0 + kRows > r.m;
+  const bool mr = el > 0 || esh < E || edh < E;
   typename P::M ms{}, md{};
   if (mr) {
     ms = P::umask(E, rq0, nsr);
@@ -303,14 +322,16 @@ __device__ __forceinline__ void rev_tile(const DProgram<float>* prog, const Span
   }
   rows_level<P, W, TF>(X, cf, mr, ms, md);
   // X[q] now holds fine row r.w0 + q
-  const bool mc = c.w0 < 0 || c.w0 + kCols > c.m;
+  const bool mc = jl > 0 || jsh < NJ || jdh < NJ;
   typename P::M msc{}, mdc{};
   if (mc) {
     msc = P::band_mask(kLaneE, j0, nsc);
     mdc = P::band_mask(kLaneE, j0, ndc);
   }
-  const long long cb = c.b >= c.m ? c.m : c.b;
-  const long long rb = r.b >= r.m ? r.m : r.b;
+  // owned window-relative fine rows [ra, rb) and columns [ca, cb)
+  const int ra = wclamp(r.a - r.w0, kRows), rb = wclamp((r.b >= r.m ? r.m : r.b) - r.w0, kRows);
+  const int ca = wclamp(c.a - c.w0, kCols), cb = wclamp((c.b >= c.m ? c.m : c.b) - c.w0, kCols);
+  const Io out = make_io(A1, o1, r.w0, c.w0);
   reg::sfor<kRows>([&](auto R) {
     constexpr int q = decltype(R)::value;
     V ls[kLaneE], ld[kLaneE];
@@ -320,46 +341,55 @@ __device__ __forceinline__ void rev_tile(const DProgram<float>* prog, const Span
       ld[k] = X[q][2 * k + 1];
     });
     reg::level<P, W, TF, kLaneE>(ls, ld, cf, mc, msc, mdc);
-    const long long pr = r.w0 + q;
-    if (pr >= r.a && pr < rb) {
+    if (q >= ra && q < rb) {
       V y[kLaneW];
       reg::sfor<kLaneE>([&](auto K) {
         constexpr int k = decltype(K)::value;
         y[2 * k] = ls[k];
         y[2 * k + 1] = ld[k];
       });
-      st_fine<P>(A1, o1 + pr * A1.sr, c.w0, c.a, cb, y);
+#ifndef LFX_CPU_SIM
+      const int pc = kLaneW * (int)(threadIdx.x & 63);
+      if (out.v && pc >= ca && pc + 1 < cb) {
+        st2x(out, (unsigned)(q * out.rowb + pc * out.es), y[0], y[1]);
+        return;
+      }
+#endif
+      reg::sfor<kLaneW>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        P::each(y[k], [&](int l, float val) {
+          const int pc2 = kLaneW * l + k;
+          if (pc2 >= ca && pc2 < cb) stx(out, (unsigned)(q * out.rowb + pc2 * out.es), val);
+        });
+      });
     }
   });
 }
 
-// Offsets of batch slice b in the three arrays (the loop over dims is
-// unrolled so that the launch arguments are only read at constant offsets and
-// stay out of private memory).
-__device__ __forceinline__ void slice_offsets(const Level& L, long long b, long long* o) {
-  o[0] = o[1] = o[2] = 0;
+// Tile w of a level (slice-major, then register-axis tiles, then lane-axis
+// tiles), its slice's element offsets in the three arrays from the batch
+// dims (fast 32-bit divisions: the launch has fewer than 2^31 tiles).
+template <class P, class W, int TF>
+__device__ __forceinline__ void level_tile(const Level& L, long long w) {
+  const uint32_t wi = (uint32_t)w;
+  uint32_t b = fdiv(L.f_per, wi);
+  const uint32_t t = wi - b * (uint32_t)L.f_per.d;
+  const uint32_t tr = fdiv(L.f_tc, t), tc = t - tr * (uint32_t)L.f_tc.d;
+  const Span r = span_of(L.mr, L.Hr, L.Or, tr);
+  const Span c = span_of(L.mc, L.Hc, L.Oc, tc);
+  long long o[3] = {0, 0, 0};
 #pragma unroll
   for (int kk = 0; kk < kMaxDims; ++kk) {
     const int k = kMaxDims - 1 - kk;
     if (k < L.nb) {
-      const long long q = b % L.bshape[k];
-      b /= L.bshape[k];
+      const uint32_t nb = fdiv(L.f_b[k], b);
+      const long long q = (long long)(b - nb * (uint32_t)L.f_b[k].d);
+      b = nb;
       o[0] += q * L.a[0].bs[k];
       o[1] += q * L.a[1].bs[k];
       o[2] += q * L.a[2].bs[k];
     }
   }
-}
-
-// Tile w of a level (slice-major, then register-axis tiles, then lane-axis tiles).
-template <class P, class W, int TF>
-__device__ __forceinline__ void level_tile(const Level& L, long long w) {
-  const long long per = L.Tr * L.Tc;
-  const long long b = w / per, t = w % per;
-  const Span r = span_of(L.mr, L.Hr, L.Or, t / L.Tc);
-  const Span c = span_of(L.mc, L.Hc, L.Oc, t % L.Tc);
-  long long o[3];
-  slice_offsets(L, b, o);
   const DProgram<float>* prog = static_cast<const DProgram<float>*>(L.prog);
   if constexpr (TF == kForward || TF == kInverseAdjoint)
     fwd_tile<P, W, TF>(prog, r, c, L.a[0], o[0], L.a[1], o[1], L.a[2], o[2]);

@@ -51,6 +51,14 @@ inline std::vector<l2d::Level> lvl2d_plan(const DProgram<float>& hp, bool fwd, i
                                           const void* prog, uint32_t wavelet) {
   std::vector<l2d::Level> lv;
   if (sin[ca] != 1 || sout[ca] != 1) return lv;  // lanes run along a unit-stride axis
+  // tiles address their rows as 32-bit byte offsets from a window origin:
+  // positive row strides, and a window's span within 31 bits
+  if (sin[ra] <= 0 || sout[ra] <= 0) return lv;
+  {
+    const long long es_ = storage == kF32 ? 4 : 2;
+    for (const long long sr : {sin[ra], sout[ra], shape[ca]})
+      if ((2 * l2d::kRows * sr + 2 * l2d::kCols) * es_ >= (1ll << 31)) return lv;
+  }
   const std::vector<long long> mr = level_lengths(shape[ra], L), mc = level_lengths(shape[ca], L);
   std::vector<long long> Hr(L), Hc(L);
   for (long long l = 0; l < L; ++l) {
@@ -114,6 +122,10 @@ inline std::vector<l2d::Level> lvl2d_plan(const DProgram<float>& hp, bool fwd, i
     v.Tr = (v.mr + v.Or - 1) / v.Or;
     v.Tc = (v.mc + v.Oc - 1) / v.Oc;
     v.ntiles = v.nbatch * v.Tr * v.Tc;
+    if (v.ntiles >= (1ll << 31)) return std::vector<l2d::Level>();
+    v.f_tc = l2d::make_fdiv((uint32_t)v.Tc);
+    v.f_per = l2d::make_fdiv((uint32_t)(v.Tr * v.Tc));
+    for (int k = 0; k < v.nb; ++k) v.f_b[k] = l2d::make_fdiv((uint32_t)v.bshape[k]);
     v.prog = prog;
     v.wavelet = wavelet;
     const long long nsr = v.mr - v.mr / 2, nsc = v.mc - v.mc / 2;

@@ -31,6 +31,23 @@ struct Arr {
   long long bs[kMaxDims];
 };
 
+// Division of 31-bit unsigned values by an invariant d >= 1 as a multiply-
+// high and a shift (round-up method): q = (umulhi(n, m) + n) >> l with
+// l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1; exact for n < 2^31.
+struct FDiv {
+  uint32_t d, m, l, pad;
+};
+inline FDiv make_fdiv(uint32_t d) {
+  uint32_t l = 0;
+  while ((1ull << l) < d) ++l;
+  const uint64_t m = ((1ull << 32) * ((1ull << l) - d)) / d + 1;
+  return FDiv{d, (uint32_t)m, l, 0};
+}
+LFX_HD uint32_t fdiv(const FDiv& f, uint32_t n) {
+  const uint32_t hi = (uint32_t)(((uint64_t)n * f.m) >> 32);
+  return (hi + n) >> f.l;
+}
+
 // One level's launch: arrays 0 (input: the sub-box, or the details for a
 // reverse level), 1 (output: the details, or the sub-box) and 2 (the ss
 // quadrant: written by a forward level, read by a reverse one).
@@ -48,6 +65,8 @@ struct Level {
   const void* prog;  // DProgram<float>
   uint32_t wavelet;
   int32_t pad2;
+  // tile index decomposition (ntiles < 2^31): w = (b Tr + tr) Tc + tc
+  FDiv f_tc, f_per, f_b[kMaxDims];
 };
 
 // Geometry of one tile along one axis: window start w0 (even, may be
