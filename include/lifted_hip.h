@@ -224,7 +224,8 @@ typedef struct {
   int32_t side_streams;    /* side streams of a batched plan (0..8), -1 = default (3) */
   int64_t line_limit;      /* register-tiled plans: longest band handed to the whole-line kernels, 0 = default */
   int32_t across;          /* ND calls: strided float32 axes of at most 512 samples (Zero boundary) read in
-                              place by the across-line kernel instead of transposed; 0 = off (default), 1 = on */
+                              place by the across-line kernel instead of transposed; 0 = off, 1 = on,
+                              -1 = default (on) */
   int32_t pad;
 } lfx_tuning;
 int lfx_set_tuning_ex(const lfx_tuning* t);

@@ -23,27 +23,26 @@ COMMON = [
 # device code: explicit fma builtins only (contraction off), no packed fp32 ops
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
+# (source, device code, object name, defines): the kernel families compile once
+# per transform type (LFX_TF; the two-axis level kernels also per storage
+# type, LFX_ST), so that their wavelet instantiations build in parallel
 SOURCES = [
-    ("lfx_kernels.hip", True),
-    ("lfx_noalloc.hip", True),
-    ("lfx_reg_tf0.hip", True),
-    ("lfx_reg_tf1.hip", True),
-    ("lfx_reg_tf2.hip", True),
-    ("lfx_reg_tf3.hip", True),
-    ("lfx_rows_tf0.hip", True),
-    ("lfx_rows_tf1.hip", True),
-    ("lfx_rows_tf2.hip", True),
-    ("lfx_rows_tf3.hip", True),
-    ("lfx_lvl2d_tf0.hip", True),
-    ("lfx_lvl2d_tf1.hip", True),
-    ("lfx_lvl2d_tf2.hip", True),
-    ("lfx_lvl2d_tf3.hip", True),
-    ("lfx_across_tf0.hip", True),
-    ("lfx_across_tf1.hip", True),
-    ("lfx_across_tf2.hip", True),
-    ("lfx_across_tf3.hip", True),
-    ("lfx_api.cpp", False),
-    ("lfx_table.cpp", False),
+    ("lfx_kernels.hip", True, "lfx_kernels", []),
+    ("lfx_noalloc.hip", True, "lfx_noalloc", []),
+]
+for _tf in range(4):
+    SOURCES += [
+        ("lfx_reg_kernels.hip", True, f"lfx_reg_tf{_tf}", [f"-DLFX_TF={_tf}", "-DLFX_ROWS=0"]),
+        ("lfx_reg_kernels.hip", True, f"lfx_rows_tf{_tf}", [f"-DLFX_TF={_tf}", "-DLFX_ROWS=1"]),
+        ("lfx_across_kernels.hip", True, f"lfx_across_tf{_tf}", [f"-DLFX_TF={_tf}"]),
+    ]
+    SOURCES += [
+        ("lfx_lvl2d_kernels.hip", True, f"lfx_lvl2d_tf{_tf}_st{_st}", [f"-DLFX_TF={_tf}", f"-DLFX_ST={_st}"])
+        for _st in (0, 2, 3)  # float32, bfloat16, float16 storage
+    ]
+SOURCES += [
+    ("lfx_api.cpp", False, "lfx_api", []),
+    ("lfx_table.cpp", False, "lfx_table", []),
 ]
 
 
@@ -74,13 +73,13 @@ def build(verbose=False, incremental=False):
     hipcc = _hipcc()
     os.makedirs(OBJDIR, exist_ok=True)
     objs, cmds = [], []
-    for src, device in SOURCES:
-        base = os.path.join(OBJDIR, os.path.splitext(src)[0])
+    for src, device, name, defines in SOURCES:
+        base = os.path.join(OBJDIR, name)
         obj, dep = base + ".o", base + ".d"
         objs.append(obj)
         if incremental and _up_to_date(obj, dep):
             continue
-        cmd = [hipcc, "-c"] + COMMON
+        cmd = [hipcc, "-c"] + COMMON + defines
         cmd += DEVICE if device else ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
         # dependency list for incremental builds
         cmd += ["-MMD", "-MF", dep]

@@ -16,7 +16,7 @@
 // so the position-by-position loads and stores are conflict-free too.
 #pragma once
 #include "lfx_across_types.hpp"
-#include "lfx_lvl2d.hpp"
+#include "lfx_reg.hpp"
 
 namespace lfx {
 namespace acr {
@@ -45,34 +45,93 @@ LFX_HDI void block_origin(const Args& A, long long blk, long long& c0, long long
 //
 // fetch: v <- the block's lines (zeros past the dim's end or the line's end);
 // the loads are only issued here, so a caller can overlap them with work on
-// the previous block. put: slab <- v.
-__device__ __forceinline__ void fetch(const Args& A, long long c0, long long oi, int tid, float (&v)[kPer]) {
+// the previous block. Every load is issued unconditionally, from a position
+// clamped into the block (column 0 of a block and position 0 of a line always
+// exist), and the values outside are zeroed afterwards: a branch around each
+// load would make the compiler wait for every load before issuing the next.
+// put: slab <- v.
+template <class S>
+__device__ __forceinline__ void fetch_t(const Args& A, long long c0, long long oi, int tid, float (&v)[kPer]) {
   const int c = tid % kLines, p0 = tid / kLines, n = (int)A.n;
   const bool cin = c < A.nc - c0;
+  const S* in = static_cast<const S*>(A.in) + oi + (cin ? c : 0);
+  if (n == kMaxN) {
+    // every position of the block in the line: one stride between loads
+    const S* q = in + (long long)p0 * A.sax_in;
+    const long long step = (long long)kPosStep * A.sax_in;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) v[j] = dev::load_as<S, float>(q, j * step);
+    if (!cin) {
+#pragma unroll
+      for (int j = 0; j < kPer; ++j) v[j] = 0.0f;
+    }
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
-    const int p = p0 + j * (kThreads / kLines);
-    v[j] = (cin && p < n) ? l2d::load_any(A.in, A.st_in, oi + p * A.sax_in + c) : 0.0f;
+    const int p = p0 + j * kPosStep;
+    v[j] = dev::load_as<S, float>(in, (long long)(p < n ? p : n - 1) * A.sax_in);
+  }
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int p = p0 + j * kPosStep;
+    if (!(cin && p < n)) v[j] = 0.0f;
   }
 }
+__device__ __forceinline__ void fetch(const Args& A, long long c0, long long oi, int tid, float (&v)[kPer]) {
+  if (A.st_in == kBF16)
+    fetch_t<dev::bf16_t>(A, c0, oi, tid, v);
+#ifndef LFX_CPU_SIM
+  else if (A.st_in == kF16)
+    fetch_t<dev::f16_t>(A, c0, oi, tid, v);
+#endif
+  else
+    fetch_t<float>(A, c0, oi, tid, v);
+}
+// Position p0 + kPosStep j of line c lies at physical slab index
+// phys(c kMaxN + p0 + 16 j) = 529 c + p0 + 16 j + j / 2 (p0 < 16): one base
+// address per thread and compile-time offsets.
+LFX_HDI int slab_line(int c, int p0) { return c * (kMaxN + kMaxN / 32 + 1) + p0; }
+LFX_HDI int slab_off(int j) { return kPosStep * j + j / 2; }
 __device__ __forceinline__ void put(const Args& A, float* slab, int tid, const float (&v)[kPer]) {
   const int c = tid % kLines, p0 = tid / kLines, n = (int)A.n;
+  float* b = slab + slab_line(c, p0);
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
-    const int p = p0 + j * (kThreads / kLines);
-    if (p < n) slab[phys(c * kMaxN + p)] = v[j];
+    const int p = p0 + j * kPosStep;
+    if (n == kMaxN || p < n) b[slab_off(j)] = v[j];
   }
 }
 
 // The block's lines <- slab (same thread mapping).
-__device__ __forceinline__ void store_block(const Args& A, const float* slab, long long c0, long long oo, int tid) {
+template <class S>
+__device__ __forceinline__ void store_block_t(const Args& A, const float* slab, long long c0, long long oo, int tid) {
   const int c = tid % kLines, p0 = tid / kLines, n = (int)A.n;
   if (c >= A.nc - c0) return;
+  S* out = static_cast<S*>(A.out) + oo + c;
+  const float* b = slab + slab_line(c, p0);
+  if (n == kMaxN) {
+    S* q = out + (long long)p0 * A.sax_out;
+    const long long step = (long long)kPosStep * A.sax_out;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) dev::store_as<S, float>(q, j * step, b[slab_off(j)]);
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
-    const int p = p0 + j * (kThreads / kLines);
-    if (p < n) l2d::store_any(A.out, A.st_out, oo + p * A.sax_out + c, slab[phys(c * kMaxN + p)]);
+    const int p = p0 + j * kPosStep;
+    if (p < n) dev::store_as<S, float>(out, (long long)p * A.sax_out, b[slab_off(j)]);
   }
+}
+__device__ __forceinline__ void store_block(const Args& A, const float* slab, long long c0, long long oo, int tid) {
+  if (A.st_out == kBF16)
+    store_block_t<dev::bf16_t>(A, slab, c0, oo, tid);
+#ifndef LFX_CPU_SIM
+  else if (A.st_out == kF16)
+    store_block_t<dev::f16_t>(A, slab, c0, oo, tid);
+#endif
+  else
+    store_block_t<float>(A, slab, c0, oo, tid);
 }
 
 // Every level of the four lines of slab rows [4 w, 4 w + 4) (virtual base

@@ -1,7 +1,6 @@
-// Across-line kernel of one transform type (included once per transform by
-// lfx_across_tf{0,1,2,3}.hip, so the table's instantiations compile in
-// parallel): lfx_across.hpp's blocks of lines along a strided axis.
-#pragma once
+// Across-line kernel of one transform type: lfx_across.hpp's blocks of lines
+// along a strided axis. build.py compiles this file once per transform type
+// (LFX_TF = 0..3), so that the table's instantiations compile in parallel.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -27,16 +26,43 @@ struct RowLds : reg::LaneGpuT<4> {
     return lfx_acr_slab[acr::phys(a)];
   }
   // x[e] of lane l = p[r rs + g], g = pos0 + E l' + e, for row r = l / 16 < nr
-  // and g in [0, n); 0 otherwise
+  // and g in [0, n); 0 otherwise. Every read is issued unconditionally, at a
+  // row and position clamped into the wave's rows, and the values outside are
+  // zeroed afterwards, so that the reads are in flight together instead of
+  // each waiting behind a branch.
+  //
+  // A chunk whose virtual start is a multiple of E (E <= 32 divides 32) lies
+  // within one 32-float block of the slab, so its physical addresses are
+  // consecutive: such chunks (every row's, whole, when the virtual start of
+  // row 0 is a multiple of E and the rows are 512 apart) are read from one
+  // base address with immediate offsets.
+  static __device__ __forceinline__ bool whole(const float* q, long long pos0, long long lo, long long hi, int E,
+                                               long long rs, int nr) {
+    const long long v0 = (long long)(q - lfx_acr_slab) + pos0;
+    return nr == 4 && rs % acr::kMaxN == 0 && lo <= 0 && hi >= 16ll * E && v0 % E == 0;
+  }
   template <int E, class S>
   static __device__ __forceinline__ void ld_raw(const S* p, long long pos0, long long n, V (&t)[E], long long rs = 0,
                                                 int nr = 1) {
     const int l = lane(), r = l >> 4, lr = l & 15;
     const float* q = reinterpret_cast<const float*>(p);
+    if (whole(q, pos0, -pos0, n - pos0, E, rs, nr)) {
+      const float* b = &at(q, r * (int)rs + (int)pos0 + E * lr);
+#pragma unroll
+      for (int e = 0; e < E; ++e) t[e] = b[e];
+      return;
+    }
+    const int rc = r < nr ? r : nr - 1, nn = (int)n;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int g = (int)pos0 + E * lr + e;
-      t[e] = (r < nr && g >= 0 && g < n) ? at(q, r * (int)rs + g) : 0.0f;
+      const int gh = g < nn ? g : nn - 1, gc = gh < 0 ? 0 : gh;
+      t[e] = at(q, rc * (int)rs + gc);
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int g = (int)pos0 + E * lr + e;
+      if (!(r < nr && g >= 0 && g < nn)) t[e] = 0.0f;
     }
   }
   template <int E>
@@ -55,6 +81,12 @@ struct RowLds : reg::LaneGpuT<4> {
                                                    long long hi, float*, long long rs = 0, int nr = 1) {
     const int l = lane(), r = l >> 4, lr = l & 15;
     const float* q = reinterpret_cast<const float*>(p);
+    if (whole(q, pos0 + g0, lo - g0, hi - g0, E, rs, nr)) {
+      float* b = &at(q, r * (int)rs + (int)(pos0 + g0) + E * lr);
+#pragma unroll
+      for (int e = 0; e < E; ++e) b[e] = v[e];
+      return;
+    }
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int g = (int)g0 + E * lr + e;
@@ -131,5 +163,9 @@ hipError_t launch_across_tf(const acr::Args& A, hipStream_t st) {
       return hipErrorInvalidValue;
   }
 }
+
+#define LFX_JOIN2(a, b) a##b
+#define LFX_JOIN(a, b) LFX_JOIN2(a, b)
+hipError_t LFX_JOIN(launch_across_, LFX_TF)(const acr::Args& A, hipStream_t st) { return launch_across_tf<LFX_TF>(A, st); }
 
 }  // namespace lfx

@@ -14,6 +14,8 @@ constexpr int kWaves = kLines / 4;
 constexpr int kThreads = 64 * kWaves;
 constexpr int kMaxPasses = 8;    // passes of at most four levels
 constexpr int kPer = kLines * kMaxN / kThreads;  // slab values per thread
+constexpr int kPosStep = kThreads / kLines;      // positions between a thread's values
+static_assert(kPosStep == 16 && kMaxN == 512, "the slab index of a thread's values (lfx_across.hpp) assumes these");
 LFX_HDI int phys(int a) { return a + (a >> 5) + (a >> 9); }
 constexpr int kSlabFloats = kLines * kMaxN + kLines * kMaxN / 32 + kLines * kMaxN / 512;
 

@@ -255,7 +255,7 @@ struct Tuning {
   int streams = -1;      // side streams of a batched stage (-1: default)
   int nd_batch = -1;     // ND lines through batched plans (0: off)
   long long small = -1;  // LDS bytes up to which batch items run as one-wave workgroups (-1: default)
-  int across = 0;        // strided axes through the across-line kernel (1: on)
+  int across = -1;       // strided axes through the across-line kernel (0: off; -1: default, on)
 };
 static Tuning env_tuning() {
   Tuning t;
@@ -495,7 +495,7 @@ static bool lbl2d(const Plan& p, const WaveletDefL& w, long long L, const void* 
   char* ws = (char*)lease.p;
   const auto levels = lvl2d_plan(hp, fwd, p.ndim, p.shape, ra, ca, L, in, p.sin, out, p.sout, p.storage, ws,
                                  ws + wsb, prog, p.wavelet);
-  for (const auto& lv : levels) hip_check(launch_lvl2d(p.tf, lv, st), "two-axis level kernel");
+  for (const auto& lv : levels) hip_check(launch_lvl2d(p.tf, p.storage, lv, st), "two-axis level kernel");
   return true;
 }
 
@@ -1539,14 +1539,15 @@ static int unit_dim(int ndim, const long long* shp, const long long* str, int ax
 static bool across_axis(const Plan& p, const long long* shp, int ax, long long L, const void* src,
                         const long long* sst, int st_in, void* dst, const long long* dstr, int st_out,
                         hipStream_t st) {
-  if (g_tuning.nd_batch == 0 || g_tuning.across != 1) return false;
+  if (g_tuning.nd_batch == 0 || g_tuning.across == 0) return false;
   if (p.bc != kZero || p.storage == kF64 || (p.flags & LFX_FLAG_PLAIN) || p.wavelet >= kCustomBase ||
       p.wavelet == 0)
     return false;
   const long long n = shp[ax];
   if (n < 2 || n > acr::kMaxN || L < 1) return false;
   const int c = unit_dim(p.ndim, shp, sst, ax);
-  if (c < 0 || dstr[c] != 1) return false;
+  // the contiguous dim supplies a workgroup's lines: at least half a block
+  if (c < 0 || dstr[c] != 1 || shp[c] < acr::kLines / 2) return false;
   long long total = 1;
   for (int k = 0; k < p.ndim; ++k) total *= shp[k];
   if (total <= 0) return false;
@@ -2068,7 +2069,7 @@ int lfx_set_tuning_ex(const lfx_tuning* t) {
     check_tuning(t->tile, t->levels_per_pass);
     if (t->side_streams < -1 || t->side_streams > 8) fail(LFX_EINVAL, "side_streams must be in [-1, 8]");
     if (t->line_limit < 0 || (t->line_limit > 0 && t->line_limit < 16)) fail(LFX_EINVAL, "line_limit must be 0 or >= 16");
-    if (t->across != 0 && t->across != 1) fail(LFX_EINVAL, "across must be 0 or 1");
+    if (t->across < -1 || t->across > 1) fail(LFX_EINVAL, "across must be -1, 0 or 1");
     g_tuning.tile = t->tile;
     g_tuning.k = t->levels_per_pass;
     g_tuning.streams = t->side_streams;

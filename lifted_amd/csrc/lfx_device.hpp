@@ -398,9 +398,17 @@ __device__ __forceinline__ void store_as<bf16_t, float>(bf16_t* p, long long i, 
   }
 }
 #ifndef LFX_CPU_SIM
+// A float32 value rounded to float16. The value is first a float32 result:
+// the empty asm keeps the compiler from folding the operation that produced
+// it into the conversion (a multiply and a conversion become one v_fma_mix,
+// which rounds once, to 16 bits, where the arithmetic defines two roundings).
+__device__ __forceinline__ _Float16 to_f16(float v) {
+  __asm__("" : "+v"(v));
+  return (_Float16)v;
+}
 template <>
 __device__ __forceinline__ void store_as<f16_t, float>(f16_t* p, long long i, float v) {
-  _Float16 h = (_Float16)v;
+  const _Float16 h = to_f16(v);
   __builtin_memcpy(&p[i].v, &h, 2);
 }
 #endif

@@ -41,16 +41,31 @@ inline hipError_t launch_reg(int tf, const void* progs, const BatchItem* items, 
 namespace l2d {
 struct Level;
 }
-hipError_t launch_lvl2d_0(const l2d::Level& L, hipStream_t st);
-hipError_t launch_lvl2d_1(const l2d::Level& L, hipStream_t st);
-hipError_t launch_lvl2d_2(const l2d::Level& L, hipStream_t st);
-hipError_t launch_lvl2d_3(const l2d::Level& L, hipStream_t st);
-inline hipError_t launch_lvl2d(int tf, const l2d::Level& L, hipStream_t st) {
-  switch (tf) {
-    case 0: return launch_lvl2d_0(L, st);
-    case 1: return launch_lvl2d_1(L, st);
-    case 2: return launch_lvl2d_2(L, st);
-    case 3: return launch_lvl2d_3(L, st);
+// launch_lvl2d_<transform>_<storage>: one per transform type and storage
+// type (float32 0, bfloat16 2, float16 3)
+#define LFX_L2D_DECL(tf)                                              \
+  hipError_t launch_lvl2d_##tf##_0(const l2d::Level& L, hipStream_t st); \
+  hipError_t launch_lvl2d_##tf##_2(const l2d::Level& L, hipStream_t st); \
+  hipError_t launch_lvl2d_##tf##_3(const l2d::Level& L, hipStream_t st);
+LFX_L2D_DECL(0)
+LFX_L2D_DECL(1)
+LFX_L2D_DECL(2)
+LFX_L2D_DECL(3)
+#undef LFX_L2D_DECL
+// every array of a level has the storage type of its input (the kernels are
+// compiled per storage type; a launch whose arrays differ is refused)
+inline hipError_t launch_lvl2d(int tf, int storage, const l2d::Level& L, hipStream_t st) {
+  switch (tf * 4 + storage) {
+#define LFX_L2D_CASE(tf, s) \
+  case tf * 4 + s:          \
+    return launch_lvl2d_##tf##_##s(L, st);
+#define LFX_L2D_CASES(tf) LFX_L2D_CASE(tf, 0) LFX_L2D_CASE(tf, 2) LFX_L2D_CASE(tf, 3)
+    LFX_L2D_CASES(0)
+    LFX_L2D_CASES(1)
+    LFX_L2D_CASES(2)
+    LFX_L2D_CASES(3)
+#undef LFX_L2D_CASES
+#undef LFX_L2D_CASE
   }
   return hipErrorInvalidValue;
 }

@@ -30,69 +30,55 @@
 namespace lfx {
 namespace l2d {
 
-__device__ __forceinline__ float load_any(const void* p, int st, long long i) {
-  if (st == kBF16) return dev::load_as<dev::bf16_t, float>(static_cast<const dev::bf16_t*>(p), i);
-#ifndef LFX_CPU_SIM
-  if (st == kF16) return dev::load_as<dev::f16_t, float>(static_cast<const dev::f16_t*>(p), i);
-#endif
-  return static_cast<const float*>(p)[i];
-}
-__device__ __forceinline__ void store_any(void* p, int st, long long i, float v) {
-  if (st == kBF16) {
-    dev::store_as<dev::bf16_t, float>(static_cast<dev::bf16_t*>(p), i, v);
-    return;
-  }
-#ifndef LFX_CPU_SIM
-  if (st == kF16) {
-    dev::store_as<dev::f16_t, float>(static_cast<dev::f16_t*>(p), i, v);
-    return;
-  }
-#endif
-  static_cast<float*>(p)[i] = v;
-}
-
 // Rows of one array as a tile sees them: the byte address of the tile
 // window's origin (row 0, column 0 of the window; it may lie before the
 // array, whose elements outside the band are never accessed) and the row
 // pitch in bytes. Every access is that wave-uniform origin plus a 32-bit byte
 // offset (the pitch times a compile-time row, plus the lane's column), so the
 // 64-bit index arithmetic runs once per tile and not once per access (the
-// host keeps a slice's byte offsets within 31 bits).
+// host keeps a slice's byte offsets within 31 bits). The storage type ST is a
+// compile-time parameter: a run-time type switch around every access puts a
+// branch between consecutive loads, and the compiler then waits for each load
+// before issuing the next.
+template <int ST>
 struct Io {
+  static constexpr int es = ST == kF32 ? 4 : 2;  // element bytes
   char* org;
   int rowb;  // bytes between rows
-  int es;    // element bytes
-  int st;    // storage type
   int v;     // pairs are aligned (Arr::v)
 };
-__device__ __forceinline__ Io make_io(const Arr& A, long long o, long long row0, long long col0) {
-  Io x;
-  x.st = A.st;
-  x.es = A.st == kF32 ? 4 : 2;
+template <int ST>
+__device__ __forceinline__ Io<ST> make_io(const Arr& A, long long o, long long row0, long long col0) {
+  Io<ST> x;
   x.v = A.v;
-  x.org = static_cast<char*>(A.p) + (o + row0 * A.sr + col0) * (long long)x.es;
-  x.rowb = (int)(A.sr * x.es);
+  x.org = static_cast<char*>(A.p) + (o + row0 * A.sr + col0) * (long long)Io<ST>::es;
+  x.rowb = (int)(A.sr * Io<ST>::es);
   return x;
 }
-__device__ __forceinline__ float ldx(const Io& a, unsigned off) {
+template <int ST>
+__device__ __forceinline__ float ldx(const Io<ST>& a, unsigned off) {
   const char* q = a.org + off;
-  if (a.st == kF32) return *reinterpret_cast<const float*>(q);
-  const uint32_t h = *reinterpret_cast<const uint16_t*>(q);
-  if (a.st == kBF16) return __uint_as_float(h << 16);
+  if constexpr (ST == kF32) {
+    return *reinterpret_cast<const float*>(q);
+  } else if constexpr (ST == kBF16) {
+    return __uint_as_float((uint32_t)*reinterpret_cast<const uint16_t*>(q) << 16);
+  } else {
 #ifndef LFX_CPU_SIM
-  _Float16 f;
-  const uint16_t hh = (uint16_t)h;
-  __builtin_memcpy(&f, &hh, 2);
-  return (float)f;
+    const uint16_t h = *reinterpret_cast<const uint16_t*>(q);
+    _Float16 f;
+    __builtin_memcpy(&f, &h, 2);
+    return (float)f;
 #else
-  return 0.0f;
+    return 0.0f;
 #endif
+  }
 }
-__device__ __forceinline__ void stx(const Io& a, unsigned off, float v) {
+template <int ST>
+__device__ __forceinline__ void stx(const Io<ST>& a, unsigned off, float v) {
   char* q = a.org + off;
-  if (a.st == kF32) {
+  if constexpr (ST == kF32) {
     *reinterpret_cast<float*>(q) = v;
-  } else if (a.st == kBF16) {
+  } else if constexpr (ST == kBF16) {
     dev::store_as<dev::bf16_t, float>(reinterpret_cast<dev::bf16_t*>(q), 0, v);
   } else {
 #ifndef LFX_CPU_SIM
@@ -102,14 +88,15 @@ __device__ __forceinline__ void stx(const Io& a, unsigned off, float v) {
 }
 #ifndef LFX_CPU_SIM
 // two consecutive elements (pair-aligned) as one access
-__device__ __forceinline__ void ld2x(const Io& a, unsigned off, float& x0, float& x1) {
+template <int ST>
+__device__ __forceinline__ void ld2x(const Io<ST>& a, unsigned off, float& x0, float& x1) {
   const char* q = a.org + off;
-  if (a.st == kF32) {
+  if constexpr (ST == kF32) {
     typedef float f2 __attribute__((ext_vector_type(2)));
     const f2 v = *reinterpret_cast<const f2*>(q);
     x0 = v.x;
     x1 = v.y;
-  } else if (a.st == kBF16) {
+  } else if constexpr (ST == kBF16) {
     const uint32_t u = *reinterpret_cast<const uint32_t*>(q);
     x0 = __uint_as_float(u << 16);
     x1 = __uint_as_float(u & 0xffff0000u);
@@ -120,15 +107,16 @@ __device__ __forceinline__ void ld2x(const Io& a, unsigned off, float& x0, float
     x1 = (float)v.y;
   }
 }
-__device__ __forceinline__ void st2x(const Io& a, unsigned off, float x0, float x1) {
+template <int ST>
+__device__ __forceinline__ void st2x(const Io<ST>& a, unsigned off, float x0, float x1) {
   char* q = a.org + off;
-  if (a.st == kF32) {
+  if constexpr (ST == kF32) {
     typedef float f2 __attribute__((ext_vector_type(2)));
     f2 v;
     v.x = x0;
     v.y = x1;
     *reinterpret_cast<f2*>(q) = v;
-  } else if (a.st == kBF16) {
+  } else if constexpr (ST == kBF16) {
     dev::bf16_t h[2];
     dev::store_as<dev::bf16_t, float>(h, 0, x0);
     dev::store_as<dev::bf16_t, float>(h, 1, x1);
@@ -136,8 +124,8 @@ __device__ __forceinline__ void st2x(const Io& a, unsigned off, float x0, float 
   } else {
     typedef _Float16 h2 __attribute__((ext_vector_type(2)));
     h2 v;
-    v.x = (_Float16)x0;
-    v.y = (_Float16)x1;
+    v.x = dev::to_f16(x0);
+    v.y = dev::to_f16(x1);
     *reinterpret_cast<h2*>(q) = v;
   }
 }
@@ -174,7 +162,7 @@ __device__ __forceinline__ void rows_level(typename P::V (&X)[kRows][kLaneW], co
 // in: a[0] (the sub-box); details to a[1] (rows s | d at [0, ns_r) /
 // [ns_r, m_r), same for columns), the ss quadrant to a[2] at its (s row,
 // s col). o0..o2: the slice's element offsets.
-template <class P, class W, int TF>
+template <class P, class W, int TF, int ST>
 __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, const Span& r, const Span& c, const Arr& A0,
                                          long long o0, const Arr& A1, long long o1, const Arr& A2, long long o2) {
   using V = typename P::V;
@@ -188,25 +176,35 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, const Span
   const int rl = wclamp(-r.w0, kRows), rh = wclamp(r.m - r.w0, kRows);
   const int cl = wclamp(-c.w0, kCols), ch = wclamp(c.m - c.w0, kCols);
   const bool cin = cl == 0 && ch == kCols;
-  const Io in = make_io(A0, o0, r.w0, c.w0);
+  const Io<ST> in = make_io<ST>(A0, o0, r.w0, c.w0);
   V X[kRows][kLaneW];
-  reg::sfor<kRows>([&](auto R) {
-    constexpr int q = decltype(R)::value;
-    const bool row_in = q >= rl && q < rh;
+  if (rl == 0 && rh == kRows && cin && in.v) {
+    // the whole window in the band: every row one unconditional pair load,
+    // issued back to back
+    reg::sfor<kRows>([&](auto R) {
+      constexpr int q = decltype(R)::value;
 #ifndef LFX_CPU_SIM
-    if (row_in && cin && in.v) {
-      ld2x(in, (unsigned)(q * in.rowb + kLaneW * in.es * (int)(threadIdx.x & 63)), X[q][0], X[q][1]);
-      return;
-    }
+      ld2x(in, (unsigned)(q * in.rowb + kLaneW * Io<ST>::es * (int)(threadIdx.x & 63)), X[q][0], X[q][1]);
+#else
+      reg::sfor<kLaneW>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        X[q][k] = P::gen([&](int l) { return ldx(in, (unsigned)(q * in.rowb + (kLaneW * l + k) * Io<ST>::es)); });
+      });
 #endif
-    reg::sfor<kLaneW>([&](auto K) {
-      constexpr int k = decltype(K)::value;
-      X[q][k] = P::gen([&](int l) {
-        const int pc = kLaneW * l + k;
-        return (row_in && pc >= cl && pc < ch) ? ldx(in, (unsigned)(q * in.rowb + pc * in.es)) : 0.0f;
+    });
+  } else {
+    reg::sfor<kRows>([&](auto R) {
+      constexpr int q = decltype(R)::value;
+      const bool row_in = q >= rl && q < rh;
+      reg::sfor<kLaneW>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        X[q][k] = P::gen([&](int l) {
+          const int pc = kLaneW * l + k;
+          return (row_in && pc >= cl && pc < ch) ? ldx(in, (unsigned)(q * in.rowb + pc * Io<ST>::es)) : 0.0f;
+        });
       });
     });
-  });
+  }
   const bool mr = rl > 0 || rh < kRows;
   typename P::M ms{}, md{};
   if (mr) {
@@ -228,14 +226,14 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, const Span
   // columns [jl, jsh) / [jl, jdh)
   const int el = wclamp(r.lo() - rq0, E), esh = wclamp(r.hi_s() - rq0, E), edh = wclamp(r.hi_d() - rq0, E);
   const int jl = wclamp(c.lo() - j0, NJ), jsh = wclamp(c.hi_s() - j0, NJ), jdh = wclamp(c.hi_d() - j0, NJ);
-  const Io ss = make_io(A2, o2, rq0, j0), sd = make_io(A1, o1, rq0, nsc + j0);
-  const Io ds = make_io(A1, o1, nsr + rq0, j0), dd = make_io(A1, o1, nsr + rq0, nsc + j0);
-  auto put = [&](const Io& a, int e, int hi, const V (&x)[kLaneE]) {
+  const Io<ST> ss = make_io<ST>(A2, o2, rq0, j0), sd = make_io<ST>(A1, o1, rq0, nsc + j0);
+  const Io<ST> ds = make_io<ST>(A1, o1, nsr + rq0, j0), dd = make_io<ST>(A1, o1, nsr + rq0, nsc + j0);
+  auto put = [&](const Io<ST>& a, int e, int hi, const V (&x)[kLaneE]) {
     reg::sfor<kLaneE>([&](auto K) {
       constexpr int k = decltype(K)::value;
       P::each(x[k], [&](int l, float val) {
         const int jj = kLaneE * l + k;
-        if (jj >= jl && jj < hi) stx(a, (unsigned)(e * a.rowb + jj * a.es), val);
+        if (jj >= jl && jj < hi) stx(a, (unsigned)(e * a.rowb + jj * Io<ST>::es), val);
       });
     });
   };
@@ -266,7 +264,7 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, const Span
 // Reverse-stepping level of one tile (register axis first, then lane axis):
 // the ss quadrant from a[2], the other quadrants from a[0] (the coefficient
 // layout of the sub-box), the interleaved sub-box to a[1].
-template <class P, class W, int TF>
+template <class P, class W, int TF, int ST>
 __device__ __forceinline__ void rev_tile(const DProgram<float>* prog, const Span& r, const Span& c, const Arr& A0,
                                          long long o0, const Arr& A1, long long o1, const Arr& A2, long long o2) {
   using V = typename P::V;
@@ -281,39 +279,56 @@ __device__ __forceinline__ void rev_tile(const DProgram<float>* prog, const Span
   // columns [jl, jsh), d columns [jl, jdh)
   const int el = wclamp(-rq0, E), esh = wclamp(nsr - rq0, E), edh = wclamp(ndr - rq0, E);
   const int jl = wclamp(-j0, NJ), jsh = wclamp(nsc - j0, NJ), jdh = wclamp(ndc - j0, NJ);
-  const Io ss = make_io(A2, o2, rq0, j0), sd = make_io(A0, o0, rq0, nsc + j0);
-  const Io ds = make_io(A0, o0, nsr + rq0, j0), dd = make_io(A0, o0, nsr + rq0, nsc + j0);
-  auto get = [&](const Io& a, int e, bool row_ok, int hi, V (&x)[kLaneE]) {
+  const Io<ST> ss = make_io<ST>(A2, o2, rq0, j0), sd = make_io<ST>(A0, o0, rq0, nsc + j0);
+  const Io<ST> ds = make_io<ST>(A0, o0, nsr + rq0, j0), dd = make_io<ST>(A0, o0, nsr + rq0, nsc + j0);
+  auto get = [&](const Io<ST>& a, int e, bool row_ok, int hi, V (&x)[kLaneE]) {
     reg::sfor<kLaneE>([&](auto K) {
       constexpr int k = decltype(K)::value;
       x[k] = P::gen([&](int l) {
         const int jj = kLaneE * l + k;
-        return (row_ok && jj >= jl && jj < hi) ? ldx(a, (unsigned)(e * a.rowb + jj * a.es)) : 0.0f;
+        return (row_ok && jj >= jl && jj < hi) ? ldx(a, (unsigned)(e * a.rowb + jj * Io<ST>::es)) : 0.0f;
       });
     });
   };
   // X[2e][2k] / X[2e][2k + 1]: s row rq0 + e, s / d column band index
   // j0 + kLaneE lane + k; X[2e + 1][...]: the d row
   V X[kRows][kLaneW];
-  reg::sfor<E>([&](auto I) {
-    constexpr int e = decltype(I)::value;
-    const bool s_in = e >= el && e < esh, d_in = e >= el && e < edh;
-    V a[kLaneE], b[kLaneE];
-    get(ss, e, s_in, jsh, a);
-    get(sd, e, s_in, jdh, b);
-    reg::sfor<kLaneE>([&](auto K) {
-      constexpr int k = decltype(K)::value;
-      X[2 * e][2 * k] = a[k];
-      X[2 * e][2 * k + 1] = b[k];
+  if (el == 0 && esh == E && edh == E && jl == 0 && jsh == NJ && jdh == NJ) {
+    // the whole window in the bands: unconditional loads, issued back to back
+    auto all = [&](const Io<ST>& a, int e, int k) {
+      return P::gen([&](int l) { return ldx(a, (unsigned)(e * a.rowb + (kLaneE * l + k) * Io<ST>::es)); });
+    };
+    reg::sfor<E>([&](auto I) {
+      constexpr int e = decltype(I)::value;
+      reg::sfor<kLaneE>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        X[2 * e][2 * k] = all(ss, e, k);
+        X[2 * e][2 * k + 1] = all(sd, e, k);
+        X[2 * e + 1][2 * k] = all(ds, e, k);
+        X[2 * e + 1][2 * k + 1] = all(dd, e, k);
+      });
     });
-    get(ds, e, d_in, jsh, a);
-    get(dd, e, d_in, jdh, b);
-    reg::sfor<kLaneE>([&](auto K) {
-      constexpr int k = decltype(K)::value;
-      X[2 * e + 1][2 * k] = a[k];
-      X[2 * e + 1][2 * k + 1] = b[k];
+  } else {
+    reg::sfor<E>([&](auto I) {
+      constexpr int e = decltype(I)::value;
+      const bool s_in = e >= el && e < esh, d_in = e >= el && e < edh;
+      V a[kLaneE], b[kLaneE];
+      get(ss, e, s_in, jsh, a);
+      get(sd, e, s_in, jdh, b);
+      reg::sfor<kLaneE>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        X[2 * e][2 * k] = a[k];
+        X[2 * e][2 * k + 1] = b[k];
+      });
+      get(ds, e, d_in, jsh, a);
+      get(dd, e, d_in, jdh, b);
+      reg::sfor<kLaneE>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        X[2 * e + 1][2 * k] = a[k];
+        X[2 * e + 1][2 * k + 1] = b[k];
+      });
     });
-  });
+  }
   const bool mr = el > 0 || esh < E || edh < E;
   typename P::M ms{}, md{};
   if (mr) {
@@ -331,7 +346,7 @@ __device__ __forceinline__ void rev_tile(const DProgram<float>* prog, const Span
   // owned window-relative fine rows [ra, rb) and columns [ca, cb)
   const int ra = wclamp(r.a - r.w0, kRows), rb = wclamp((r.b >= r.m ? r.m : r.b) - r.w0, kRows);
   const int ca = wclamp(c.a - c.w0, kCols), cb = wclamp((c.b >= c.m ? c.m : c.b) - c.w0, kCols);
-  const Io out = make_io(A1, o1, r.w0, c.w0);
+  const Io<ST> out = make_io<ST>(A1, o1, r.w0, c.w0);
   reg::sfor<kRows>([&](auto R) {
     constexpr int q = decltype(R)::value;
     V ls[kLaneE], ld[kLaneE];
@@ -351,7 +366,7 @@ __device__ __forceinline__ void rev_tile(const DProgram<float>* prog, const Span
 #ifndef LFX_CPU_SIM
       const int pc = kLaneW * (int)(threadIdx.x & 63);
       if (out.v && pc >= ca && pc + 1 < cb) {
-        st2x(out, (unsigned)(q * out.rowb + pc * out.es), y[0], y[1]);
+        st2x(out, (unsigned)(q * out.rowb + pc * Io<ST>::es), y[0], y[1]);
         return;
       }
 #endif
@@ -359,7 +374,7 @@ __device__ __forceinline__ void rev_tile(const DProgram<float>* prog, const Span
         constexpr int k = decltype(K)::value;
         P::each(y[k], [&](int l, float val) {
           const int pc2 = kLaneW * l + k;
-          if (pc2 >= ca && pc2 < cb) stx(out, (unsigned)(q * out.rowb + pc2 * out.es), val);
+          if (pc2 >= ca && pc2 < cb) stx(out, (unsigned)(q * out.rowb + pc2 * Io<ST>::es), val);
         });
       });
     }
@@ -369,7 +384,7 @@ __device__ __forceinline__ void rev_tile(const DProgram<float>* prog, const Span
 // Tile w of a level (slice-major, then register-axis tiles, then lane-axis
 // tiles), its slice's element offsets in the three arrays from the batch
 // dims (fast 32-bit divisions: the launch has fewer than 2^31 tiles).
-template <class P, class W, int TF>
+template <class P, class W, int TF, int ST>
 __device__ __forceinline__ void level_tile(const Level& L, long long w) {
   const uint32_t wi = (uint32_t)w;
   uint32_t b = fdiv(L.f_per, wi);
@@ -392,9 +407,9 @@ __device__ __forceinline__ void level_tile(const Level& L, long long w) {
   }
   const DProgram<float>* prog = static_cast<const DProgram<float>*>(L.prog);
   if constexpr (TF == kForward || TF == kInverseAdjoint)
-    fwd_tile<P, W, TF>(prog, r, c, L.a[0], o[0], L.a[1], o[1], L.a[2], o[2]);
+    fwd_tile<P, W, TF, ST>(prog, r, c, L.a[0], o[0], L.a[1], o[1], L.a[2], o[2]);
   else
-    rev_tile<P, W, TF>(prog, r, c, L.a[0], o[0], L.a[1], o[1], L.a[2], o[2]);
+    rev_tile<P, W, TF, ST>(prog, r, c, L.a[0], o[0], L.a[1], o[1], L.a[2], o[2]);
 }
 
 }  // namespace l2d

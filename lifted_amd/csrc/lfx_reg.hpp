@@ -306,12 +306,28 @@ struct LaneGpuT {
         const S* q = p + pos0;
 #pragma unroll
         for (int j = 0; j < E; ++j) t[j] = dev::load_as<S, float>(q, j * kLanes + l);  // all loads in flight
-      } else {
+      } else if (n > 0) {
+        // a window over a band end: every load issued unconditionally, from a
+        // position clamped into the band, and the values outside zeroed
+        // afterwards (a branch around each load would make the compiler wait
+        // for it before issuing the next)
+        // byte offsets from the window start: the clamped position lies
+        // within the window (it overlaps the band), so they are 32-bit
+        const char* q = reinterpret_cast<const char*>(p + pos0);
 #pragma unroll
         for (int j = 0; j < E; ++j) {
           const long long g = pos0 + j * kLanes + l;
-          t[j] = (g >= 0 && g < n) ? dev::load_as<S, float>(p, g) : 0.0f;
+          const unsigned off = (unsigned)((g < 0 ? 0 : (g < n ? g : n - 1)) - pos0) * (unsigned)sizeof(S);
+          t[j] = dev::load_as<S, float>(reinterpret_cast<const S*>(q + off), 0);
         }
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+          const long long g = pos0 + j * kLanes + l;
+          if (!(g >= 0 && g < n)) t[j] = 0.0f;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < E; ++j) t[j] = 0.0f;
       }
     } else {
       constexpr int SEG = E * kRowLanes;

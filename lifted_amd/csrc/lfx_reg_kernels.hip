@@ -1,7 +1,8 @@
-// Register-tile kernels of one transform type (included once per transform by
-// lfx_reg_tf{0,1,2,3}.hip, so the 34 wavelet instantiations of each transform
-// compile in parallel). One wavefront runs one BatchItem of kind register tile.
-#pragma once
+// Register-tile kernels of one transform type. build.py compiles this file
+// once per transform type (LFX_TF = 0..3) and family (LFX_ROWS = 0: windows
+// of one line, 1: rows of four equal lines), so that the 34 wavelet
+// instantiations of each compile in parallel. One wavefront runs one
+// BatchItem of kind register tile.
 #include <hip/hip_runtime.h>
 
 #include "lfx_kernels.hpp"
@@ -130,5 +131,19 @@ hipError_t launch_reg_tf(const void* progs, const BatchItem* items, long long ni
                      (const float*)in, (float*)out, (float*)scr);
   return hipGetLastError();
 }
+
+#define LFX_JOIN2(a, b) a##b
+#define LFX_JOIN(a, b) LFX_JOIN2(a, b)
+#if LFX_ROWS
+hipError_t LFX_JOIN(launch_rows_, LFX_TF)(const void* progs, const BatchItem* items, long long nitems, const void* in,
+                                          void* out, void* scr, hipStream_t st) {
+  return launch_rows_tf<LFX_TF>(progs, items, nitems, in, out, scr, st);
+}
+#else
+hipError_t LFX_JOIN(launch_reg_, LFX_TF)(const void* progs, const BatchItem* items, long long nitems, const void* in,
+                                         void* out, void* scr, hipStream_t st) {
+  return launch_reg_tf<LFX_TF>(progs, items, nitems, in, out, scr, st);
+}
+#endif
 
 }  // namespace lfx

@@ -11,7 +11,8 @@ when the plan is created (lfx_set_tuning_ex in the C ABI):
     tile             tile length of LDS-tiled passes (float64 and other plans
                      without register tiles)
     across           ND calls: strided float32 axes of at most 512 samples
-                     read in place by the across-line kernel (0 / 1)
+                     read in place by the across-line kernel (0 off, 1 on,
+                     -1 default: on)
 
 tune_batch() builds the plans of a signal list for every candidate setting,
 times them on the device with the caller's data, and returns the fastest.
@@ -50,7 +51,7 @@ def get_tuning():
     return {f: getattr(t, f) for f in FIELDS}
 
 
-def set_tuning(tile=0, levels_per_pass=0, side_streams=-1, line_limit=0, across=0):
+def set_tuning(tile=0, levels_per_pass=0, side_streams=-1, line_limit=0, across=-1):
     """Planning knobs for plans created afterwards on this thread."""
     t = LfxTuning(int(tile), int(levels_per_pass), int(side_streams), int(line_limit), int(across), 0)
     _lib.check(_bind().lfx_set_tuning_ex(ctypes.byref(t)))
@@ -60,7 +61,7 @@ def set_tuning(tile=0, levels_per_pass=0, side_streams=-1, line_limit=0, across=
 def tuning(**knobs):
     """Temporarily set planning knobs (unnamed ones keep their defaults)."""
     old = get_tuning()
-    cfg = {"tile": 0, "levels_per_pass": 0, "side_streams": -1, "line_limit": 0, "across": 0}
+    cfg = {"tile": 0, "levels_per_pass": 0, "side_streams": -1, "line_limit": 0, "across": -1}
     cfg.update(knobs)
     set_tuning(**cfg)
     try:

@@ -17,12 +17,12 @@
 
 using namespace lfx;
 
-template <int TF>
-static bool run_tile(uint32_t wid, const l2d::Level& L, long long w) {
+template <int TF, int ST>
+static bool run_tile_st(uint32_t wid, const l2d::Level& L, long long w) {
   switch (wid) {
 #define LFX_SIM_W(nm)                                          \
   case wt::WL_##nm::id:                                        \
-    l2d::level_tile<reg::LaneCpu, wt::WL_##nm, TF>(L, w);      \
+    l2d::level_tile<reg::LaneCpu, wt::WL_##nm, TF, ST>(L, w);  \
     return true;
     LFX_SIM_W(Daubechies1)
     LFX_SIM_W(Daubechies2)
@@ -37,6 +37,12 @@ static bool run_tile(uint32_t wid, const l2d::Level& L, long long w) {
 #undef LFX_SIM_W
   }
   return false;
+}
+
+// the kernels are compiled per storage type (float32 or bfloat16 here)
+template <int TF>
+static bool run_tile(uint32_t wid, const l2d::Level& L, long long w) {
+  return L.a[0].st == kBF16 ? run_tile_st<TF, kBF16>(wid, L, w) : run_tile_st<TF, kF32>(wid, L, w);
 }
 
 int main(int argc, char** argv) {

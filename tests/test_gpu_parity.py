@@ -502,16 +502,19 @@ def test_two_axis_level_by_level(gpu, oracle, storage):
         assert torch.equal(a, xi), f"{storage} in place tf={tf}"
 
 
-def test_across_strided_axes(gpu, oracle):
-    """With the across knob on, strided axes of at most 512 samples (Zero
-    boundary) run through the across-line kernel, which reads 32 neighbouring
-    lines per workgroup where they lie: single strided axes (partial blocks
-    of lines, several batch dims, row and column ranges of a larger array,
-    in place, more than four levels in two passes), separable and level-by-level transforms whose strided axes chain through
-    it, all transforms, bit for bit against the oracle; 16-bit storage rounded
-    once (single axis) against the float32 oracle on the 16-bit input."""
+@pytest.mark.parametrize("across", [1, 0])
+def test_across_strided_axes(gpu, oracle, across):
+    """Strided axes of at most 512 samples (Zero boundary) run through the
+    across-line kernel (the default, knob across=1), which reads 32
+    neighbouring lines per workgroup where they lie, or through a transpose
+    and a row plan (across=0): single strided axes (partial blocks of lines,
+    several batch dims, row and column ranges of a larger array, in place,
+    more than four levels in two passes), separable and level-by-level
+    transforms whose strided axes chain through it, all transforms, bit for
+    bit against the oracle; 16-bit storage rounded once (single axis) against
+    the float32 oracle on the 16-bit input."""
     from lifted_amd import tune
-    with tune.tuning(across=1):
+    with tune.tuning(across=across):
         _across_cases(gpu, oracle)
 
 
