@@ -96,10 +96,15 @@ LFX_HDI int slab_off(int j) { return kPosStep * j + j / 2; }
 __device__ __forceinline__ void put(const Args& A, float* slab, int tid, const float (&v)[kPer]) {
   const int c = tid % kLines, p0 = tid / kLines, n = (int)A.n;
   float* b = slab + slab_line(c, p0);
+  if (n == kMaxN) {
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) b[slab_off(j)] = v[j];
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const int p = p0 + j * kPosStep;
-    if (n == kMaxN || p < n) b[slab_off(j)] = v[j];
+    if (p < n) b[slab_off(j)] = v[j];
   }
 }
 

@@ -8,7 +8,8 @@
 namespace lfx {
 namespace acr {
 
-constexpr int kLines = 32;       // lines per workgroup
+constexpr int kLines = 16;       // lines per workgroup
+constexpr int kMinLines = 16;    // fewest lines along the contiguous dim worth a launch
 constexpr int kMaxN = 512;       // samples per line (row tiles: 16 lanes x 32)
 constexpr int kWaves = kLines / 4;
 constexpr int kThreads = 64 * kWaves;

@@ -1546,8 +1546,8 @@ static bool across_axis(const Plan& p, const long long* shp, int ax, long long L
   const long long n = shp[ax];
   if (n < 2 || n > acr::kMaxN || L < 1) return false;
   const int c = unit_dim(p.ndim, shp, sst, ax);
-  // the contiguous dim supplies a workgroup's lines: at least half a block
-  if (c < 0 || dstr[c] != 1 || shp[c] < acr::kLines / 2) return false;
+  // the contiguous dim supplies a workgroup's lines
+  if (c < 0 || dstr[c] != 1 || shp[c] < acr::kMinLines) return false;
   long long total = 1;
   for (int k = 0; k < p.ndim; ++k) total *= shp[k];
   if (total <= 0) return false;

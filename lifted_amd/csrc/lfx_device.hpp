@@ -387,15 +387,24 @@ __device__ __forceinline__ float load_as<f16_t, float>(const f16_t* p, long long
 
 template <class S, class C>
 __device__ __forceinline__ void store_as(S* p, long long i, C v) { p[i] = (S)v; }
+// A float32 value rounded to bfloat16: to nearest, ties to even; a NaN keeps
+// its sign and leading payload bits and becomes quiet. On gfx950 this is one
+// v_cvt_pk_bf16_f32, which gives the same 16 bits as the integer form below for
+// every float32 pattern (checked on the device over all 2^32 of them).
+__device__ __forceinline__ uint16_t to_bf16_bits(float v) {
+#ifndef LFX_CPU_SIM
+  __asm__("" : "+v"(v));
+  return __builtin_bit_cast(uint16_t, (__bf16)v);
+#else
+  uint32_t u = __float_as_uint(v);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)(u >> 16);
+#endif
+}
 template <>
 __device__ __forceinline__ void store_as<bf16_t, float>(bf16_t* p, long long i, float v) {
-  uint32_t u = __float_as_uint(v);
-  if ((u & 0x7fffffffu) > 0x7f800000u) {
-    p[i].v = (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
-  } else {
-    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-    p[i].v = (uint16_t)(u >> 16);
-  }
+  p[i].v = to_bf16_bits(v);
 }
 #ifndef LFX_CPU_SIM
 // A float32 value rounded to float16. The value is first a float32 result:

@@ -117,10 +117,7 @@ __device__ __forceinline__ void st2x(const Io<ST>& a, unsigned off, float x0, fl
     v.y = x1;
     *reinterpret_cast<f2*>(q) = v;
   } else if constexpr (ST == kBF16) {
-    dev::bf16_t h[2];
-    dev::store_as<dev::bf16_t, float>(h, 0, x0);
-    dev::store_as<dev::bf16_t, float>(h, 1, x1);
-    *reinterpret_cast<uint32_t*>(q) = (uint32_t)h[0].v | ((uint32_t)h[1].v << 16);
+    *reinterpret_cast<uint32_t*>(q) = (uint32_t)dev::to_bf16_bits(x0) | ((uint32_t)dev::to_bf16_bits(x1) << 16);
   } else {
     typedef _Float16 h2 __attribute__((ext_vector_type(2)));
     h2 v;
