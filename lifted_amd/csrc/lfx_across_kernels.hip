@@ -112,7 +112,10 @@ __global__ void __launch_bounds__(acr::kThreads) k_across(acr::Args A) {
     acr::put(A, lfx_acr_slab, tid, v);
   }
   __syncthreads();
-  acr::run_lines<RowLds, W, TF>(A, lfx_acr_slab + (tid >> 6) * 4 * acr::kMaxN);
+  // the wave's rows of the slab: a wave-uniform base (scalar registers), so
+  // the policy's address and alignment arithmetic on it stays scalar
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  acr::run_lines<RowLds, W, TF>(A, lfx_acr_slab + wave * 4 * acr::kMaxN);
   __syncthreads();
   acr::store_block(A, lfx_acr_slab, c0, oo, tid);
 }
