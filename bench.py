@@ -529,7 +529,7 @@ def workload_c5(steps, warmup):
         del w, f
     axes = (1, 2)
     ref = la.lwt_forward(x64, 100907, 0, axes=axes, level=6)
-    rows = []
+    rows, cases = [], []
     for name, dt in (("fp32", torch.float32), ("bf16", torch.bfloat16), ("fp16", torch.float16)):
         xs = x64.to(dt)
         y = la.lwt_forward(xs, 100907, 0, axes=axes, level=6)
@@ -539,22 +539,40 @@ def workload_c5(steps, warmup):
         e_coef_max_rel = e_coef_max / float(ref.abs().max())
         e_rt = float((z.double() - xs.double()).norm() / xs.double().norm())
         e_rt64 = float((z.double() - x64).norm() / x64.norm())
-        yo = torch.empty_like(xs)
-        zo = torch.empty_like(xs)
-
-        def fi():
-            la.lwt_forward(xs, 100907, 0, axes=axes, level=6, out=yo)
-            la.lwt_inverse(yo, 100907, 0, axes=axes, level=6, out=zo)
-
-        ms = _time_calls(fi, steps, warmup)
-        es = xs.element_size()
+        del y, z
         rows.append({"storage": name, "coef_rel_err_vs_fp64": e_coef, "coef_max_abs_err_vs_fp64": e_coef_max,
                      "coef_max_err_rel_to_max_coef": e_coef_max_rel, "roundtrip_rel_err": e_rt,
-                     "roundtrip_rel_err_vs_fp64_input": e_rt64, "ms_F_plus_I": round(ms, 3),
-                     "GBps_compulsory": round(4 * xs.numel() * es / (ms * 1e-3) / 1e9, 1),
-                     "Gsamples_per_s": round(2 * xs.numel() / (ms * 1e-3) / 1e9, 2)})
-        del xs, y, z, yo, zo
-        torch.cuda.empty_cache()
+                     "roundtrip_rel_err_vs_fp64_input": e_rt64})
+        cases.append((xs, torch.empty_like(xs), torch.empty_like(xs)))
+    torch.cuda.empty_cache()
+
+    def fi(xs, yo, zo):
+        la.lwt_forward(xs, 100907, 0, axes=axes, level=6, out=yo)
+        la.lwt_inverse(yo, 100907, 0, axes=axes, level=6, out=zo)
+
+    # the storage types are timed in turns (one F + I each per round, median of
+    # the rounds), so that run order (clocks, caches) favours none of them
+    for c in cases:
+        for _ in range(max(warmup, 1)):
+            fi(*c)
+    torch.cuda.synchronize()
+    times = [[] for _ in cases]
+    for _ in range(max(steps, 1)):
+        for i, c in enumerate(cases):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            fi(*c)
+            b.record()
+            b.synchronize()
+            times[i].append(a.elapsed_time(b))
+    for row, (xs, _, _), ts in zip(rows, cases, times):
+        ms = sorted(ts)[len(ts) // 2]
+        es = xs.element_size()
+        row.update({"ms_F_plus_I": round(ms, 3),
+                    "GBps_compulsory": round(4 * xs.numel() * es / (ms * 1e-3) / 1e9, 1),
+                    "Gsamples_per_s": round(2 * xs.numel() / (ms * 1e-3) / 1e9, 2)})
+    del cases
+    torch.cuda.empty_cache()
     return {"workload": "c5: 16 x 8192^2 1/f^2 fields (unit variance), CDF9_7, level=6 (level by level), "
                         "Zero BC, forward + inverse; errors are relative L2 against the same kernels in fp64",
             "rows": rows}
