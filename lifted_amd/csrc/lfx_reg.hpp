@@ -418,15 +418,21 @@ struct LaneGpuT {
     // owned window-relative range [t0, t1) of the E * 64 values (one window)
     const int t0 = (int)(lo - g0 < 0 ? 0 : (lo - g0 > E * kLanes ? E * kLanes : lo - g0));
     const int t1 = (int)(hi - g0 > E * kLanes ? E * kLanes : (hi - g0 < 0 ? 0 : hi - g0));
+    // Every value is read back from the buffer before the first (predicated)
+    // store: a read inside each store's branch would wait for the LDS before
+    // every store instead of once.
     if constexpr (E <= 16) {
 #pragma unroll
       for (int e = 0; e < E; ++e) lds[l * (E + 1) + e] = v[e];
       wave_sync();
+      float t[E];
 #pragma unroll
       for (int j = 0; j < E; ++j) {
         const int u = j * kLanes + l;
-        st_one<E>(p, pos0, g0, lo, hi, t0, t1, u, lds[(u / E) * (E + 1) + (u % E)], rs, nr);
+        t[j] = lds[(u / E) * (E + 1) + (u % E)];
       }
+#pragma unroll
+      for (int j = 0; j < E; ++j) st_one<E>(p, pos0, g0, lo, hi, t0, t1, j * kLanes + l, t[j], rs, nr);
       wave_sync();
     } else {
       static_assert(E == 32 || E == 64, "register chunks hold at most 64 samples per lane");
@@ -439,11 +445,14 @@ struct LaneGpuT {
           for (int e = 0; e < E; ++e) lds[(l - h * P) * (E + 1) + e] = v[e];
         }
         wave_sync();
+        float t[16];
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
           const int u = (16 * h + j) * kLanes + l;
-          st_one<E>(p, pos0, g0, lo, hi, t0, t1, u, lds[(u / E - h * P) * (E + 1) + (u % E)], rs, nr);
+          t[j] = lds[(u / E - h * P) * (E + 1) + (u % E)];
         }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) st_one<E>(p, pos0, g0, lo, hi, t0, t1, (16 * h + j) * kLanes + l, t[j], rs, nr);
         wave_sync();
       }
     }
