@@ -518,6 +518,26 @@ def test_across_strided_axes(gpu, oracle, across):
         _across_cases(gpu, oracle)
 
 
+@pytest.mark.parametrize("w", [w for w in WAVELETS if w != 0])
+def test_across_and_two_axis_every_wavelet(gpu, oracle, w):
+    """Every table wavelet through the two kernels that float32 ND calls take
+    by default under the Zero boundary: the across-line kernel (a strided axis
+    of 130 samples read where it lies, 48 lines along the contiguous dim) and
+    the two-axis level kernel (level by level over the last two axes, the
+    second contiguous), all four transforms, bit for bit against the oracle."""
+    import lifted_amd as la
+    g = rng(1000 + w)
+    x = g.standard_normal((130, 3, 48)).astype(np.float32)
+    y = g.standard_normal((2, 96, 160)).astype(np.float32)
+    for tf in (F, I, FA, IA):
+        ref = oracle.lwt(x, w, tf, 0, axes=[0], level=[3])
+        got = la.lwt(_gpu(x, gpu), w, tf, axes=[0], level=[3]).cpu().numpy()
+        _check(got, ref, np.float32, f"across w={w} tf={tf}", axes=[0], level=[3])
+        ref = oracle.lwt(y, w, tf, 0, axes=[1, 2], level=2)
+        got = la.lwt(_gpu(y, gpu), w, tf, axes=[1, 2], level=2).cpu().numpy()
+        _check(got, ref, np.float32, f"two-axis w={w} tf={tf}", axes=[1, 2], level=2)
+
+
 def _across_cases(gpu, oracle):
     import torch
     import lifted_amd as la
