@@ -226,7 +226,8 @@ typedef struct {
   int32_t across;          /* ND calls: strided float32 axes of at most 512 samples (Zero boundary) read in
                               place by the across-line kernel instead of transposed; 0 = off, 1 = on,
                               -1 = default (on) */
-  int32_t pad;
+  int32_t strip;           /* register window (strip) of float32 plans in samples: 1024 or 2048,
+                              0 = default (2048) */
 } lfx_tuning;
 int lfx_set_tuning_ex(const lfx_tuning* t);
 int lfx_get_tuning(lfx_tuning* t);

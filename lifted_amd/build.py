@@ -24,15 +24,17 @@ COMMON = [
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 # (source, device code, object name, defines): the kernel families compile once
-# per transform type (LFX_TF; the two-axis level kernels also per storage
-# type, LFX_ST), so that their wavelet instantiations build in parallel
+# per transform type (LFX_TF; the register windows also per samples per lane,
+# LFX_PL, the two-axis level kernels per storage type, LFX_ST), so that their
+# wavelet instantiations build in parallel
 SOURCES = [
     ("lfx_kernels.hip", True, "lfx_kernels", []),
     ("lfx_noalloc.hip", True, "lfx_noalloc", []),
 ]
 for _tf in range(4):
     SOURCES += [
-        ("lfx_reg_kernels.hip", True, f"lfx_reg_tf{_tf}", [f"-DLFX_TF={_tf}", "-DLFX_ROWS=0"]),
+        ("lfx_reg_kernels.hip", True, f"lfx_reg_tf{_tf}", [f"-DLFX_TF={_tf}", "-DLFX_ROWS=0", "-DLFX_PL=32"]),
+        ("lfx_reg_kernels.hip", True, f"lfx_reg_tf{_tf}_pl16", [f"-DLFX_TF={_tf}", "-DLFX_ROWS=0", "-DLFX_PL=16"]),
         ("lfx_reg_kernels.hip", True, f"lfx_rows_tf{_tf}", [f"-DLFX_TF={_tf}", "-DLFX_ROWS=1"]),
         ("lfx_across_kernels.hip", True, f"lfx_across_tf{_tf}", [f"-DLFX_TF={_tf}"]),
     ]

@@ -256,6 +256,7 @@ struct Tuning {
   int nd_batch = -1;     // ND lines through batched plans (0: off)
   long long small = -1;  // LDS bytes up to which batch items run as one-wave workgroups (-1: default)
   int across = -1;       // strided axes through the across-line kernel (0: off; -1: default, on)
+  int strip = 0;         // register window of float32 plans in samples (0: default kRegWindow; 1024, 2048)
 };
 static Tuning env_tuning() {
   Tuning t;
@@ -266,6 +267,7 @@ static Tuning env_tuning() {
   if (const char* e = std::getenv("LFX_SMALL")) t.small = std::atoll(e);
   if (const char* e = std::getenv("LFX_ND_BATCH")) t.nd_batch = std::atoi(e);
   if (const char* e = std::getenv("LFX_ACROSS")) t.across = std::atoi(e);
+  if (const char* e = std::getenv("LFX_STRIP")) t.strip = std::atoi(e);
   return t;
 }
 static thread_local Tuning g_tuning = env_tuning();
@@ -733,6 +735,7 @@ struct BatchPlanImpl {
   void* d_scr = nullptr;
   size_t scr_bytes = 0;
   void* d_progs = nullptr;  // DProgram<float> table of the register items
+  int pl = kRegPerLane;      // samples per lane of its register windows
   // side streams: the launches of one stage are independent (disjoint owned
   // outputs), so they are spread over the caller's stream and these, joined
   // by events before the next stage
@@ -801,6 +804,7 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
   plan->bc = bc;
   plan->storage = storage;
   plan->fwd = tf == kForward || tf == kInverseAdjoint;
+  plan->pl = g_tuning.strip == 64 * kRegPerLaneSmall ? kRegPerLaneSmall : kRegPerLane;
   // plain: one level per group (one launch per level, the band round-tripping
   // through memory between levels); register tiles still run each level
   // unless LFX_FLAG_LDS_ONLY
@@ -862,7 +866,7 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
     auto items_of = [&](const Group& gr) {
       const PassPlan pp{gr.m0, gr.k, gr.T, gr.H, gr.ntiles};
       return cs == 8 ? pass_items(hpd, plan->fwd, pp, false, storage, bc)
-                     : pass_items(hpf, plan->fwd, pp, reg, storage, bc);
+                     : pass_items(hpf, plan->fwd, pp, reg, storage, bc, 64ll * plan->pl);
     };
     const int G = (int)gs.size();
     // where each group's output band goes: user output or a scratch region
@@ -1049,7 +1053,7 @@ static BatchPlanImpl* batch_create(const lfx_signal* sigs, int nsig, int tf, int
       double kept = 0;
       for (size_t j = 0; j < reg_stage[st].size(); ++j) {
         const BatchItem& it = reg_stage[st][j];
-        const bool row = nrep >= 4 && bc == kZero && it.a == 0 && it.b == it.m0 && it.m0 <= kRegWindow / 4;
+        const bool row = nrep >= 4 && bc == kZero && it.a == 0 && it.b == it.m0 && it.m0 <= kRowMax;
         if (!row) {
           keep.push_back(it);
           kept += reg_ibytes[st][j];
@@ -1223,7 +1227,8 @@ static void batch_stage(BatchPlanImpl* p, int st, const void* in, void* out, hip
   }
   if (bs.nreg > 0) {
     mark(true, s);
-    hip_check(launch_reg(p->tf, p->d_progs, p->d_items + bs.reg0, bs.nreg, in, out, p->d_scr, s), "register kernel");
+    hip_check(launch_reg(p->tf, p->pl, p->d_progs, p->d_items + bs.reg0, bs.nreg, in, out, p->d_scr, s),
+              "register kernel");
     mark(false, s);
   }
   if (bs.nrows > 0) {
@@ -1431,14 +1436,15 @@ struct LinePlanKey {
   uint32_t wavelet;
   int tf, bc;
   long long tile, line, small;
-  int k, streams;
+  int k, streams, strip;
   static LinePlanKey make(int dev, long long nlines, long long n, long long L, uint32_t wavelet, int tf, int bc,
                           const Tuning& t) {
-    return LinePlanKey{dev, nlines, n, L, wavelet, tf, bc, t.tile, t.line, t.small, t.k, t.streams};
+    return LinePlanKey{dev, nlines, n, L, wavelet, tf, bc, t.tile, t.line, t.small, t.k, t.streams, t.strip};
   }
   bool operator<(const LinePlanKey& o) const {
-    return std::tie(dev, nlines, n, L, wavelet, tf, bc, tile, line, small, k, streams) <
-           std::tie(o.dev, o.nlines, o.n, o.L, o.wavelet, o.tf, o.bc, o.tile, o.line, o.small, o.k, o.streams);
+    return std::tie(dev, nlines, n, L, wavelet, tf, bc, tile, line, small, k, streams, strip) <
+           std::tie(o.dev, o.nlines, o.n, o.L, o.wavelet, o.tf, o.bc, o.tile, o.line, o.small, o.k, o.streams,
+                    o.strip);
   }
 };
 struct CachedPlan {
@@ -2070,11 +2076,14 @@ int lfx_set_tuning_ex(const lfx_tuning* t) {
     if (t->side_streams < -1 || t->side_streams > 8) fail(LFX_EINVAL, "side_streams must be in [-1, 8]");
     if (t->line_limit < 0 || (t->line_limit > 0 && t->line_limit < 16)) fail(LFX_EINVAL, "line_limit must be 0 or >= 16");
     if (t->across < -1 || t->across > 1) fail(LFX_EINVAL, "across must be -1, 0 or 1");
+    if (t->strip != 0 && t->strip != 64 * kRegPerLaneSmall && t->strip != kRegWindow)
+      fail(LFX_EINVAL, "strip must be 0, 1024 or 2048");
     g_tuning.tile = t->tile;
     g_tuning.k = t->levels_per_pass;
     g_tuning.streams = t->side_streams;
     g_tuning.line = t->line_limit;
     g_tuning.across = t->across;
+    g_tuning.strip = t->strip;
   });
 }
 
@@ -2086,7 +2095,7 @@ int lfx_get_tuning(lfx_tuning* t) {
     t->side_streams = g_tuning.streams;
     t->line_limit = g_tuning.line;
     t->across = g_tuning.across;
-    t->pad = 0;
+    t->strip = g_tuning.strip;
   });
 }
 

@@ -28,10 +28,15 @@ constexpr size_t kMaxLdsBytes = 160 * 1024 - kProgLdsBytes;
 // Register tiles: one wavefront holds a window of kRegWindow samples,
 // kRegPerLane consecutive ones per lane, and runs up to kRegLevels levels;
 // reverse tiles count their window in level-kRegLevels units (kRegUnits).
+// Plans may take windows of kRegPerLaneSmall samples per lane instead (the
+// `strip` tuning knob). Row mode holds one band of at most kRowMax samples per
+// 16-lane row.
 constexpr int kRegLevels = 4;
 constexpr int kRegPerLane = 32;
+constexpr int kRegPerLaneSmall = 16;
 constexpr long long kRegWindow = 64ll * kRegPerLane;
 constexpr long long kRegUnits = kRegWindow >> kRegLevels;
+constexpr long long kRowMax = 16ll * kRegPerLane;
 // Wavefronts per workgroup of the register-tile launches (each wave runs its
 // own item; workgroups only group waves for the dispatcher).
 constexpr int kRegWaves = 1;

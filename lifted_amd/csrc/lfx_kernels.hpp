@@ -17,23 +17,32 @@ hipError_t launch_gline(const LineLaunch& ll, void* gscr, long long lines_per_la
 // threads: workgroup size (0: the default of block_threads())
 hipError_t launch_batch(int storage, bool fwd, int bc, const BatchItem* items, long long nitems, size_t lds_bytes,
                         int threads, const void* in, void* out, void* scr, hipStream_t st);
-// Register-tile items (float32) of one transform type, any table wavelet;
+// Register-tile items (float32) of one transform type, any table wavelet, in
+// windows of pl (16 or 32) samples per lane: launch_reg_<transform>_<pl>;
 // progs: the plan's device table of DProgram<float>, indexed by BatchItem::pidx.
-hipError_t launch_reg_0(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
-                        void* scr, hipStream_t st);
-hipError_t launch_reg_1(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
-                        void* scr, hipStream_t st);
-hipError_t launch_reg_2(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
-                        void* scr, hipStream_t st);
-hipError_t launch_reg_3(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
-                        void* scr, hipStream_t st);
-inline hipError_t launch_reg(int tf, const void* progs, const BatchItem* items, long long nitems, const void* in,
-                             void* out, void* scr, hipStream_t st) {
-  switch (tf) {
-    case 0: return launch_reg_0(progs, items, nitems, in, out, scr, st);
-    case 1: return launch_reg_1(progs, items, nitems, in, out, scr, st);
-    case 2: return launch_reg_2(progs, items, nitems, in, out, scr, st);
-    case 3: return launch_reg_3(progs, items, nitems, in, out, scr, st);
+#define LFX_REG_DECL(tf, pl)                                                                               \
+  hipError_t launch_reg_##tf##_##pl(const void* progs, const BatchItem* items, long long nitems, const void* in, \
+                                    void* out, void* scr, hipStream_t st);
+LFX_REG_DECL(0, 16)
+LFX_REG_DECL(1, 16)
+LFX_REG_DECL(2, 16)
+LFX_REG_DECL(3, 16)
+LFX_REG_DECL(0, 32)
+LFX_REG_DECL(1, 32)
+LFX_REG_DECL(2, 32)
+LFX_REG_DECL(3, 32)
+#undef LFX_REG_DECL
+inline hipError_t launch_reg(int tf, int pl, const void* progs, const BatchItem* items, long long nitems,
+                             const void* in, void* out, void* scr, hipStream_t st) {
+  switch (tf * 100 + pl) {
+    case 16: return launch_reg_0_16(progs, items, nitems, in, out, scr, st);
+    case 116: return launch_reg_1_16(progs, items, nitems, in, out, scr, st);
+    case 216: return launch_reg_2_16(progs, items, nitems, in, out, scr, st);
+    case 316: return launch_reg_3_16(progs, items, nitems, in, out, scr, st);
+    case 32: return launch_reg_0_32(progs, items, nitems, in, out, scr, st);
+    case 132: return launch_reg_1_32(progs, items, nitems, in, out, scr, st);
+    case 232: return launch_reg_2_32(progs, items, nitems, in, out, scr, st);
+    case 332: return launch_reg_3_32(progs, items, nitems, in, out, scr, st);
   }
   return hipErrorInvalidValue;
 }

@@ -2,10 +2,11 @@
 // levels in registers; LDS serves only to transpose global loads and stores
 // into coalesced ones (no workgroup barriers).
 //
-// Each of the 64 lanes holds kPerLane (32) consecutive samples of the tile's
-// window (kWindow = 2048 samples). Level 1 splits them into 16 s and 16 d
-// values per lane (s[e] = x[2e], d[e] = x[2e+1]); level l + 1 splits s_l the
-// same way, so after four levels each lane holds two s_4 and two d_4. A
+// Each of the 64 lanes holds PL consecutive samples of the tile's window (the
+// tile functions' template parameter: 32 by default, a 2048-sample window; 16
+// for 1024-sample windows). Level 1 splits them into PL/2 s and PL/2 d values
+// per lane (s[e] = x[2e], d[e] = x[2e+1]); level l + 1 splits s_l the same
+// way, so after four levels each lane holds PL/16 s_4 and PL/16 d_4. A
 // lifting step
 //     y[e] += sum_k c[k] * x[e + off + k]
 // reads x values past the lane's own chunk from neighbouring lanes (DPP shifts
@@ -41,11 +42,8 @@ namespace reg {
 
 constexpr int kLanes = 64;
 constexpr int kLevels = kRegLevels;           // levels per register tile
-constexpr int kPerLane = kRegPerLane;         // samples per lane at the finest level
-constexpr int kWindow = (int)kRegWindow;      // kLanes * kPerLane samples
-constexpr int kUnits = (int)kRegUnits;        // the window in level-4 units
-static_assert(kLevels == 4 && kWindow == kLanes * kPerLane && (kPerLane == 16 || kPerLane == 32 || kPerLane == 64),
-              "register tiles run four levels on 16, 32 or 64 samples per lane");
+constexpr int kPerLane = kRegPerLane;         // default samples per lane at the finest level
+static_assert(kLevels == 4 && kRegWindow == kLanes * kPerLane, "register tiles run four levels");
 
 // Lane policies. ROWS = 1: the wavefront holds one window of a band. ROWS = 4
 // ("row mode"): it holds four equal lines of at most 512 samples, one per row
@@ -605,27 +603,28 @@ struct RowArgs {
 
 // Forward tile (Forward / InverseAdjoint) of K (1..4) levels: owned [a, b)
 // of the input band of length m0 (b == m0 for the last tile), window
-// [w0, w0 + kWindow) with w0 = a - H (a multiple of 16). Lane l holds the
-// kPerLane samples w0 + kPerLane l + e; after level l it holds kPerLane / 2^l
+// [w0, w0 + 64 PL) with w0 = a - H (a multiple of 16). Lane l holds the
+// PL samples w0 + PL l + e; after level l it holds PL / 2^l
 // values of s_l and of d_l. Writes d_l of the owned range to out[ns_l + i]
 // (l = 1..K) and s_K to sout[i]. A window inside the band reads no boundary;
 // one that overlaps a band end (Zero boundary only) runs masked. K is uniform
 // over the wavefront.
-template <class P, class W, int TF, class SI, class SO, class SK>
+template <class P, class W, int TF, int PL = kPerLane, class SI, class SO, class SK>
 __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long m0, int K, long long a, long long b,
                                          long long H, const SI* in, SO* out, SK* sout, float* lds,
                                          const RowArgs& ra = RowArgs{{0, 0, 0}, 1}) {
   using V = typename P::V;
   using M = typename P::M;
-  constexpr int E1 = kPerLane / 2, E2 = kPerLane / 4, E3 = kPerLane / 8, E4 = kPerLane / 16;
+  static_assert(PL == 16 || PL == 32 || PL == 64, "register windows of 16, 32 or 64 samples per lane");
+  constexpr int E1 = PL / 2, E2 = PL / 4, E3 = PL / 8, E4 = PL / 16;
   static_assert(P::kRows == 1 || E4 > 1, "row mode stores level-4 bands through the chunk transposes");
   const long long w0 = a - H;
-  const bool masked = P::kRows > 1 || w0 < 0 || w0 + kWindow > m0;
+  const bool masked = P::kRows > 1 || w0 < 0 || w0 + kLanes * PL > m0;
   constexpr int NC = wt::NCoef<typename W::list, TF>::value;
   float cf[NC > 0 ? NC : 1];
   P::template coefs<NC>(prog->coef, cf);
-  V x[kPerLane];
-  P::template ld_chunks<kPerLane>(in, w0, m0, x, lds, ra.rs[0], ra.n);
+  V x[PL];
+  P::template ld_chunks<PL>(in, w0, m0, x, lds, ra.rs[0], ra.n);
   V s1[E1], d1[E1], s2[E2], d2[E2], s3[E3], d3[E3], s4[E4], d4[E4];
   M ms{}, md{};
   long long m = m0, ns = m - m / 2, nd = m / 2;
@@ -692,29 +691,31 @@ __device__ __forceinline__ void fwd_tile(const DProgram<float>* prog, long long 
 // Reverse tile (Inverse / ForwardAdjoint) of K (1..4) levels: owned [a, b) of
 // the output band of length m0 (b == m0 for the last tile). Windows are
 // counted in level-4 units whatever K is: with g0 = (a >> 4) - Hk, the level-l
-// windows are [2^(4-l) g0, 2^(4-l) (g0 + kUnits)), kPerLane / 2^l values per
+// windows are [2^(4-l) g0, 2^(4-l) (g0 + 4 PL)), PL / 2^l values per
 // lane. Reads s_K from sin[i] and d_l from in[ns_l + i]; writes the owned band
 // values to out[q]. Windows that overlap a band end (Zero boundary only) run
 // masked. All global loads of the tile are issued first (the coarse levels
 // then compute while the fine bands are in flight); each band is transposed
 // to the chunk layout when its level runs.
-template <class P, class W, int TF, int K, class SI, class SK, class SO>
+template <class P, class W, int TF, int K, int PL, class SI, class SK, class SO>
 __device__ __forceinline__ void rev_tile_k(const DProgram<float>* prog, long long m0, long long a, long long b,
                                            long long Hk, const SI* in, const SK* sin, SO* out, float* lds,
                                            const RowArgs& ra) {
   using V = typename P::V;
   using M = typename P::M;
-  constexpr int E1 = kPerLane / 2, E2 = kPerLane / 4, E3 = kPerLane / 8, E4 = kPerLane / 16;
+  static_assert(PL == 16 || PL == 32 || PL == 64, "register windows of 16, 32 or 64 samples per lane");
+  constexpr int E1 = PL / 2, E2 = PL / 4, E3 = PL / 8, E4 = PL / 16;
+  constexpr int kU = (kLanes * PL) >> kLevels;  // the window in level-4 units
   static_assert(P::kRows == 1 || E4 > 1, "row mode loads level-4 bands through the chunk transposes");
   const long long g0 = (a >> 4) - Hk;
-  const bool masked = P::kRows > 1 || g0 < 0 || 16 * (g0 + kUnits) > m0;
+  const bool masked = P::kRows > 1 || g0 < 0 || 16 * (g0 + kU) > m0;
   const long long m1 = m0, m2 = m1 - m1 / 2, m3 = m2 - m2 / 2, m4 = m3 - m3 / 2;  // band split by level l
   const long long ns1 = m1 - m1 / 2, ns2 = m2 - m2 / 2, ns3 = m3 - m3 / 2, ns4 = m4 - m4 / 2;
   constexpr int NC = wt::NCoef<typename W::list, TF>::value;
   float cf[NC > 0 ? NC : 1];
   P::template coefs<NC>(prog->coef, cf);
   // raw (coalesced-order) loads: d_1 .. d_K and s_K
-  constexpr int EK = kPerLane >> K;  // values per lane of the level-K bands
+  constexpr int EK = PL >> K;  // values per lane of the level-K bands
   V rs[EK], r1[E1], r2[E2], r3[E3], r4[E4];
   if constexpr (K == 4) {
     if constexpr (E4 == 1) {
@@ -734,7 +735,7 @@ __device__ __forceinline__ void rev_tile_k(const DProgram<float>* prog, long lon
   if constexpr (K >= 3) P::template ld_raw<E3>(in + ns3, 2 * g0, m3 / 2, r3, ra.rs[0], ra.n);
   if constexpr (K >= 2) P::template ld_raw<E2>(in + ns2, 4 * g0, m2 / 2, r2, ra.rs[0], ra.n);
   P::template ld_raw<E1>(in + ns1, 8 * g0, m1 / 2, r1, ra.rs[0], ra.n);
-  V s3[E3], d3[E3], s2[E2], d2[E2], s1[E1], d1[E1], x[kPerLane];
+  V s3[E3], d3[E3], s2[E2], d2[E2], s1[E1], d1[E1], x[PL];
   M ms{}, md{};
   if constexpr (K == 4) {
     V s4[E4], d4[E4];
@@ -783,18 +784,18 @@ __device__ __forceinline__ void rev_tile_k(const DProgram<float>* prog, long lon
   }
   level<P, W, TF, E1>(s1, d1, cf, masked, ms, md);
   merge<P, E1>(s1, d1, x);
-  P::template st_chunks<kPerLane>(out, 0, x, 16 * g0, a, b >= m0 ? m0 : b, lds, ra.rs[1], ra.n);
+  P::template st_chunks<PL>(out, 0, x, 16 * g0, a, b >= m0 ? m0 : b, lds, ra.rs[1], ra.n);
 }
 
-template <class P, class W, int TF, class SI, class SK, class SO>
+template <class P, class W, int TF, int PL = kPerLane, class SI, class SK, class SO>
 __device__ __forceinline__ void rev_tile(const DProgram<float>* prog, long long m0, int K, long long a, long long b,
                                          long long Hk, const SI* in, const SK* sin, SO* out, float* lds,
                                          const RowArgs& ra = RowArgs{{0, 0, 0}, 1}) {
   switch (K) {
-    case 4: rev_tile_k<P, W, TF, 4>(prog, m0, a, b, Hk, in, sin, out, lds, ra); return;
-    case 3: rev_tile_k<P, W, TF, 3>(prog, m0, a, b, Hk, in, sin, out, lds, ra); return;
-    case 2: rev_tile_k<P, W, TF, 2>(prog, m0, a, b, Hk, in, sin, out, lds, ra); return;
-    default: rev_tile_k<P, W, TF, 1>(prog, m0, a, b, Hk, in, sin, out, lds, ra); return;
+    case 4: rev_tile_k<P, W, TF, 4, PL>(prog, m0, a, b, Hk, in, sin, out, lds, ra); return;
+    case 3: rev_tile_k<P, W, TF, 3, PL>(prog, m0, a, b, Hk, in, sin, out, lds, ra); return;
+    case 2: rev_tile_k<P, W, TF, 2, PL>(prog, m0, a, b, Hk, in, sin, out, lds, ra); return;
+    default: rev_tile_k<P, W, TF, 1, PL>(prog, m0, a, b, Hk, in, sin, out, lds, ra); return;
   }
 }
 

@@ -1,8 +1,8 @@
 // Register-tile kernels of one transform type. build.py compiles this file
 // once per transform type (LFX_TF = 0..3) and family (LFX_ROWS = 0: windows
-// of one line, 1: rows of four equal lines), so that the 34 wavelet
-// instantiations of each compile in parallel. One wavefront runs one
-// BatchItem of kind register tile.
+// of one line, of LFX_PL = 16 or 32 samples per lane; 1: rows of four equal
+// lines), so that the 34 wavelet instantiations of each compile in parallel.
+// One wavefront runs one BatchItem of kind register tile.
 #include <hip/hip_runtime.h>
 
 #include "lfx_kernels.hpp"
@@ -13,19 +13,21 @@ namespace lfx {
 
 constexpr int kStageFloats = 64 * 17;
 
-template <class W, int TF>
+template <class W, int TF, int PL>
 __device__ __forceinline__ void reg_item(const BatchItem& it, const DProgram<float>* prog, float* a0, float* a1,
                                          float* a2, float* lds) {
   if constexpr (TF == kForward || TF == kInverseAdjoint)
-    reg::fwd_tile<reg::LaneGpu, W, TF>(prog, it.m0, it.k, it.a, it.b, it.H, (const float*)a0, a1, a2, lds);
+    reg::fwd_tile<reg::LaneGpu, W, TF, PL>(prog, it.m0, it.k, it.a, it.b, it.H, (const float*)a0, a1, a2, lds);
   else
-    reg::rev_tile<reg::LaneGpu, W, TF>(prog, it.m0, it.k, it.a, it.b, it.H, (const float*)a0, (const float*)a2, a1, lds);
+    reg::rev_tile<reg::LaneGpu, W, TF, PL>(prog, it.m0, it.k, it.a, it.b, it.H, (const float*)a0, (const float*)a2,
+                                           a1, lds);
 }
 
 // All register-tile items of a stage, any table wavelet: the item's wavelet id
 // selects the compiled step shapes (a wave-uniform branch), its program index
-// the coefficients in the plan's program table.
-template <int TF>
+// the coefficients in the plan's program table. PL: samples per lane of the
+// plan's windows.
+template <int TF, int PL>
 __global__ void __launch_bounds__(64 * kRegWaves) k_reg(const BatchItem* items, long long nitems, const DProgram<float>* progs,
                                              const float* uin, float* uout, float* scr) {
   // per-wave transposition buffer: 64 rows of 17 floats (16 values per lane,
@@ -45,7 +47,7 @@ __global__ void __launch_bounds__(64 * kRegWaves) k_reg(const BatchItem* items, 
   switch (it.wavelet) {
 #define LFX_WAVELET(nm, idv, w)                                     \
   case idv:                                                         \
-    reg_item<wt::WL_##nm, TF>(it, prog, a0, a1, a2, lds);           \
+    reg_item<wt::WL_##nm, TF, PL>(it, prog, a0, a1, a2, lds);       \
     return;
 #define LFX_GENERIC(...)
 #define LFX_UNIT(...)
@@ -122,12 +124,12 @@ hipError_t launch_rows_tf(const void* progs, const BatchItem* items, long long n
   return hipGetLastError();
 }
 
-template <int TF>
+template <int TF, int PL>
 hipError_t launch_reg_tf(const void* progs, const BatchItem* items, long long nitems, const void* in, void* out,
                          void* scr, hipStream_t st) {
   if (nitems <= 0) return hipSuccess;
   const unsigned blocks = (unsigned)((nitems + kRegWaves - 1) / kRegWaves);
-  hipLaunchKernelGGL((k_reg<TF>), dim3(blocks), dim3(64 * kRegWaves), 0, st, items, nitems, (const DProgram<float>*)progs,
+  hipLaunchKernelGGL((k_reg<TF, PL>), dim3(blocks), dim3(64 * kRegWaves), 0, st, items, nitems, (const DProgram<float>*)progs,
                      (const float*)in, (float*)out, (float*)scr);
   return hipGetLastError();
 }
@@ -140,9 +142,10 @@ hipError_t LFX_JOIN(launch_rows_, LFX_TF)(const void* progs, const BatchItem* it
   return launch_rows_tf<LFX_TF>(progs, items, nitems, in, out, scr, st);
 }
 #else
-hipError_t LFX_JOIN(launch_reg_, LFX_TF)(const void* progs, const BatchItem* items, long long nitems, const void* in,
-                                         void* out, void* scr, hipStream_t st) {
-  return launch_reg_tf<LFX_TF>(progs, items, nitems, in, out, scr, st);
+hipError_t LFX_JOIN(LFX_JOIN(LFX_JOIN(launch_reg_, LFX_TF), _), LFX_PL)(const void* progs, const BatchItem* items,
+                                                                      long long nitems, const void* in, void* out,
+                                                                      void* scr, hipStream_t st) {
+  return launch_reg_tf<LFX_TF, LFX_PL>(progs, items, nitems, in, out, scr, st);
 }
 #endif
 

@@ -126,9 +126,10 @@ LineSchedule schedule(const DProgram<C>& hp, bool fwd, long long n, long long L,
 // Work items of one group of levels of one signal.
 //
 // A tiled group of one to four levels of a float32 table wavelet runs its
-// interior as register tiles (one wavefront per kRegWindow-sample window,
-// lfx_reg.hpp) and the parts next to the band ends as LDS tiles; other groups
-// are LDS tiles of the schedule's tile length, or one whole-line item.
+// interior as register tiles (one wavefront per window of `win` samples,
+// kRegWindow by default, lfx_reg.hpp) and the parts next to the band ends as
+// LDS tiles; other groups are LDS tiles of the schedule's tile length, or one
+// whole-line item.
 
 enum ItemKind { kSpecLine = 0, kSpecTile = 1, kSpecReg = 2 };
 struct ItemSpec {
@@ -138,19 +139,19 @@ struct ItemSpec {
   long long a, b, H;
 };
 
-// Halo of an interior register tile of K levels: forward H (finest-level
-// samples, multiple of 16, owned kRegWindow - 2H); reverse Hk (level-4 units
-// whatever K is, owned 16 (kRegUnits - 2 Hk)). -1 when the stencil is too wide
-// for the register window to pay off.
+// Halo of an interior register tile of K levels in a window of `win` samples
+// (units = win / 16): forward H (finest-level samples, multiple of 16, owned
+// win - 2H); reverse Hk (level-4 units whatever K is, owned 16 (units - 2 Hk)).
+// -1 when the stencil is too wide for the register window to pay off.
 template <class C>
-long long reg_halo(const DProgram<C>& hp, bool fwd, int K, int bc) {
-  const long long m0 = 1ll << 40, a0 = 1ll << 36;
+long long reg_halo(const DProgram<C>& hp, bool fwd, int K, int bc, long long win = kRegWindow) {
+  const long long m0 = 1ll << 40, a0 = 1ll << 36, units = win >> kRegLevels;
   if (fwd) {
-    for (long long H = 0; 2 * H <= kRegWindow - 256; H += 16)
-      if (fwd_tile_ok(hp, m0, K, a0, a0 + kRegWindow - 2 * H, H, bc)) return H;
+    for (long long H = 0; 2 * H <= win - 256; H += 16)
+      if (fwd_tile_ok(hp, m0, K, a0, a0 + win - 2 * H, H, bc)) return H;
   } else {
-    for (long long Hk = 0; 16 * (kRegUnits - 2 * Hk) >= 256; ++Hk)
-      if (rev_tile_ok(hp, m0, K, a0, a0 + 16 * (kRegUnits - 2 * Hk), Hk << (kRegLevels - K), bc)) return Hk;
+    for (long long Hk = 0; 16 * (units - 2 * Hk) >= 256; ++Hk)
+      if (rev_tile_ok(hp, m0, K, a0, a0 + 16 * (units - 2 * Hk), Hk << (kRegLevels - K), bc)) return Hk;
   }
   return -1;
 }
@@ -184,18 +185,19 @@ long long lds_halo(const DProgram<C>& hp, bool fwd, long long m0, int k, long lo
   return -1;
 }
 
-// Items of a tiled pass (band m0, k levels). `reg` allows register tiles.
+// Items of a tiled pass (band m0, k levels). `reg` allows register tiles, of
+// windows of `win` samples.
 template <class C>
 std::vector<ItemSpec> pass_items(const DProgram<C>& hp, bool fwd, const PassPlan& pp, bool reg, int storage,
-                                 int bc) {
+                                 int bc, long long win = kRegWindow) {
   std::vector<ItemSpec> out;
-  const long long m0 = pp.m0;
+  const long long m0 = pp.m0, units = win >> kRegLevels;
   long long r0 = 0, r1 = 0, Tr = 0, H = -1;
-  if (reg && pp.k >= 1 && pp.k <= kRegLevels) H = reg_halo(hp, fwd, pp.k, bc);
+  if (reg && pp.k >= 1 && pp.k <= kRegLevels) H = reg_halo(hp, fwd, pp.k, bc, win);
   if (H >= 0 && bc == kZero && zero_cover_ok(hp, m0, pp.k)) {
     // Zero boundary: register windows may overlap the band ends (they run
     // masked, reading zeros there), so register tiles cover the whole band
-    const long long Tr = fwd ? kRegWindow - 2 * H : 16 * (kRegUnits - 2 * H);
+    const long long Tr = fwd ? win - 2 * H : 16 * (units - 2 * H);
     const long long nt = (m0 + Tr - 1) / Tr;
     auto ok = [&](long long t) {
       const long long a = t * Tr, b = std::min(a + Tr, m0);
@@ -215,23 +217,23 @@ std::vector<ItemSpec> pass_items(const DProgram<C>& hp, bool fwd, const PassPlan
     // Register windows stay strictly inside the band at every level (they
     // apply no boundary map), with a margin of 64 samples at the finest level.
     if (fwd) {
-      Tr = kRegWindow - 2 * H;
+      Tr = win - 2 * H;
       r0 = H + 64;
-      const long long last = m0 - kRegWindow + H - 64;  // window [a - H, a - H + kRegWindow)
+      const long long last = m0 - win + H - 64;  // window [a - H, a - H + win)
       if (last >= r0) r1 = r0 + ((last - r0) / Tr + 1) * Tr;
     } else {
-      Tr = 16 * (kRegUnits - 2 * H);
+      Tr = 16 * (units - 2 * H);
       r0 = 16 * (H + 4);
-      // every level's window [2^(4-l) g0, 2^(4-l) (g0 + kRegUnits)) inside its d band, and s_k's inside ns_k
+      // every level's window [2^(4-l) g0, 2^(4-l) (g0 + units)) inside its d band, and s_k's inside ns_k
       long long gmax = -1;
       long long m = m0;
       bool ok = true;
       for (int l = 1; l <= pp.k; ++l) {
         const long long nd = m / 2, ns = m - nd;
         const long long span = 1ll << (kRegLevels - l);
-        const long long g = nd / span - (kRegUnits + 4);  // g0 + kRegUnits < nd / span, with a margin
+        const long long g = nd / span - (units + 4);  // g0 + units < nd / span, with a margin
         gmax = (gmax < 0 || g < gmax) ? g : gmax;
-        if (l == pp.k) gmax = std::min(gmax, ns / span - (kRegUnits + 4));
+        if (l == pp.k) gmax = std::min(gmax, ns / span - (units + 4));
         m = ns;
         if (g < 0) ok = false;
       }

@@ -13,6 +13,8 @@ when the plan is created (lfx_set_tuning_ex in the C ABI):
     across           ND calls: strided float32 axes of at most 512 samples
                      read in place by the across-line kernel (0 off, 1 on,
                      -1 default: on)
+    strip            register window (strip) of float32 plans in samples:
+                     1024 or 2048 (0: default, 2048)
 
 tune_batch() builds the plans of a signal list for every candidate setting,
 times them on the device with the caller's data, and returns the fastest.
@@ -25,12 +27,12 @@ import time
 
 from . import _lib
 
-FIELDS = ("tile", "levels_per_pass", "side_streams", "line_limit", "across")
+FIELDS = ("tile", "levels_per_pass", "side_streams", "line_limit", "across", "strip")
 
 
 class LfxTuning(ctypes.Structure):
     _fields_ = [("tile", ctypes.c_int64), ("levels_per_pass", ctypes.c_int32), ("side_streams", ctypes.c_int32),
-                ("line_limit", ctypes.c_int64), ("across", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("line_limit", ctypes.c_int64), ("across", ctypes.c_int32), ("strip", ctypes.c_int32)]
 
 
 def _bind():
@@ -51,9 +53,9 @@ def get_tuning():
     return {f: getattr(t, f) for f in FIELDS}
 
 
-def set_tuning(tile=0, levels_per_pass=0, side_streams=-1, line_limit=0, across=-1):
+def set_tuning(tile=0, levels_per_pass=0, side_streams=-1, line_limit=0, across=-1, strip=0):
     """Planning knobs for plans created afterwards on this thread."""
-    t = LfxTuning(int(tile), int(levels_per_pass), int(side_streams), int(line_limit), int(across), 0)
+    t = LfxTuning(int(tile), int(levels_per_pass), int(side_streams), int(line_limit), int(across), int(strip))
     _lib.check(_bind().lfx_set_tuning_ex(ctypes.byref(t)))
 
 
@@ -61,7 +63,7 @@ def set_tuning(tile=0, levels_per_pass=0, side_streams=-1, line_limit=0, across=
 def tuning(**knobs):
     """Temporarily set planning knobs (unnamed ones keep their defaults)."""
     old = get_tuning()
-    cfg = {"tile": 0, "levels_per_pass": 0, "side_streams": -1, "line_limit": 0, "across": -1}
+    cfg = {"tile": 0, "levels_per_pass": 0, "side_streams": -1, "line_limit": 0, "across": -1, "strip": 0}
     cfg.update(knobs)
     set_tuning(**cfg)
     try:
@@ -81,8 +83,8 @@ def default_space(dtype, chain=False):
             space = {"chunks": [2, 1], "tile": [0, 2048, 8192], "levels_per_pass": [0, 4]}
         return space
     if chain:
-        return {"chunks": [2, 1, 3], "levels_per_pass": [0, 3], "line_limit": [0, 32]}
-    return {"levels_per_pass": [0, 3], "line_limit": [0, 32, 1024], "side_streams": [-1, 0]}
+        return {"chunks": [2, 1, 3], "levels_per_pass": [0, 3], "line_limit": [0, 32], "strip": [0, 1024]}
+    return {"levels_per_pass": [0, 3], "line_limit": [0, 32, 1024], "side_streams": [-1, 0], "strip": [0, 1024]}
 
 
 def _time_plans(plans, x, bufs, reps, warmup):

@@ -169,7 +169,7 @@ static void run_rows(const WaveletDefL& w, int tf, int bc, long long n, long lon
     if (!ok) fail(LFX_EINVAL, "no row kernel for this wavelet");
   };
   auto row_ok = [&](const ItemSpec& it) {
-    return it.kind == kSpecReg && bc == kZero && it.a == 0 && it.b == it.m0 && it.m0 <= kRegWindow / 4;
+    return it.kind == kSpecReg && bc == kZero && it.a == 0 && it.b == it.m0 && it.m0 <= kRowMax;
   };
   auto line_run = [&](bool f, long long m, long long Ll, const float* src, const float* ssrc, float* dst) {
     std::vector<float> lds(line_lds_bytes(m, 4) / 4);

@@ -168,13 +168,16 @@ def test_tuning_knobs_round_trip_and_validation():
     base = tune.get_tuning()
     with tune.tuning(levels_per_pass=3, line_limit=1024, side_streams=0):
         assert tune.get_tuning() == {"tile": 0, "levels_per_pass": 3, "side_streams": 0, "line_limit": 1024,
-                                     "across": -1}
+                                     "across": -1, "strip": 0}
     for v in (0, 1):
         with tune.tuning(across=v):
             assert tune.get_tuning()["across"] == v
+    for v in (1024, 2048):
+        with tune.tuning(strip=v):
+            assert tune.get_tuning()["strip"] == v
     assert tune.get_tuning() == base
     for bad in ({"tile": 8}, {"levels_per_pass": 13}, {"side_streams": 9}, {"line_limit": 5}, {"across": 2},
-                {"across": -2}):
+                {"across": -2}, {"strip": 512}, {"strip": 4096}):
         with pytest.raises(LfxError):
             tune.set_tuning(**bad)
     assert tune.get_tuning() == base

@@ -27,18 +27,21 @@ def make_signals(seed, nsig, lo, hi, oracle):
     return sigs, off
 
 
-@pytest.mark.parametrize("variant", ["fused", "plain", "plain_lds"])
+@pytest.mark.parametrize("variant", ["fused", "fused_strip1024", "plain", "plain_lds"])
 @pytest.mark.parametrize("tf", [0, 1, 2, 3])
 def test_batch_matches_oracle(gpu, oracle, tf, variant):
-    """Fused plans, plain plans (one launch per level, register tiles) and
-    plain LDS-only plans against the oracle per signal, bit for bit."""
+    """Fused plans (register windows of 2048 samples, and of 1024 with the
+    strip knob), plain plans (one launch per level, register tiles) and plain
+    LDS-only plans against the oracle per signal, bit for bit."""
     import torch
     from lifted_amd.batch import BatchPlan
-    plain = variant != "fused"
+    from lifted_amd.tune import tuning
+    plain = variant.startswith("plain")
     sigs, total = make_signals(10 + tf, 40, 16, 300000, oracle)
     for bc in (0, 3):
         x = np.random.default_rng(1).standard_normal(total).astype(np.float32)
-        plan = BatchPlan(sigs, tf, mode=bc, dtype=torch.float32, plain=plain, lds_only=variant == "plain_lds")
+        with tuning(strip=1024 if variant == "fused_strip1024" else 0):
+            plan = BatchPlan(sigs, tf, mode=bc, dtype=torch.float32, plain=plain, lds_only=variant == "plain_lds")
         if plain:
             assert plan.num_stages >= max(L for _, _, _, L in sigs)
         y = plan.execute(torch.from_numpy(x).to(gpu)).cpu().numpy()
@@ -159,7 +162,8 @@ def test_autotuned_plans_match_oracle(gpu, oracle):
     xn = np.random.default_rng(5).standard_normal(total).astype(np.float32)
     x = torch.from_numpy(xn).to(gpu)
     best, tried = tune_batch(sigs, (0, 2), x, reps=1, warmup=1)
-    assert tried[0][0] == {"levels_per_pass": 0, "line_limit": 0, "side_streams": -1}
+    assert tried[0][0] == {"levels_per_pass": 0, "line_limit": 0, "side_streams": -1, "strip": 0}
+    assert {k["strip"] for k, _ in tried} == {0, 1024}
     assert best in [k for k, _ in tried]
     for knobs, _ in tried:
         for tf in (0, 2):
