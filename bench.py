@@ -51,9 +51,9 @@ HBM_PEAK = 8.0e12
 SEED = 20261201 + 3  # fourth workload of the benchmark set
 NSIG = 4096
 CAP = 1 << 30
-KIND_NAMES = {  # (plan transform, launch kind) -> kernel symbol prefix
-    (0, "register"): "void lfx::k_reg<0>(",
-    (2, "register"): "void lfx::k_reg<2>(",
+KIND_NAMES = {  # (plan transform, launch kind) -> kernel symbol prefix ({pl}: samples per lane)
+    (0, "register"): "void lfx::k_reg<0, {pl}>(",
+    (2, "register"): "void lfx::k_reg<2, {pl}>(",
     (0, "lds"): "void lfx::k_batch<float, float, true>(",
     (2, "lds"): "void lfx::k_batch<float, float, false>(",
 }
@@ -172,11 +172,12 @@ def time_chain(chain, x, bufs, steps, warmup):
     return t0.elapsed_time(t1) / steps, launches
 
 
-def chain_dominant(chain, launches):
+def chain_dominant(chain, launches, strip=0):
     """The kernel (transform, launch kind) with the most time per step, over
     its big-stage launches (the finest levels of every signal: most of the
     kernel's bytes; its coarse-level launches run beside the other chunk's big
-    stage and are listed separately in `launches`)."""
+    stage and are listed separately in `launches`). strip: the plans'
+    register window (0: the default, 2048 samples)."""
     by = {}
     for r in launches:
         key = (chain.transforms[r["transform_index"]], r["kind"])
@@ -187,7 +188,8 @@ def chain_dominant(chain, launches):
     if not big:
         big = [r for r in launches if (chain.transforms[r["transform_index"]], r["kind"]) == key]
     n = len(big)
-    return {"symbol": KIND_NAMES.get(key, str(key)), "launches_per_step": n,
+    pl = 16 if strip == 1024 else 32
+    return {"symbol": KIND_NAMES.get(key, str(key)).format(pl=pl), "launches_per_step": n,
             "alg_bytes_per_launch": sum(r["alg_bytes"] for r in big) / n,
             "avg_launch_ms": sum(r["ms"] for r in big) / n, "items": [r["items"] for r in big],
             "ms_per_step_all_launches": by[key]["ms"]}
@@ -650,7 +652,7 @@ def main():
           f"chunks built in {t_plan:.1f}s", file=sys.stderr, flush=True)
     # eager chain steps with HIP events around every launch (per-kernel roofline)
     eager_ms, launches = time_chain(chain, x, bufs, args.steps, args.warmup)
-    dom = chain_dominant(chain, launches)
+    dom = chain_dominant(chain, launches, best.get("strip", 0))
     achieved = dom["alg_bytes_per_launch"] / (dom["avg_launch_ms"] * 1e-3)
     print(f"[bench] eager: {eager_ms:.3f} ms/step, {2 * total / (eager_ms * 1e-3) / 1e9:.2f} Gsamples/s; dominant "
           f"{dom['symbol']} big stages {dom['avg_launch_ms']:.3f} ms/launch, {achieved / 1e9:.0f} GB/s", file=sys.stderr,
