@@ -11,9 +11,9 @@
 //
 // Slab layout: line c, sample p at virtual float index c * kMaxN + p, stored
 // at phys(index) = index + index / 32 + index / 512. The 32-sample chunks of
-// a lane then start in distinct banks (reads and writes of the row tiles are
-// conflict-free), and the lines sit 529 floats apart (an odd number of banks),
-// so the position-by-position loads and stores are conflict-free too.
+// a lane then start in distinct banks, and the lines sit 529 floats apart (an
+// odd number of banks), so both the row tiles' chunk accesses and the
+// position-by-position loads and stores meet at most two-way bank conflicts.
 #pragma once
 #include "lfx_across_types.hpp"
 #include "lfx_reg.hpp"

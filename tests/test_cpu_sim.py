@@ -348,7 +348,7 @@ def _across_sim(w, tf, x, L, storage=0):
 def test_across_lines_match_oracle(sim, oracle):
     """The across-line kernel (lfx_across.hpp) on the 64-lane CPU model with a
     model of its LDS slab (same padded layout), driven by the library's
-    planning: lines along a strided axis, blocks of 32 neighbouring lines
+    planning: lines along a strided axis, blocks of 16 neighbouring lines
     (partial last blocks), every pass of up to four levels in place in the
     slab, all transforms, bit for bit against the oracle; lines too short for
     the row tiles' passes are refused."""
