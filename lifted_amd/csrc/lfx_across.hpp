@@ -89,10 +89,11 @@ __device__ __forceinline__ void fetch(const Args& A, long long c0, long long oi,
     fetch_t<float>(A, c0, oi, tid, v);
 }
 // Position p0 + kPosStep j of line c lies at physical slab index
-// phys(c kMaxN + p0 + 16 j) = 529 c + p0 + 16 j + j / 2 (p0 < 16): one base
-// address per thread and compile-time offsets.
+// phys(c kMaxN + p0 + kPosStep j) = 529 c + p0 + kPosStep j + kPosStep j / 32
+// (p0 < kPosStep, which divides 32): one base address per thread and
+// compile-time offsets.
 LFX_HDI int slab_line(int c, int p0) { return c * (kMaxN + kMaxN / 32 + 1) + p0; }
-LFX_HDI int slab_off(int j) { return kPosStep * j + j / 2; }
+LFX_HDI int slab_off(int j) { return kPosStep * j + kPosStep * j / 32; }
 __device__ __forceinline__ void put(const Args& A, float* slab, int tid, const float (&v)[kPer]) {
   const int c = tid % kLines, p0 = tid / kLines, n = (int)A.n;
   float* b = slab + slab_line(c, p0);

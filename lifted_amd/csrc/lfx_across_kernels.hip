@@ -97,7 +97,8 @@ struct RowLds : reg::LaneGpuT<4> {
 
 // One workgroup per block of acr::kLines lines: the block's lines are
 // loaded (all of a thread's loads issued at once) into the slab, every level
-// of four lines per wavefront runs from the slab, and the block is stored.
+// of each group of four lines runs from the slab (acr::kGroups / acr::kWaves
+// groups per wavefront, one after the other), and the block is stored.
 // One kernel per table wavelet (its compiled step shapes), chosen on the host.
 template <int TF, class W>
 __global__ void __launch_bounds__(acr::kThreads) k_across(acr::Args A) {
@@ -115,7 +116,8 @@ __global__ void __launch_bounds__(acr::kThreads) k_across(acr::Args A) {
   // the wave's rows of the slab: a wave-uniform base (scalar registers), so
   // the policy's address and alignment arithmetic on it stays scalar
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  acr::run_lines<RowLds, W, TF>(A, lfx_acr_slab + wave * 4 * acr::kMaxN);
+  for (int g = wave; g < acr::kGroups; g += acr::kWaves)
+    acr::run_lines<RowLds, W, TF>(A, lfx_acr_slab + g * 4 * acr::kMaxN);
   __syncthreads();
   acr::store_block(A, lfx_acr_slab, c0, oo, tid);
 }

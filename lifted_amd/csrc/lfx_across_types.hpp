@@ -8,15 +8,17 @@
 namespace lfx {
 namespace acr {
 
-constexpr int kLines = 16;       // lines per workgroup
+constexpr int kLines = 32;       // lines per workgroup
 constexpr int kMinLines = 16;    // fewest lines along the contiguous dim worth a launch
 constexpr int kMaxN = 512;       // samples per line (row tiles: 16 lanes x 32)
-constexpr int kWaves = kLines / 4;
+constexpr int kGroups = kLines / 4;  // groups of four lines (one row tile each)
+constexpr int kWaves = 4;            // wavefronts per workgroup, kGroups / kWaves groups each
 constexpr int kThreads = 64 * kWaves;
 constexpr int kMaxPasses = 8;    // passes of at most four levels
 constexpr int kPer = kLines * kMaxN / kThreads;  // slab values per thread
 constexpr int kPosStep = kThreads / kLines;      // positions between a thread's values
-static_assert(kPosStep == 16 && kMaxN == 512, "the slab index of a thread's values (lfx_across.hpp) assumes these");
+static_assert(kGroups % kWaves == 0 && 32 % kPosStep == 0 && kMaxN == 512,
+              "the slab index of a thread's values (lfx_across.hpp) assumes these");
 LFX_HDI int phys(int a) { return a + (a >> 5) + (a >> 9); }
 constexpr int kSlabFloats = kLines * kMaxN + kLines * kMaxN / 32 + kLines * kMaxN / 512;
 

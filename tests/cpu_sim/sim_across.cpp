@@ -118,7 +118,7 @@ int main(int argc, char** argv) {
         acr::fetch(A, c0, oi, t, v);
         acr::put(A, g_slab, t, v);
       }
-      for (int wv = 0; wv < acr::kWaves; ++wv) {
+      for (int wv = 0; wv < acr::kGroups; ++wv) {
         float* vb = g_slab + wv * 4 * acr::kMaxN;
         const bool ok = tf == 0 ? run_wave<0>(wid, A, vb)
                         : tf == 1 ? run_wave<1>(wid, A, vb)
