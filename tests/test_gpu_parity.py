@@ -191,18 +191,22 @@ def test_config1_db2_65536_fp64_8_levels(gpu, oracle):
         assert np.abs(r - x).max() <= 1e-13 * np.abs(x).max()
 
 
+@pytest.mark.parametrize("strip", [0, 1024])
 @pytest.mark.parametrize("tf", [F, I, FA, IA])
-def test_long_lines_tiled(gpu, oracle, tf):
-    """Lines longer than one workgroup's LDS take the tiled multi-level passes."""
+def test_long_lines_tiled(gpu, oracle, tf, strip):
+    """Lines longer than one workgroup's LDS take the tiled multi-level passes
+    (float32: register windows of 2048 samples, or 1024 with the strip knob)."""
     import lifted_amd as la
+    from lifted_amd import tune
     g = rng(100 + tf)
     for (w, bc, n, L, dtype) in [(100907, 0, 200003, 9, np.float32), (100907, 3, 65536 * 3, 12, np.float32),
                                  (3, 4, 70001, 5, np.float64), (10208, 1, 40000, 7, np.float32),
                                  (10, 3, 1 << 18, 13, np.float32), (100503, 0, (1 << 20) + 7, 17, np.float32)]:
         x = g.standard_normal(n).astype(dtype)
         ref = oracle.lwt(x, w, tf, bc, level=[L])
-        got = la.lwt(_gpu(x, gpu), w, tf, mode=bc, level=[L]).cpu().numpy()
-        _check(got, ref, dtype, f"w={w} bc={bc} n={n} L={L} tf={tf}", axes=[0], level=[L])
+        with tune.tuning(strip=strip):
+            got = la.lwt(_gpu(x, gpu), w, tf, mode=bc, level=[L]).cpu().numpy()
+        _check(got, ref, dtype, f"w={w} bc={bc} n={n} L={L} tf={tf} strip={strip}", axes=[0], level=[L])
 
 
 @pytest.mark.parametrize("tf", [F, I, FA, IA])
