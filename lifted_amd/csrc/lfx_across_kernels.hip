@@ -18,7 +18,7 @@ extern __shared__ __align__(16) float lfx_acr_slab[];
 // handed to the tile code is a virtual slab address (slab + virtual index),
 // mapped through acr::phys on every access. The row tiles' chunk layout is
 // read and written in place (no staging transposes: the slab layout keeps
-// those accesses conflict-free).
+// those accesses at most two-way bank-conflicted).
 struct RowLds : reg::LaneGpuT<4> {
   using V = float;
   static __device__ __forceinline__ float& at(const float* p, int off) {
@@ -113,8 +113,8 @@ __global__ void __launch_bounds__(acr::kThreads) k_across(acr::Args A) {
     acr::put(A, lfx_acr_slab, tid, v);
   }
   __syncthreads();
-  // the wave's rows of the slab: a wave-uniform base (scalar registers), so
-  // the policy's address and alignment arithmetic on it stays scalar
+  // the wave's groups of rows of the slab: a wave-uniform base (scalar
+  // registers), so the policy's address and alignment arithmetic stays scalar
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   for (int g = wave; g < acr::kGroups; g += acr::kWaves)
     acr::run_lines<RowLds, W, TF>(A, lfx_acr_slab + g * 4 * acr::kMaxN);
