@@ -30,21 +30,33 @@ bool sim_reg_tf0(uint32_t, const DProgram<float>&, const ItemSpec&, const float*
 bool sim_reg_tf1(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*);
 bool sim_reg_tf2(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*);
 bool sim_reg_tf3(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*);
+bool sim_reg16_tf0(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*);
+bool sim_reg16_tf1(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*);
+bool sim_reg16_tf2(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*);
+bool sim_reg16_tf3(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*);
 bool sim_rows_tf0(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*, const reg::RowArgs&);
 bool sim_rows_tf1(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*, const reg::RowArgs&);
 bool sim_rows_tf2(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*, const reg::RowArgs&);
 bool sim_rows_tf3(uint32_t, const DProgram<float>&, const ItemSpec&, const float*, float*, float*, const reg::RowArgs&);
 }  // namespace lfx
 
+// register window of the simulated plans: LFX_SIM_STRIP=1024 for 16 samples
+// per lane (a subset of the table), else the default 2048
+static long long sim_window() {
+  const char* e = std::getenv("LFX_SIM_STRIP");
+  return e && std::atoll(e) == 1024 ? 1024 : kRegWindow;
+}
+
 template <class C>
 static void reg_dispatch(int tf, uint32_t wid, const DProgram<C>& prog, const ItemSpec& it, const C* in, C* out, C* sk) {
   if constexpr (sizeof(C) == 4) {
     bool ok = false;
+    const bool small = sim_window() == 1024;
     switch (tf) {
-      case 0: ok = sim_reg_tf0(wid, prog, it, in, out, sk); break;
-      case 1: ok = sim_reg_tf1(wid, prog, it, in, out, sk); break;
-      case 2: ok = sim_reg_tf2(wid, prog, it, in, out, sk); break;
-      case 3: ok = sim_reg_tf3(wid, prog, it, in, out, sk); break;
+      case 0: ok = small ? sim_reg16_tf0(wid, prog, it, in, out, sk) : sim_reg_tf0(wid, prog, it, in, out, sk); break;
+      case 1: ok = small ? sim_reg16_tf1(wid, prog, it, in, out, sk) : sim_reg_tf1(wid, prog, it, in, out, sk); break;
+      case 2: ok = small ? sim_reg16_tf2(wid, prog, it, in, out, sk) : sim_reg_tf2(wid, prog, it, in, out, sk); break;
+      case 3: ok = small ? sim_reg16_tf3(wid, prog, it, in, out, sk) : sim_reg_tf3(wid, prog, it, in, out, sk); break;
     }
     if (!ok) fail(LFX_EINVAL, "no register kernel for this wavelet");
   } else {
@@ -100,7 +112,7 @@ static void run(const WaveletDefL& w, bool use_reg, int tf, int bc, long long n,
       const PassPlan& pp = sc.passes[i];
       const bool last = (i + 1 == sc.passes.size()) && sc.L_line == 0;
       std::vector<C> sk(band_len(pp.m0, pp.k));
-      for (const ItemSpec& it : pass_items(prog, true, pp, use_reg, storage, bc)) {
+      for (const ItemSpec& it : pass_items(prog, true, pp, use_reg, storage, bc, sim_window())) {
         C* skp = last ? out.data() : sk.data();
         if (it.kind == kSpecReg && !std::getenv("LFX_SIM_REG_AS_LDS")) {
           std::fprintf(stderr, "  reg [%lld, %lld) H=%lld\n", it.a, it.b, it.H);
@@ -128,7 +140,7 @@ static void run(const WaveletDefL& w, bool use_reg, int tf, int bc, long long n,
     const PassPlan& pp = sc.passes[j];
     std::vector<C> band(j == 0 ? 0 : pp.m0);
     C* dst = j == 0 ? out.data() : band.data();
-    for (const ItemSpec& it : pass_items(prog, false, pp, use_reg, storage, bc)) {
+    for (const ItemSpec& it : pass_items(prog, false, pp, use_reg, storage, bc, sim_window())) {
       const C* sp = s_user ? in.data() : s.data();
       if (it.kind == kSpecReg && !std::getenv("LFX_SIM_REG_AS_LDS")) {
         std::fprintf(stderr, "  reg [%lld, %lld) Hk=%lld\n", it.a, it.b, it.H);

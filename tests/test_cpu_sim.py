@@ -46,13 +46,14 @@ def sim():
     return SIM
 
 
-def run_sim(sim, w, tf, bc, x, L, limit, tile, k, steps=None):
+def run_sim(sim, w, tf, bc, x, L, limit, tile, k, steps=None, env=None):
     dt = 1 if x.dtype == np.float64 else 0
     args = [sim, "custom" if steps else str(w), str(tf), str(bc), str(dt), str(len(x)), str(L), str(limit),
             str(tile), str(k)]
     if steps:
         args.append(steps)
-    r = subprocess.run(args, input=x.tobytes(), capture_output=True)
+    r = subprocess.run(args, input=x.tobytes(), capture_output=True,
+                       env=None if env is None else dict(os.environ, **env))
     assert r.returncode == 0, r.stderr.decode()[-3000:]
     return np.frombuffer(r.stdout, dtype=x.dtype), r.stderr.decode()
 
@@ -67,12 +68,13 @@ def ok_bands(bc, n, L):
     return True
 
 
-def _case(sim, oracle, w, tf, bc, x, L, limit, tile, k, msg, ref_level=None, steps=None, spec=None, check=None):
+def _case(sim, oracle, w, tf, bc, x, L, limit, tile, k, msg, ref_level=None, steps=None, spec=None, check=None,
+          env=None):
     """One simulator run against the oracle (bit-exact)."""
     def run():
         ref = oracle.lwt(x, w if steps is None else 0, tf, bc, level=[L] if ref_level is None else ref_level,
                          steps=steps)
-        got, info = run_sim(sim, w, tf, bc, x, L, limit=limit, tile=tile, k=k, steps=spec)
+        got, info = run_sim(sim, w, tf, bc, x, L, limit=limit, tile=tile, k=k, steps=spec, env=env)
         np.testing.assert_array_equal(got, ref, err_msg=f"{msg}\n{info}")
         if check is not None:
             check(info)
@@ -202,6 +204,25 @@ def test_register_tiles_zero_boundary_whole_band(sim, oracle, w):
             if n >= 1024:
                 assert "reg [0," in info, info
         cases.append(_case(sim, oracle, w, tf, 0, x, L, 64, 8192, 4, f"w={w} tf={tf} n={n}", check=whole))
+    run_cases(cases)
+
+
+@pytest.mark.parametrize("w", [1, 2, 10, 10301, 10608, 100503, 100907])
+def test_register_tiles_1024_sample_windows(sim, oracle, w):
+    """Register windows of 1024 samples (16 per lane, the strip knob's other
+    setting): interior tiles with LDS band ends (Symmetric) and masked windows
+    over the band ends (Zero), passes of one to four levels, all transforms."""
+    g = np.random.default_rng(w + 17)
+    cases = []
+    for bc, tf, n, L in itertools.product((0, 3), (0, 1, 2, 3), (20011, 4096), (3, 6)):
+        L = min(L, oracle.max_level(w, n))
+        x = g.standard_normal(n).astype(np.float32)
+
+        def has_reg(info):
+            assert "reg [" in info, info
+        cases.append(_case(sim, oracle, w, tf, bc, x, L, 600 if bc else 64, 8192, 4,
+                           f"w={w} bc={bc} tf={tf} n={n} L={L} strip=1024", check=has_reg,
+                           env={"LFX_SIM_STRIP": "1024"}))
     run_cases(cases)
 
 
