@@ -172,6 +172,26 @@ def test_half_storage_single_axis_matches_fp32_rounded(gpu, oracle, storage):
         np.testing.assert_array_equal(got, ref)
 
 
+@pytest.mark.parametrize("storage", ["bfloat16", "float16"])
+def test_half_storage_separable_3d_matches_fp32_rounded(gpu, oracle, storage):
+    """16-bit storage over every axis of a 3-D box (the strided axes of at
+    most 512 samples read in place by the across-line kernel into float32
+    workspaces, the contiguous one through a row plan): equals the fp32 oracle
+    on the 16-bit input, rounded once to the storage type at the end, with the
+    across-line kernel on and off."""
+    import torch
+    import lifted_amd as la
+    from lifted_amd import tune
+    tdt = getattr(torch, storage)
+    x = torch.from_numpy(rng(12).standard_normal((40, 48, 64)).astype(np.float32)).to(tdt)
+    x32 = x.float().cpu().numpy()
+    for tf, across in itertools.product((F, I, FA, IA), (1, 0)):
+        ref = torch.from_numpy(oracle.lwt(x32, 100907, tf, 0, axes=[0, 1, 2], level=[2, 3, 2])).to(tdt).float().numpy()
+        with tune.tuning(across=across):
+            got = la.lwt(x.to(gpu), 100907, tf, axes=[0, 1, 2], level=[2, 3, 2]).float().cpu().numpy()
+        np.testing.assert_array_equal(got, ref, err_msg=f"{storage} tf={tf} across={across}")
+
+
 def test_config1_db2_65536_fp64_8_levels(gpu, oracle):
     """1-D float64 signal of 65536 N(0,1) samples, Daubechies2, 8 levels,
     forward then inverse (the reference's CPU-runnable workload): coefficients
